@@ -1,0 +1,199 @@
+/*
+ * rwhip — C ABI of the MI355X (gfx950) octree-build backend.
+ *
+ * The hot path of the reference's tree app, as seven stages on device memory:
+ *   S1 Morton encode, S2 radix sort, S3 duplicate removal, S4 Karras radix
+ *   tree, S5 edge count, S6 exclusive prefix sum, S7 octree nodes.
+ * Every entry point takes plain device pointers and sizes, enqueues work on
+ * the given stream (NULL = the default stream) and returns a status code
+ * (RWH_OK = 0).  Counts that the reference reads back on the host between
+ * stages (n_unique after S3, n_octree_nodes after S6) stay in device memory
+ * here and are passed to later stages as device pointers, so a whole build is
+ * enqueued without a host round trip.
+ *
+ * Each stage entry point replaces one reference dispatcher; the C++ layer in
+ * redwood-comp_amd/host (redwood::hip::run_stage1..7 returning
+ * std::future<void>) is the drop-in for them and calls only this ABI.
+ *
+ * Thread safety: a workspace serves one stream at a time.  Different
+ * workspaces may run concurrently on different streams.
+ */
+#ifndef RWHIP_H
+#define RWHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define RWH_OK 0
+#define RWH_ERR_INVALID_ARG 1 /* bad pointer, size, alignment */
+#define RWH_ERR_HIP 2         /* a HIP runtime call failed (detail string) */
+#define RWH_ERR_CAPACITY 3    /* size above the workspace, or octree above capacity */
+#define RWH_ERR_NO_DEVICE 4   /* no usable GPU */
+#define RWH_ERR_INJECTED 5    /* failure injected by rwh_debug_fail_next */
+
+typedef void* rwh_stream_t; /* a hipStream_t */
+typedef void* rwh_event_t;  /* a hipEvent_t */
+typedef struct rwh_workspace rwh_workspace;
+
+/* Status text; the detail of the calling thread's last error. */
+const char* rwh_error_string(int status);
+const char* rwh_last_error(void);
+/* Library version (major*10000 + minor*100 + patch). */
+int rwh_version(void);
+
+/* ---- device, memory, streams, events (plumbing) ---------------------- */
+int rwh_device_count(int* count);
+int rwh_device_name(char* buf, size_t len);
+int rwh_malloc(void** ptr, size_t bytes);
+int rwh_free(void* ptr);
+int rwh_host_alloc(void** ptr, size_t bytes); /* pinned host memory */
+int rwh_host_free(void* ptr);
+int rwh_memcpy_h2d(void* dst, const void* src, size_t bytes, rwh_stream_t stream);
+int rwh_memcpy_d2h(void* dst, const void* src, size_t bytes, rwh_stream_t stream);
+int rwh_memcpy_d2d(void* dst, const void* src, size_t bytes, rwh_stream_t stream);
+int rwh_memset(void* dst, int value, size_t bytes, rwh_stream_t stream);
+int rwh_stream_create(rwh_stream_t* stream);
+int rwh_stream_destroy(rwh_stream_t stream);
+int rwh_stream_sync(rwh_stream_t stream);
+int rwh_device_sync(void);
+int rwh_event_create(rwh_event_t* event);
+int rwh_event_destroy(rwh_event_t event);
+int rwh_event_record(rwh_event_t event, rwh_stream_t stream);
+int rwh_event_sync(rwh_event_t event);
+int rwh_event_query(rwh_event_t event); /* RWH_OK when complete, 1 while pending */
+int rwh_event_elapsed_ms(float* ms, rwh_event_t start, rwh_event_t stop);
+
+/* ---- workspace (the reference's cuda::ImStorage, im_storage.cu:29-40) ---
+ * Temporary device storage for the sort / unique / scan look-back state and
+ * the device-resident counts of up to max_segments clouds.  max_points is the
+ * largest index space a call will use (a single cloud: its point count). */
+int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspace** ws);
+int rwh_workspace_destroy(rwh_workspace* ws);
+size_t rwh_workspace_bytes(const rwh_workspace* ws);
+/* Device array of per-cloud counts written by S3 and S6: n_unique of cloud s
+ * at [s], n_octree_nodes at [max_segments + s] (a single build is cloud 0). */
+uint32_t* rwh_workspace_counts(rwh_workspace* ws);
+uint32_t rwh_workspace_max_segments(const rwh_workspace* ws);
+
+/* ---- S1  (reference: cpu::run_stage1 host_dispatchers.cpp:17-41,
+ *          cuda::run_stage1 cu_dispatcher.cu:501-522)
+ * codes[i] = Morton code of float4 point i (x,y,z used), 10 bits per axis.
+ * d_xyzw: 16-byte aligned. */
+int rwh_morton(const float* d_xyzw, uint32_t n, float min_coord, float range, uint32_t* d_codes,
+               rwh_stream_t stream);
+
+/* ---- S2  (reference: cpu::run_stage2 host_dispatchers.cpp:135-177,
+ *          cuda::run_stage2 cu_dispatcher.cu:528-591)
+ * Ascending sort of n keys: d_in is left unchanged, d_out gets the result.
+ * key_bits: significant bits of every key (30 for Morton codes, 32 for any
+ * u32); it sets the number of 8-bit passes.  16-byte aligned buffers. */
+int rwh_sort_u32(rwh_workspace* ws, const uint32_t* d_in, uint32_t* d_out, uint32_t n,
+                 uint32_t key_bits, rwh_stream_t stream);
+
+/* ---- S3  (reference: cpu::run_stage3 host_dispatchers.cpp:183-194,
+ *          cuda::run_stage3 cu_dispatcher.cu:597-627)
+ * First key of every run of equal keys, in order; *d_n_unique (device) gets
+ * the count. */
+int rwh_unique_u32(rwh_workspace* ws, const uint32_t* d_sorted, uint32_t n, uint32_t* d_unique,
+                   uint32_t* d_n_unique, rwh_stream_t stream);
+
+/* ---- S4  (reference: cpu::run_stage4 host_dispatchers.cpp:200-228,
+ *          cuda::run_stage4 cu_dispatcher.cu:633-648)
+ * Radix-tree nodes i in [0, n_unique-1) over the unique codes; n_unique is
+ * read from device memory; n_max bounds it (buffers hold n_max entries). */
+int rwh_radix_tree(const uint32_t* d_codes, const uint32_t* d_n_unique, uint32_t n_max,
+                   uint8_t* d_prefix_n, uint8_t* d_has_leaf_left, uint8_t* d_has_leaf_right,
+                   int32_t* d_left_child, int32_t* d_parents, rwh_stream_t stream);
+
+/* ---- S5  (reference: cpu::run_stage5 host_dispatchers.cpp:234-257,
+ *          cuda::run_stage5 cu_dispatcher.cu:654-666) */
+int rwh_edge_count(const uint8_t* d_prefix_n, const int32_t* d_parents,
+                   const uint32_t* d_n_unique, uint32_t n_max, int32_t* d_edge_count,
+                   rwh_stream_t stream);
+
+/* ---- S6  (reference: cpu::run_stage6 host_dispatchers.cpp:263-297,
+ *          cuda::run_stage6 cu_dispatcher.cu:672-686)
+ * Exclusive scan of the n_unique-1 edge counts; *d_n_octree_nodes (device)
+ * gets their sum.  16-byte aligned buffers. */
+int rwh_prefix_sum(rwh_workspace* ws, const int32_t* d_edge_count, const uint32_t* d_n_unique,
+                   uint32_t n_max, int32_t* d_edge_offset, uint32_t* d_n_octree_nodes,
+                   rwh_stream_t stream);
+
+/* Device buffers of one build, laid out as the reference's AppData
+ * (apps/tree/app_data.hpp:48-96).  oct_capacity = floor(0.6 * n) there. */
+typedef struct rwh_tree_buffers {
+  const float* points;        /* n x (x,y,z,w) */
+  uint32_t* morton_keys;      /* n: S1 codes, then S3 unique keys */
+  uint32_t* morton_keys_alt;  /* n: S2 sorted keys */
+  uint8_t* prefix_n;          /* n */
+  uint8_t* has_leaf_left;     /* n */
+  uint8_t* has_leaf_right;    /* n */
+  int32_t* left_child;        /* n */
+  int32_t* parents;           /* n */
+  int32_t* edge_count;        /* n */
+  int32_t* edge_offset;       /* n */
+  int32_t* children;          /* 8 x oct_capacity */
+  float* corner;              /* 4 x oct_capacity */
+  float* cell_size;           /* oct_capacity */
+  int32_t* child_node_mask;   /* oct_capacity */
+  int32_t* child_leaf_mask;   /* oct_capacity */
+  uint32_t oct_capacity;
+} rwh_tree_buffers;
+
+/* ---- S7  (reference: cpu::run_stage7 host_dispatchers.cpp:303-336,
+ *          cuda::run_stage7 cu_dispatcher.cu:692-752)
+ * Octree nodes for brt nodes [1, n_unique-1); rows [0, min(n_oct, cap)) of
+ * children / masks are rewritten, corner / cell_size of those slots are set. */
+int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
+               const uint32_t* d_n_octree_nodes, uint32_t n_max, float min_coord, float range,
+               rwh_stream_t stream);
+
+/* Leaf linking: the reference's k_LinkLeafNodes (07_octree.cu:228-258), which
+ * it never launches.  Opt-in; runs after rwh_octree. */
+int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
+                  const uint32_t* d_n_octree_nodes, uint32_t n_max, rwh_stream_t stream);
+
+/* ---- whole build (the reference's stage loop, pipe_main_baseline.cpp:23-31)
+ * Enqueues S1..S7 for n points on the stream; the counts of cloud 0 go to
+ * rwh_workspace_counts(ws). */
+int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
+                     float range, rwh_stream_t stream);
+/* Waits for the stream and copies {n_unique, n_brt, n_octree_nodes} of cloud
+ * 0 to host; RWH_ERR_CAPACITY if n_octree_nodes exceeds the capacity given. */
+int rwh_read_counts(rwh_workspace* ws, uint32_t oct_capacity, uint32_t* counts3,
+                    rwh_stream_t stream);
+
+/* ---- batched build: many clouds in one pass ----------------------------
+ * Cloud s has n[s] points.  rwh_batch_layout fills, for every cloud, the
+ * first index in the shared point index space (point_base, aligned to 8192)
+ * and in the octree slot space (oct_base, capacity oct_cap = floor(0.6 n));
+ * *index_space and *oct_space are the buffer sizes to allocate. */
+int rwh_batch_layout(const uint32_t* n, uint32_t n_clouds, uint32_t* point_base, uint32_t* oct_base,
+                     uint32_t* oct_cap, uint32_t* index_space, uint32_t* oct_space);
+/* Enqueues S1..S7 for all clouds at once (segmented kernels).  Points of cloud
+ * s sit at bufs->points[4*point_base[s]..]; outputs use the same bases.  Host
+ * arrays n, point_base, oct_base, oct_cap as from rwh_batch_layout.  Counts
+ * go to rwh_workspace_counts(ws) per cloud. */
+int rwh_build_octree_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, const uint32_t* n,
+                           const uint32_t* point_base, const uint32_t* oct_base,
+                           const uint32_t* oct_cap, uint32_t n_clouds, float min_coord,
+                           float range, rwh_stream_t stream);
+/* Waits and copies the counts of every cloud: n_unique to counts[s],
+ * n_octree_nodes to counts[n_clouds + s]. */
+int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts,
+                          rwh_stream_t stream);
+
+/* ---- testing hook: the next call of stage k (1..7; 0 = any) fails with
+ * RWH_ERR_INJECTED before enqueuing anything.  Thread-local, one shot. */
+void rwh_debug_fail_next(int stage);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,665 @@
+// The C ABI of the octree-build backend (include/rwhip.h): argument checks,
+// workspace management, HIP plumbing, and the stage / build entry points that
+// enqueue the kernels of k_sort.hip, k_stages.hip and k_octree.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rw_internal.hpp"
+#include "rwhip.h"
+
+using namespace rw;
+
+namespace {
+
+thread_local std::string g_detail;
+thread_local int g_fail_stage = -1;
+
+int fail(int status, const std::string& msg) {
+  g_detail = msg;
+  return status;
+}
+
+#define RWH_HIP(call)                                                                   \
+  do {                                                                                  \
+    const hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                               \
+      return fail(e_ == hipErrorNoDevice ? RWH_ERR_NO_DEVICE : RWH_ERR_HIP,             \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+#define RWH_CHECK(cond, msg) \
+  do {                       \
+    if (!(cond)) return fail(RWH_ERR_INVALID_ARG, msg); \
+  } while (0)
+
+#define RWH_TRY(expr)             \
+  do {                            \
+    const int s_ = (expr);        \
+    if (s_ != RWH_OK) return s_;  \
+  } while (0)
+
+int injected(int stage) {
+  if (g_fail_stage < 0) return 0;
+  if (g_fail_stage == 0 || g_fail_stage == stage) {
+    g_fail_stage = -1;
+    g_detail = "failure injected at stage " + std::to_string(stage);
+    return 1;
+  }
+  return 0;
+}
+
+int launched() {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(RWH_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return RWH_OK;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+constexpr uint32_t kMaxPoints = 1u << 30;
+
+SegArgs single_seg(uint32_t n) {
+  SegArgs s{};
+  s.n_segs = 1;
+  s.single_count = n;
+  return s;
+}
+
+SegArgs single_seg_dev(const uint32_t* d_count) {
+  SegArgs s{};
+  s.n_segs = 1;
+  s.count = d_count;
+  return s;
+}
+
+}  // namespace
+
+struct rwh_workspace {
+  uint32_t max_points = 0;  // index-space capacity
+  uint32_t max_segs = 0;
+  uint32_t max_chunks = 0;
+  uint32_t max_sort_tiles = 0;
+  uint32_t max_scan_tiles = 0;
+  size_t bytes = 0;
+  void* block = nullptr;  // one device allocation for everything below
+  uint32_t* keys_tmp = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t* digit_start = nullptr;
+  uint64_t* lb_sort = nullptr;
+  uint64_t* lb_unique = nullptr;
+  uint64_t* lb_scan = nullptr;
+  uint32_t* ctrl = nullptr;    // 3 users x kCtrlWords
+  uint32_t* counts = nullptr;  // n_unique[max_segs], n_oct[max_segs]
+  uint32_t* tables = nullptr;  // seg_base, seg_n, oct_base, oct_cap [max_segs each], chunk_seg
+  uint32_t* h_stage = nullptr;  // pinned staging of tables
+  uint32_t* h_counts = nullptr;  // pinned counts read-back
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
+
+  uint32_t* seg_base() { return tables; }
+  uint32_t* seg_n() { return tables + max_segs; }
+  uint32_t* oct_base() { return tables + 2 * max_segs; }
+  uint32_t* oct_cap() { return tables + 3 * max_segs; }
+  uint32_t* chunk_seg() { return tables + 4 * max_segs; }
+  size_t table_words() const { return 4ull * max_segs + max_chunks; }
+  uint32_t* n_unique() { return counts; }
+  uint32_t* n_oct() { return counts + max_segs; }
+};
+
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs seg, uint32_t n_segs,
+                 uint32_t n_index, uint32_t key_bits, bool plain, hipStream_t st) {
+  SortLaunch L{};
+  L.keys_in = in;
+  L.keys_out = out;
+  L.keys_tmp = ws->keys_tmp;
+  L.seg = seg;
+  L.n_segs = n_segs;
+  L.n_tiles = (n_index + kSortTile - 1) / kSortTile;
+  if (plain) {
+    L.hist_unit_keys = kPlainHistUnit;
+  } else {
+    L.hist_unit_keys = kChunk;
+  }
+  L.n_hist_units = (n_index + L.hist_unit_keys - 1) / L.hist_unit_keys;
+  L.key_bits = key_bits;
+  L.hist = ws->hist;
+  L.digit_start = ws->digit_start;
+  L.lookback = ws->lb_sort;
+  L.ctrl = ws->ctrl + 0 * kCtrlWords;
+  if (L.n_tiles == 0) return RWH_OK;
+  launch_sort(L, st);
+  return launched();
+}
+
+int check_ws(const rwh_workspace* ws, uint32_t n) {
+  if (!ws) return fail(RWH_ERR_INVALID_ARG, "null workspace");
+  if (n > ws->max_points)
+    return fail(RWH_ERR_CAPACITY, "n = " + std::to_string(n) + " exceeds the workspace capacity " +
+                                      std::to_string(ws->max_points));
+  return RWH_OK;
+}
+
+OctArgs oct_args(const rwh_tree_buffers* b, float min_coord, float range) {
+  OctArgs a{};
+  a.edge_offset = b->edge_offset;
+  a.edge_count = b->edge_count;
+  a.codes = b->morton_keys;
+  a.prefix_n = b->prefix_n;
+  a.parent = b->parents;
+  a.has_leaf_left = b->has_leaf_left;
+  a.has_leaf_right = b->has_leaf_right;
+  a.left_child = b->left_child;
+  a.single_cap = b->oct_capacity;
+  a.min_coord = min_coord;
+  a.range = range;
+  a.children = b->children;
+  a.corner = reinterpret_cast<float4*>(b->corner);
+  a.cell_size = b->cell_size;
+  a.child_node_mask = b->child_node_mask;
+  a.child_leaf_mask = b->child_leaf_mask;
+  return a;
+}
+
+int check_bufs(const rwh_tree_buffers* b) {
+  RWH_CHECK(b, "null buffers");
+  RWH_CHECK(b->points && b->morton_keys && b->morton_keys_alt && b->prefix_n && b->has_leaf_left &&
+                b->has_leaf_right && b->left_child && b->parents && b->edge_count &&
+                b->edge_offset && b->children && b->corner && b->cell_size &&
+                b->child_node_mask && b->child_leaf_mask,
+            "a tree buffer is null");
+  RWH_CHECK(aligned16(b->points) && aligned16(b->morton_keys) && aligned16(b->morton_keys_alt) &&
+                aligned16(b->edge_count) && aligned16(b->edge_offset) && aligned16(b->children) &&
+                aligned16(b->corner),
+            "tree buffers must be 16-byte aligned");
+  return RWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rwh_error_string(int status) {
+  switch (status) {
+    case RWH_OK: return "ok";
+    case RWH_ERR_INVALID_ARG: return "invalid argument";
+    case RWH_ERR_HIP: return "HIP runtime error";
+    case RWH_ERR_CAPACITY: return "capacity exceeded";
+    case RWH_ERR_NO_DEVICE: return "no GPU device";
+    case RWH_ERR_INJECTED: return "injected failure";
+    default: return "unknown status";
+  }
+}
+
+const char* rwh_last_error(void) { return g_detail.c_str(); }
+int rwh_version(void) { return 10000; }
+void rwh_debug_fail_next(int stage) { g_fail_stage = stage; }
+
+// ---- plumbing ---------------------------------------------------------------
+int rwh_device_count(int* count) {
+  RWH_CHECK(count, "null count");
+  *count = 0;
+  const hipError_t e = hipGetDeviceCount(count);
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) {
+    *count = 0;
+    return RWH_OK;
+  }
+  RWH_HIP(e);
+  return RWH_OK;
+}
+
+int rwh_device_name(char* buf, size_t len) {
+  RWH_CHECK(buf && len > 0, "bad buffer");
+  int dev = 0;
+  RWH_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  RWH_HIP(hipGetDeviceProperties(&prop, dev));
+  std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return RWH_OK;
+}
+
+int rwh_malloc(void** ptr, size_t bytes) {
+  RWH_CHECK(ptr, "null out pointer");
+  *ptr = nullptr;
+  if (bytes == 0) return RWH_OK;
+  RWH_HIP(hipMalloc(ptr, bytes));
+  return RWH_OK;
+}
+int rwh_free(void* ptr) {
+  if (ptr) RWH_HIP(hipFree(ptr));
+  return RWH_OK;
+}
+int rwh_host_alloc(void** ptr, size_t bytes) {
+  RWH_CHECK(ptr, "null out pointer");
+  *ptr = nullptr;
+  if (bytes == 0) return RWH_OK;
+  RWH_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  return RWH_OK;
+}
+int rwh_host_free(void* ptr) {
+  if (ptr) RWH_HIP(hipHostFree(ptr));
+  return RWH_OK;
+}
+int rwh_memcpy_h2d(void* dst, const void* src, size_t bytes, rwh_stream_t stream) {
+  if (bytes == 0) return RWH_OK;
+  RWH_CHECK(dst && src, "null pointer");
+  RWH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_memcpy_d2h(void* dst, const void* src, size_t bytes, rwh_stream_t stream) {
+  if (bytes == 0) return RWH_OK;
+  RWH_CHECK(dst && src, "null pointer");
+  RWH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_memcpy_d2d(void* dst, const void* src, size_t bytes, rwh_stream_t stream) {
+  if (bytes == 0) return RWH_OK;
+  RWH_CHECK(dst && src, "null pointer");
+  RWH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_memset(void* dst, int value, size_t bytes, rwh_stream_t stream) {
+  if (bytes == 0) return RWH_OK;
+  RWH_CHECK(dst, "null pointer");
+  RWH_HIP(hipMemsetAsync(dst, value, bytes, static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_stream_create(rwh_stream_t* stream) {
+  RWH_CHECK(stream, "null out pointer");
+  hipStream_t s = nullptr;
+  RWH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return RWH_OK;
+}
+int rwh_stream_destroy(rwh_stream_t stream) {
+  if (stream) RWH_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_stream_sync(rwh_stream_t stream) {
+  RWH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_device_sync(void) {
+  RWH_HIP(hipDeviceSynchronize());
+  return RWH_OK;
+}
+int rwh_event_create(rwh_event_t* event) {
+  RWH_CHECK(event, "null out pointer");
+  hipEvent_t e = nullptr;
+  RWH_HIP(hipEventCreate(&e));
+  *event = e;
+  return RWH_OK;
+}
+int rwh_event_destroy(rwh_event_t event) {
+  if (event) RWH_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return RWH_OK;
+}
+int rwh_event_record(rwh_event_t event, rwh_stream_t stream) {
+  RWH_CHECK(event, "null event");
+  RWH_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+int rwh_event_sync(rwh_event_t event) {
+  RWH_CHECK(event, "null event");
+  RWH_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+  return RWH_OK;
+}
+int rwh_event_query(rwh_event_t event) {
+  RWH_CHECK(event, "null event");
+  const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+  if (e == hipErrorNotReady) return 1;
+  RWH_HIP(e);
+  return RWH_OK;
+}
+int rwh_event_elapsed_ms(float* ms, rwh_event_t start, rwh_event_t stop) {
+  RWH_CHECK(ms && start && stop, "null argument");
+  RWH_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+  return RWH_OK;
+}
+
+// ---- workspace ---------------------------------------------------------------
+int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspace** out) {
+  RWH_CHECK(out, "null out pointer");
+  *out = nullptr;
+  RWH_CHECK(max_points <= kMaxPoints, "max_points above 2^30");
+  RWH_CHECK(max_segments >= 1 && max_segments <= (1u << 20), "max_segments out of range");
+  auto* ws = new rwh_workspace();
+  ws->max_points = max_points;
+  ws->max_segs = max_segments;
+  const uint32_t mp = std::max<uint32_t>(max_points, 1);
+  ws->max_chunks = (mp + kChunk - 1) / kChunk;
+  ws->max_sort_tiles = (mp + kSortTile - 1) / kSortTile;
+  ws->max_scan_tiles = (mp + kScanTile - 1) / kScanTile;
+  // carve one allocation
+  size_t off = 0;
+  const auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const size_t o_tmp = take(sizeof(uint32_t) * static_cast<size_t>(mp));
+  const size_t o_hist = take(sizeof(uint32_t) * static_cast<size_t>(max_segments) * kHistCopies * kMaxPasses * 256);
+  const size_t o_ds = take(sizeof(uint32_t) * static_cast<size_t>(max_segments) * kMaxPasses * 256);
+  const size_t o_lbs = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_sort_tiles) * 256);
+  const size_t o_lbu = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_scan_tiles));
+  const size_t o_lbc = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_scan_tiles));
+  const size_t o_ctrl = take(sizeof(uint32_t) * 3 * kCtrlWords);
+  const size_t o_cnt = take(sizeof(uint32_t) * 2ull * max_segments);
+  const size_t o_tab = take(sizeof(uint32_t) * ws->table_words());
+  ws->bytes = off;
+  char* base = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), off);
+  if (e != hipSuccess) {
+    delete ws;
+    return fail(e == hipErrorNoDevice ? RWH_ERR_NO_DEVICE : RWH_ERR_HIP,
+                std::string("hipMalloc(workspace): ") + hipGetErrorString(e));
+  }
+  ws->block = base;
+  ws->keys_tmp = reinterpret_cast<uint32_t*>(base + o_tmp);
+  ws->hist = reinterpret_cast<uint32_t*>(base + o_hist);
+  ws->digit_start = reinterpret_cast<uint32_t*>(base + o_ds);
+  ws->lb_sort = reinterpret_cast<uint64_t*>(base + o_lbs);
+  ws->lb_unique = reinterpret_cast<uint64_t*>(base + o_lbu);
+  ws->lb_scan = reinterpret_cast<uint64_t*>(base + o_lbc);
+  ws->ctrl = reinterpret_cast<uint32_t*>(base + o_ctrl);
+  ws->counts = reinterpret_cast<uint32_t*>(base + o_cnt);
+  ws->tables = reinterpret_cast<uint32_t*>(base + o_tab);
+  // Look-back words, histograms and counters must start at zero; later
+  // launches keep them consistent by themselves.
+  e = hipMemset(base, 0, off);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ws->h_stage),
+                                         sizeof(uint32_t) * ws->table_words(), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ws->h_counts),
+                                         sizeof(uint32_t) * 2ull * max_segments, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ws->upload_done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    rwh_workspace_destroy(ws);
+    return fail(RWH_ERR_HIP, std::string("workspace setup: ") + hipGetErrorString(e));
+  }
+  *out = ws;
+  return RWH_OK;
+}
+
+int rwh_workspace_destroy(rwh_workspace* ws) {
+  if (!ws) return RWH_OK;
+  if (ws->upload_done) (void)hipEventDestroy(ws->upload_done);
+  if (ws->h_stage) (void)hipHostFree(ws->h_stage);
+  if (ws->h_counts) (void)hipHostFree(ws->h_counts);
+  const hipError_t e = ws->block ? hipFree(ws->block) : hipSuccess;
+  delete ws;
+  RWH_HIP(e);
+  return RWH_OK;
+}
+
+size_t rwh_workspace_bytes(const rwh_workspace* ws) { return ws ? ws->bytes : 0; }
+uint32_t* rwh_workspace_counts(rwh_workspace* ws) { return ws ? ws->counts : nullptr; }
+uint32_t rwh_workspace_max_segments(const rwh_workspace* ws) { return ws ? ws->max_segs : 0; }
+
+// ---- stages -------------------------------------------------------------------
+int rwh_morton(const float* d_xyzw, uint32_t n, float min_coord, float range, uint32_t* d_codes,
+               rwh_stream_t stream) {
+  if (injected(1)) return RWH_ERR_INJECTED;
+  RWH_CHECK(n <= kMaxPoints, "n above 2^30");
+  if (n == 0) return RWH_OK;
+  RWH_CHECK(d_xyzw && d_codes, "null buffer");
+  RWH_CHECK(aligned16(d_xyzw), "points must be 16-byte aligned");
+  launch_morton(reinterpret_cast<const float4*>(d_xyzw), d_codes, single_seg(n), n, min_coord,
+                range, static_cast<hipStream_t>(stream));
+  return launched();
+}
+
+int rwh_sort_u32(rwh_workspace* ws, const uint32_t* d_in, uint32_t* d_out, uint32_t n,
+                 uint32_t key_bits, rwh_stream_t stream) {
+  if (injected(2)) return RWH_ERR_INJECTED;
+  RWH_TRY(check_ws(ws, n));
+  RWH_CHECK(key_bits >= 1 && key_bits <= 32, "key_bits must be in [1, 32]");
+  if (n == 0) return RWH_OK;
+  RWH_CHECK(d_in && d_out, "null buffer");
+  RWH_CHECK(aligned16(d_in) && aligned16(d_out), "keys must be 16-byte aligned");
+  RWH_CHECK(d_in != d_out, "in-place sort is not supported");
+  return enqueue_sort(ws, d_in, d_out, single_seg(n), 1, n, key_bits, true,
+                      static_cast<hipStream_t>(stream));
+}
+
+int rwh_unique_u32(rwh_workspace* ws, const uint32_t* d_sorted, uint32_t n, uint32_t* d_unique,
+                   uint32_t* d_n_unique, rwh_stream_t stream) {
+  if (injected(3)) return RWH_ERR_INJECTED;
+  RWH_TRY(check_ws(ws, n));
+  RWH_CHECK(d_n_unique, "null count pointer");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (n == 0) {
+    RWH_HIP(hipMemsetAsync(d_n_unique, 0, sizeof(uint32_t), st));
+    return RWH_OK;
+  }
+  RWH_CHECK(d_sorted && d_unique, "null buffer");
+  RWH_CHECK(aligned16(d_sorted), "keys must be 16-byte aligned");
+  launch_unique(d_sorted, d_unique, single_seg(n), n, d_n_unique, ws->lb_unique,
+                ws->ctrl + 1 * kCtrlWords, st);
+  return launched();
+}
+
+int rwh_radix_tree(const uint32_t* d_codes, const uint32_t* d_n_unique, uint32_t n_max,
+                   uint8_t* d_prefix_n, uint8_t* d_has_leaf_left, uint8_t* d_has_leaf_right,
+                   int32_t* d_left_child, int32_t* d_parents, rwh_stream_t stream) {
+  if (injected(4)) return RWH_ERR_INJECTED;
+  RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
+  if (n_max == 0) return RWH_OK;
+  RWH_CHECK(d_codes && d_n_unique && d_prefix_n && d_has_leaf_left && d_has_leaf_right &&
+                d_left_child && d_parents,
+            "null buffer");
+  TreeArgs a{d_codes, d_prefix_n, d_has_leaf_left, d_has_leaf_right, d_left_child, d_parents};
+  launch_radix_tree(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
+  return launched();
+}
+
+int rwh_edge_count(const uint8_t* d_prefix_n, const int32_t* d_parents, const uint32_t* d_n_unique,
+                   uint32_t n_max, int32_t* d_edge_count, rwh_stream_t stream) {
+  if (injected(5)) return RWH_ERR_INJECTED;
+  RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
+  if (n_max == 0) return RWH_OK;
+  RWH_CHECK(d_prefix_n && d_parents && d_n_unique && d_edge_count, "null buffer");
+  launch_edge_count(d_prefix_n, d_parents, d_edge_count, single_seg_dev(d_n_unique), n_max,
+                    static_cast<hipStream_t>(stream));
+  return launched();
+}
+
+int rwh_prefix_sum(rwh_workspace* ws, const int32_t* d_edge_count, const uint32_t* d_n_unique,
+                   uint32_t n_max, int32_t* d_edge_offset, uint32_t* d_n_octree_nodes,
+                   rwh_stream_t stream) {
+  if (injected(6)) return RWH_ERR_INJECTED;
+  RWH_TRY(check_ws(ws, n_max));
+  RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (n_max == 0) {
+    RWH_HIP(hipMemsetAsync(d_n_octree_nodes, 0, sizeof(uint32_t), st));
+    return RWH_OK;
+  }
+  RWH_CHECK(d_edge_count && d_edge_offset, "null buffer");
+  RWH_CHECK(aligned16(d_edge_count) && aligned16(d_edge_offset), "buffers must be 16-byte aligned");
+  launch_prefix_sum(d_edge_count, d_edge_offset, single_seg_dev(d_n_unique), n_max,
+                    d_n_octree_nodes, ws->lb_scan, ws->ctrl + 2 * kCtrlWords, st);
+  return launched();
+}
+
+int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
+               const uint32_t* d_n_octree_nodes, uint32_t n_max, float min_coord, float range,
+               rwh_stream_t stream) {
+  if (injected(7)) return RWH_ERR_INJECTED;
+  RWH_TRY(check_bufs(bufs));
+  RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
+  RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
+  if (n_max == 0) return RWH_OK;
+  OctArgs a = oct_args(bufs, min_coord, range);
+  a.n_oct = d_n_octree_nodes;
+  launch_octree(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
+  return launched();
+}
+
+int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
+                  const uint32_t* d_n_octree_nodes, uint32_t n_max, rwh_stream_t stream) {
+  RWH_TRY(check_bufs(bufs));
+  RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
+  RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
+  if (n_max == 0) return RWH_OK;
+  OctArgs a = oct_args(bufs, 0.0f, 1.0f);
+  a.n_oct = d_n_octree_nodes;
+  launch_link_leaf(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
+  return launched();
+}
+
+int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
+                     float range, rwh_stream_t stream) {
+  RWH_TRY(check_ws(ws, n));
+  RWH_TRY(check_bufs(bufs));
+  uint32_t* d_nu = ws->n_unique();
+  uint32_t* d_no = ws->n_oct();
+  RWH_TRY(rwh_morton(bufs->points, n, min_coord, range, bufs->morton_keys, stream));
+  RWH_TRY(rwh_sort_u32(ws, bufs->morton_keys, bufs->morton_keys_alt, n, 30, stream));
+  RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, stream));
+  RWH_TRY(rwh_radix_tree(bufs->morton_keys, d_nu, n, bufs->prefix_n, bufs->has_leaf_left,
+                         bufs->has_leaf_right, bufs->left_child, bufs->parents, stream));
+  RWH_TRY(rwh_edge_count(bufs->prefix_n, bufs->parents, d_nu, n, bufs->edge_count, stream));
+  RWH_TRY(rwh_prefix_sum(ws, bufs->edge_count, d_nu, n, bufs->edge_offset, d_no, stream));
+  RWH_TRY(rwh_octree(bufs, d_nu, d_no, n, min_coord, range, stream));
+  return RWH_OK;
+}
+
+int rwh_read_counts(rwh_workspace* ws, uint32_t oct_capacity, uint32_t* counts3,
+                    rwh_stream_t stream) {
+  RWH_CHECK(ws && counts3, "null argument");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  RWH_HIP(hipMemcpyAsync(ws->h_counts, ws->n_unique(), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipMemcpyAsync(ws->h_counts + 1, ws->n_oct(), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipStreamSynchronize(st));
+  counts3[0] = ws->h_counts[0];
+  counts3[1] = ws->h_counts[0] > 0 ? ws->h_counts[0] - 1 : 0;
+  counts3[2] = ws->h_counts[1];
+  if (counts3[2] > oct_capacity)
+    return fail(RWH_ERR_CAPACITY, "n_octree_nodes = " + std::to_string(counts3[2]) +
+                                      " exceeds the octree capacity " + std::to_string(oct_capacity));
+  return RWH_OK;
+}
+
+// ---- batched build ------------------------------------------------------------
+int rwh_batch_layout(const uint32_t* n, uint32_t n_clouds, uint32_t* point_base, uint32_t* oct_base,
+                     uint32_t* oct_cap, uint32_t* index_space, uint32_t* oct_space) {
+  RWH_CHECK(n && point_base && oct_base && oct_cap && index_space && oct_space, "null argument");
+  uint64_t p = 0, o = 0;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    RWH_CHECK(n[s] <= kMaxPoints, "cloud above 2^30 points");
+    point_base[s] = static_cast<uint32_t>(p);
+    const uint64_t chunks = std::max<uint64_t>(1, (n[s] + kChunk - 1) / kChunk);
+    p += chunks * kChunk;
+    // the reference sizes the octree at n * 0.6f (app_data.hpp:13, 28)
+    const uint32_t cap = static_cast<uint32_t>(static_cast<float>(n[s]) * 0.6f);
+    oct_base[s] = static_cast<uint32_t>(o);
+    oct_cap[s] = cap;
+    o += align_up(cap, 64);
+    RWH_CHECK(p <= kMaxPoints && o <= kMaxPoints, "batch index space above 2^30");
+  }
+  *index_space = static_cast<uint32_t>(p);
+  *oct_space = static_cast<uint32_t>(o);
+  return RWH_OK;
+}
+
+int rwh_build_octree_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, const uint32_t* n,
+                           const uint32_t* point_base, const uint32_t* oct_base,
+                           const uint32_t* oct_cap, uint32_t n_clouds, float min_coord,
+                           float range, rwh_stream_t stream) {
+  RWH_CHECK(ws, "null workspace");
+  RWH_TRY(check_bufs(bufs));
+  RWH_CHECK(n && point_base && oct_base && oct_cap, "null layout array");
+  RWH_CHECK(n_clouds >= 1, "empty batch");
+  if (n_clouds > ws->max_segs)
+    return fail(RWH_ERR_CAPACITY, "batch of " + std::to_string(n_clouds) +
+                                      " clouds exceeds the workspace's " + std::to_string(ws->max_segs));
+  uint64_t space = 0;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    RWH_CHECK(point_base[s] % kChunk == 0, "point_base must be a multiple of 8192");
+    RWH_CHECK(s == 0 || point_base[s] > point_base[s - 1], "point_base must increase");
+    const uint64_t chunks = std::max<uint64_t>(1, (n[s] + kChunk - 1) / kChunk);
+    space = std::max<uint64_t>(space, point_base[s] + chunks * kChunk);
+  }
+  if (space > ws->max_points)
+    return fail(RWH_ERR_CAPACITY, "batch index space " + std::to_string(space) +
+                                      " exceeds the workspace capacity " + std::to_string(ws->max_points));
+  const uint32_t n_index = static_cast<uint32_t>(space);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+
+  // Segment tables through the pinned staging area (reused only after the
+  // previous upload has completed).
+  if (ws->upload_pending) RWH_HIP(hipEventSynchronize(ws->upload_done));
+  uint32_t* h = ws->h_stage;
+  const uint32_t S = ws->max_segs;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    h[s] = point_base[s];
+    h[S + s] = n[s];
+    h[2 * S + s] = oct_base[s];
+    h[3 * S + s] = oct_cap[s];
+  }
+  uint32_t* cs = h + 4 * S;
+  const uint32_t n_chunks = n_index / kChunk;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    const uint32_t c0 = point_base[s] / kChunk;
+    const uint32_t c1 = (s + 1 < n_clouds) ? point_base[s + 1] / kChunk : n_chunks;
+    for (uint32_t c = c0; c < c1; ++c) cs[c] = s;
+  }
+  RWH_HIP(hipMemcpyAsync(ws->tables, h, sizeof(uint32_t) * 4 * S, hipMemcpyHostToDevice, st));
+  RWH_HIP(hipMemcpyAsync(ws->chunk_seg(), cs, sizeof(uint32_t) * n_chunks, hipMemcpyHostToDevice, st));
+  RWH_HIP(hipEventRecord(ws->upload_done, st));
+  ws->upload_pending = true;
+
+  SegArgs pts{};
+  pts.n_segs = n_clouds;
+  pts.base = ws->seg_base();
+  pts.count = ws->seg_n();
+  pts.chunk_seg = ws->chunk_seg();
+  SegArgs uni = pts;
+  uni.count = ws->n_unique();
+
+  launch_morton(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, pts, n_index,
+                min_coord, range, st);
+  RWH_TRY(launched());
+  RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, pts, n_clouds, n_index, 30,
+                       false, st));
+  launch_unique(bufs->morton_keys_alt, bufs->morton_keys, pts, n_index, ws->n_unique(),
+                ws->lb_unique, ws->ctrl + 1 * kCtrlWords, st);
+  RWH_TRY(launched());
+  TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
+              bufs->left_child, bufs->parents};
+  launch_radix_tree(ta, uni, n_index, st);
+  launch_edge_count(bufs->prefix_n, bufs->parents, bufs->edge_count, uni, n_index, st);
+  launch_prefix_sum(bufs->edge_count, bufs->edge_offset, uni, n_index, ws->n_oct(), ws->lb_scan,
+                    ws->ctrl + 2 * kCtrlWords, st);
+  OctArgs oa = oct_args(bufs, min_coord, range);
+  oa.n_oct = ws->n_oct();
+  oa.oct_base = ws->oct_base();
+  oa.oct_cap = ws->oct_cap();
+  launch_octree(oa, uni, n_index, st);
+  return launched();
+}
+
+int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts,
+                          rwh_stream_t stream) {
+  RWH_CHECK(ws && counts, "null argument");
+  RWH_CHECK(n_clouds <= ws->max_segs, "n_clouds above the workspace's segments");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  RWH_HIP(hipMemcpyAsync(ws->h_counts, ws->n_unique(), sizeof(uint32_t) * n_clouds,
+                         hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipMemcpyAsync(ws->h_counts + ws->max_segs, ws->n_oct(), sizeof(uint32_t) * n_clouds,
+                         hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipStreamSynchronize(st));
+  std::memcpy(counts, ws->h_counts, sizeof(uint32_t) * n_clouds);
+  std::memcpy(counts + n_clouds, ws->h_counts + ws->max_segs, sizeof(uint32_t) * n_clouds);
+  return RWH_OK;
+}
+
+}  // extern "C"

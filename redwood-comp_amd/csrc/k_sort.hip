@@ -1,0 +1,258 @@
+// Stage 2: LSD radix sort of 32-bit keys as an 8-bit-digit onesweep.
+//
+// What it replaces: the reference's CPU binning passes
+// (apps/tree/host/func_sort.cpp:117-211) and its CUDA onesweep
+// (apps/tree/cuda/cu_dispatcher.cu:82-468).  The design is the published
+// Onesweep scheme (Adinets & Merrill 2022) with decoupled look-back
+// (Merrill & Garland 2016), laid out for gfx950:
+//   * one sweep over the keys builds the digit histograms of every pass
+//     (16-byte loads, LDS histograms, spread global atomics);
+//   * a small kernel turns them into per-segment exclusive digit starts;
+//   * one launch per digit pass: a workgroup takes the next tile from an atomic
+//     counter, ranks its keys with 64-lane ballots (wave-level multisplit),
+//     publishes its digit counts, stages the tile in LDS in digit order while
+//     it looks back over its predecessors, then writes each digit run to its
+//     final place (coalesced along the run).
+// Segments (clouds) are sorted independently in one launch: tiles never cross
+// a segment boundary and the look-back stops at the segment's first tile.
+#include "rw_device.hpp"
+#include "rw_internal.hpp"
+
+namespace rw {
+
+// ---------------------------------------------------------------------------
+// Histogram sweep.  Workgroup u reads histogram unit u (unit_keys keys, within
+// one segment) with uint4 loads and counts the digits of all passes in LDS;
+// the unit's counts are added to copy (u % kHistCopies) of its segment's
+// histogram, which spreads the global atomics over many cache lines.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __restrict__ keys,
+                                                           SegArgs seg, uint32_t unit_keys,
+                                                           uint32_t n_units, uint32_t n_passes,
+                                                           uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kMaxPasses * 256];
+  for (uint32_t i = threadIdx.x; i < kMaxPasses * 256; i += kHistBlock) s_h[i] = 0;
+  __syncthreads();
+  const uint32_t u = blockIdx.x;
+  const uint32_t first = u * unit_keys;
+  const Seg sg = seg_of(seg, first);
+  const uint32_t lim = sg.base + sg.count;
+  if (first < lim) {
+    const uint32_t end = min(first + unit_keys, lim);
+    // aligned uint4 body (first is a multiple of 4), scalar tail
+    const uint32_t body = first + ((end - first) & ~3u);
+    const uint4* v4 = reinterpret_cast<const uint4*>(keys + first);
+    const uint32_t n4 = (body - first) >> 2;
+    for (uint32_t i = threadIdx.x; i < n4; i += kHistBlock) {
+      const uint4 q = v4[i];
+      const uint32_t k[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        for (uint32_t p = 0; p < n_passes; ++p)
+          atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & 255u)], 1u);
+    }
+    for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
+      const uint32_t k = keys[i];
+      for (uint32_t p = 0; p < n_passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t* dst = hist + (static_cast<size_t>(sg.id) * kHistCopies + (u % kHistCopies)) *
+                             (kMaxPasses * 256);
+  for (uint32_t i = threadIdx.x; i < n_passes * 256; i += kHistBlock) {
+    const uint32_t c = s_h[i];
+    if (c) atomicAdd(&dst[i], c);
+  }
+}
+
+// Block-wide exclusive scan of one value per thread for the first 256 threads
+// (4 waves); s_tmp holds 4 words.  Returns the exclusive prefix; *total gets
+// the sum.  All threads of the block must call it.
+__device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t w = t >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (t < 256 && (t & 63u) == 63u) s_tmp[w] = incl;
+  __syncthreads();
+  uint32_t off = 0, sum = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    const uint32_t x = s_tmp[i];
+    off += (i < w) ? x : 0u;
+    sum += x;
+  }
+  __syncthreads();
+  *total = sum;
+  return off + incl - v;
+}
+
+// Digit starts: one workgroup per (segment, pass).  Sums the histogram copies,
+// clears them for the next sort, and writes the exclusive scan over digits.
+__global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
+                                                    uint32_t* __restrict__ digit_start,
+                                                    uint32_t n_passes) {
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t s = blockIdx.x, p = blockIdx.y, d = threadIdx.x;
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kHistCopies; ++k) {
+    uint32_t* h = hist + (static_cast<size_t>(s) * kHistCopies + k) * (kMaxPasses * 256) + p * 256 + d;
+    c += *h;
+    *h = 0;
+  }
+  uint32_t total;
+  const uint32_t ex = scan256(c, s_tmp, &total);
+  digit_start[(static_cast<size_t>(s) * n_passes + p) * 256 + d] = ex;
+}
+
+// ---------------------------------------------------------------------------
+// One onesweep digit pass.
+// Keys are loaded wave-striped (key k of lane l of wave w sits at
+// tile + w*KPT*64 + k*64 + l), which is the order the ranking walks, so the
+// sort is stable.
+// ---------------------------------------------------------------------------
+template <int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg, uint32_t n_tiles,
+    const uint32_t* __restrict__ digit_start, uint32_t pass, uint32_t n_passes, uint32_t shift,
+    uint32_t digit_bits, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
+  constexpr int WAVES = BLOCK / kWave;
+  constexpr uint32_t TILE = BLOCK * KPT;
+  static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
+  static_assert(kChunk % TILE == 0, "tiles must not straddle segments");
+
+  __shared__ uint32_t s_stage[TILE];
+  __shared__ uint32_t s_whist[WAVES][256];
+  __shared__ uint32_t s_tstart[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_tile;
+
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
+  for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t epoch = epoch_of(ctrl);
+  const uint32_t first = tile * TILE;
+  Seg sg{0, 0, 0};
+  uint32_t lim = 0;
+  if (tile < n_tiles) {
+    sg = seg_of(seg, first);
+    lim = sg.base + sg.count;
+  }
+  if (tile >= n_tiles || first >= lim) {  // past the end, or a segment's padding
+    if (tid == 0) finish_launch(ctrl, gridDim.x);
+    return;
+  }
+  const uint32_t n_valid = min(TILE, lim - first);
+  const uint32_t tile_in_seg = (first - sg.base) / TILE;
+  const uint32_t dmask = (1u << digit_bits) - 1u;
+
+  const uint32_t wave = tid / kWave;
+  const uint32_t lane = tid % kWave;
+  const uint32_t wbase = first + wave * (KPT * kWave) + lane;
+
+  uint32_t keys[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const uint32_t idx = wbase + k * kWave;
+    keys[k] = (idx < lim) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
+  }
+
+  // Wave-level multisplit: lanes holding the same digit find each other with
+  // digit_bits ballots; the lowest lane of each group advances the wave's
+  // counter for that digit.
+  uint32_t* wh = s_whist[wave];
+  uint32_t rank[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const uint32_t d = (keys[k] >> shift) & dmask;
+    uint64_t m = ~0ull;
+    for (uint32_t b = 0; b < digit_bits; ++b) {
+      const uint32_t bit = (d >> b) & 1u;
+      const uint64_t bal = ballot(bit != 0);
+      m &= bit ? bal : ~bal;
+    }
+    const uint32_t below = popc_below(m);
+    const uint32_t old = wh[d];
+    if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
+    rank[k] = old + below;
+  }
+  __syncthreads();
+
+  // Per digit: exclusive offsets of the waves, tile count, publish aggregate.
+  uint32_t cnt = 0, cnt_valid = 0;
+  if (tid < 256) {
+    const uint32_t d = tid;
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t t = s_whist[w][d];
+      s_whist[w][d] = run;
+      run += t;
+    }
+    cnt = run;
+    cnt_valid = run - ((d == dmask) ? (TILE - n_valid) : 0u);
+    lb_store(&lookback[static_cast<size_t>(tile) * 256 + d],
+             lb_pack(epoch, tile_in_seg == 0 ? kIncl : kAgg, cnt_valid));
+  }
+  uint32_t total;
+  const uint32_t tstart = scan256(cnt, s_tmp, &total);
+  if (tid < 256) s_tstart[tid] = tstart;
+  __syncthreads();
+
+  // Stage the tile in digit order.
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const uint32_t d = (keys[k] >> shift) & dmask;
+    s_stage[s_tstart[d] + s_whist[wave][d] + rank[k]] = keys[k];
+  }
+
+  // Look back for this digit's exclusive prefix over the segment's earlier tiles.
+  if (tid < 256) {
+    const uint32_t d = tid;
+    uint32_t excl = 0;
+    if (tile_in_seg > 0) {
+      excl = lb_exclusive(lookback + d, 256, tile, tile_in_seg, epoch);
+      lb_store(&lookback[static_cast<size_t>(tile) * 256 + d], lb_pack(epoch, kIncl, excl + cnt_valid));
+    }
+    s_gbase[d] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + d] +
+                 excl - tstart;
+  }
+  __syncthreads();
+
+  // Scatter: consecutive staged keys of one digit go to consecutive addresses.
+  for (uint32_t p = tid; p < n_valid; p += BLOCK) {
+    const uint32_t key = s_stage[p];
+    out[s_gbase[(key >> shift) & dmask] + p] = key;
+  }
+  if (tid == 0) finish_launch(ctrl, gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+static_assert(kSortBlock * kSortKpt == kSortTile, "tile shape");
+
+void launch_sort(const SortLaunch& L, hipStream_t stream) {
+  const uint32_t n_passes = (L.key_bits + 7) / 8;
+  hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
+                     L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, L.hist);
+  hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
+                     L.digit_start, n_passes);
+  // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an
+  // odd count starts with in -> out)
+  const uint32_t* src = L.keys_in;
+  for (uint32_t p = 0; p < n_passes; ++p) {
+    const bool to_out = ((n_passes - p) % 2) == 1;
+    uint32_t* dst = to_out ? L.keys_out : L.keys_tmp;
+    const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
+    hipLaunchKernelGGL((k_onesweep<kSortBlock, kSortKpt>), dim3(L.n_tiles), dim3(kSortBlock), 0,
+                       stream, src, dst, L.seg, L.n_tiles, L.digit_start, p, n_passes, 8 * p, bits,
+                       L.lookback, L.ctrl);
+    src = dst;
+  }
+}
+
+}  // namespace rw

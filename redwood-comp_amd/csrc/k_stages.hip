@@ -1,0 +1,395 @@
+// Stages 1, 3, 4, 5, 6 of the octree build: Morton encode, duplicate removal,
+// Karras radix tree, edge count, exclusive prefix sum.  Every kernel works on
+// segments (clouds) of one index space; a single build is one segment.
+#include "rw_device.hpp"
+#include "rw_internal.hpp"
+
+namespace rw {
+
+// ---------------------------------------------------------------------------
+// S1  Morton encode (reference: apps/tree/cuda/01_morton.cu:8-19,
+// apps/tree/host/func_morton.hpp:31-39).  A workgroup covers 4*256 points;
+// every load is a whole float4 (16 bytes) and lanes read consecutive points.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMortonPerThread = 4;
+constexpr uint32_t kMortonTile = kElemBlock * kMortonPerThread;  // 1024
+
+__global__ __launch_bounds__(kElemBlock) void k_morton(const float4* __restrict__ pts,
+                                                       uint32_t* __restrict__ codes, SegArgs seg,
+                                                       float min_coord, float range) {
+  const uint32_t first = blockIdx.x * kMortonTile;
+  const Seg sg = seg_of(seg, first);
+  const uint32_t lim = sg.base + sg.count;
+#pragma unroll
+  for (uint32_t j = 0; j < kMortonPerThread; ++j) {
+    const uint32_t i = first + j * kElemBlock + threadIdx.x;
+    if (i < lim) {
+      const float4 p = pts[i];
+      codes[i] = morton_encode(p.x, p.y, p.z, min_coord, range);
+    }
+  }
+}
+
+void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index,
+                   float min_coord, float range, hipStream_t stream) {
+  const uint32_t grid = (n_index + kMortonTile - 1) / kMortonTile;
+  if (grid == 0) return;
+  hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kElemBlock), 0, stream, pts, codes, seg, min_coord,
+                     range);
+}
+
+// ---------------------------------------------------------------------------
+// Tile-wide exclusive scan for kScanBlock threads (4 waves).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  constexpr int W = kScanBlock / kWave;
+  const uint32_t t = threadIdx.x, w = t / kWave;
+  const uint32_t incl = wave_incl_scan(v);
+  if ((t % kWave) == kWave - 1) s_w[w] = incl;
+  __syncthreads();
+  uint32_t off = 0, sum = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const uint32_t x = s_w[i];
+    off += (static_cast<uint32_t>(i) < w) ? x : 0u;
+    sum += x;
+  }
+  __syncthreads();
+  *total = sum;
+  return off + incl - v;
+}
+
+// Tile prefix over the segment's earlier tiles by decoupled look-back; thread 0
+// publishes and looks back, the result is broadcast through LDS.
+__device__ __forceinline__ uint32_t tile_prefix(uint32_t tile_total, uint32_t tile,
+                                                uint32_t tile_in_seg, uint32_t epoch,
+                                                uint64_t* lookback, uint32_t* s_bcast) {
+  if (threadIdx.x == 0) {
+    uint32_t excl = 0;
+    if (tile_in_seg == 0) {
+      lb_store(&lookback[tile], lb_pack(epoch, kIncl, tile_total));
+    } else {
+      lb_store(&lookback[tile], lb_pack(epoch, kAgg, tile_total));
+      excl = lb_exclusive(lookback, 1, tile, tile_in_seg, epoch);
+      lb_store(&lookback[tile], lb_pack(epoch, kIncl, excl + tile_total));
+    }
+    *s_bcast = excl;
+  }
+  __syncthreads();
+  return *s_bcast;
+}
+
+// ---------------------------------------------------------------------------
+// S3  unique (reference: CPU std::unique_copy host_dispatchers.cpp:183-194;
+// CUDA flag/scan/scatter cu_dispatcher.cu:597-627).  Single pass: head flags,
+// tile scan, look-back, scatter; the segment's last tile writes n_unique.
+// Layout: wave w of a tile owns kScanKpt*64 consecutive keys, read as uint4
+// groups g (lane l holds keys 4*(g*64+l) .. +3 of the wave's range).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out, SegArgs seg,
+                                                       uint32_t n_tiles, uint32_t* __restrict__ n_unique,
+                                                       uint64_t* __restrict__ lookback,
+                                                       uint32_t* __restrict__ ctrl) {
+  constexpr int G = kScanKpt / 4;
+  __shared__ uint32_t s_w[kScanBlock / kWave];
+  __shared__ uint32_t s_bcast, s_tile;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t epoch = epoch_of(ctrl);
+  const uint32_t first = tile * kScanTile;
+  Seg sg{0, 0, 0};
+  uint32_t lim = 0;
+  if (tile < n_tiles) {
+    sg = seg_of(seg, first);
+    lim = sg.base + sg.count;
+  }
+  if (tile >= n_tiles || first >= lim) {
+    // an empty segment still reports n_unique = 0 from its first tile
+    if (tid == 0 && tile < n_tiles && first == sg.base) n_unique[sg.id] = 0;
+    if (tid == 0) finish_launch(ctrl, gridDim.x);
+    return;
+  }
+  const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
+  const uint32_t wave = tid / kWave, lane = tid % kWave;
+  const uint32_t wbase = first + wave * (kScanKpt * kWave);
+
+  uint32_t k[G][4];
+  uint32_t flags[G];  // 4 head bits per group
+  uint32_t cnt[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+    if (i0 + 3 < lim) {
+      const uint4 q = *reinterpret_cast<const uint4*>(in + i0);
+      k[g][0] = q.x; k[g][1] = q.y; k[g][2] = q.z; k[g][3] = q.w;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) k[g][c] = (i0 + c < lim) ? in[i0 + c] : 0u;
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+    // predecessor of the group's first key: previous lane's last key, or memory
+    uint32_t prev = __shfl_up(k[g][3], 1, kWave);
+    if (lane == 0 && i0 > sg.base && i0 < lim) prev = in[i0 - 1];
+    uint32_t f = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t i = i0 + c;
+      const uint32_t pk = (c == 0) ? prev : k[g][c - 1];
+      const bool head = (i < lim) && (i == sg.base || k[g][c] != pk);
+      f |= (head ? 1u : 0u) << c;
+    }
+    flags[g] = f;
+    cnt[g] = __popc(f);
+  }
+  // Order is (wave, group, lane, c): per group a wave scan over lanes.
+  uint32_t wave_run = 0;
+  uint32_t gexcl[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t incl = wave_incl_scan(cnt[g]);
+    gexcl[g] = wave_run + incl - cnt[g];
+    wave_run += __shfl(incl, kWave - 1, kWave);
+  }
+  uint32_t tile_total;
+  const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
+  const uint32_t wave_off = __shfl(wexcl, 0, kWave);
+  const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+  const uint32_t base = sg.base + excl + wave_off;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    uint32_t pos = base + gexcl[g];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if ((flags[g] >> c) & 1u) out[pos++] = k[g][c];
+  }
+  // the tile holding the segment's last key reports the count
+  if (tid == 0 && lim - first <= kScanTile) n_unique[sg.id] = excl + tile_total;
+  if (tid == 0) finish_launch(ctrl, gridDim.x);
+}
+
+void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, uint32_t n_index,
+                   uint32_t* n_unique, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
+  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
+  if (n_tiles == 0) return;
+  hipLaunchKernelGGL(k_unique, dim3(n_tiles), dim3(kScanBlock), 0, stream, sorted, unique_out, seg,
+                     n_tiles, n_unique, lookback, ctrl);
+}
+
+// ---------------------------------------------------------------------------
+// S4  Karras binary radix tree (reference: apps/tree/host/func_brt.hpp:45-138,
+// apps/tree/cuda/04_radix_tree.cu:12-144), one thread per node i in [0, n_brt).
+// Quirks kept: delta = uint8(clz(a^b) - 1) (Q1), the root's range is
+// [0, n_brt - 1] (Q2), clz(0) = 32 and log2_ceil(0) = 0 (D3).
+// Parent slots: every internal node other than the root of the complete tree
+// over [0, n_brt] has exactly one claimer among nodes i >= 1 (the Karras
+// construction); the Q2 root's own claims lose to any such writer under the
+// max-writer rule (D7) and otherwise write 0.  So nodes i >= 1 store plainly,
+// node 0 stores nothing for its Q2 claims, and node 0's thread writes 0 to
+// parent[0] and to the internal children of the complete-range root (the only
+// slots no node i >= 1 claims).  The result equals max-of-writers over zeroed
+// slots without atomics or a clearing pass.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int delta8(uint32_t a, uint32_t b) {
+  return static_cast<uint8_t>(__clz(a ^ b) - 1);  // __clz(0) == 32
+}
+__device__ __forceinline__ int log2_ceil_u32(uint32_t x) {
+  if (x == 0) return 0;
+  const int fl = 31 - __clz(x);
+  return fl + (((1u << fl) < x) ? 1 : 0);
+}
+
+// Split search of the reference (func_brt.hpp:90-104) for node i with range
+// length l, direction d; returns gamma.
+__device__ __forceinline__ int brt_split(const uint32_t* __restrict__ codes, int i, int n, int d,
+                                         int l, int dnode, uint32_t ci) {
+  int s = 0;
+  const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
+  const int scut = (d == -1) ? i - 1 : n - i - 1;
+  uint32_t t = (static_cast<uint32_t>(l) + 1) / 2;
+  for (uint32_t div = 2; div <= max_div; div <<= 1) {
+    if (s + static_cast<int>(t) <= scut && delta8(ci, codes[i + (s + static_cast<int>(t)) * d]) > dnode)
+      s += static_cast<int>(t);
+    t = (static_cast<uint32_t>(l) + (div << 1) - 1) / (div << 1);
+  }
+  return i + s * d + min(d, 0);
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs seg) {
+  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
+  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
+  const int n = static_cast<int>(sg.count);  // n_brt
+  const int i = static_cast<int>(gi - sg.base);
+  if (gi < sg.base || i >= n) return;
+  const uint32_t* __restrict__ codes = a.codes + sg.base;
+  int32_t* __restrict__ parent = a.parent + sg.base;
+  const uint32_t ci = codes[i];
+
+  int d, l = 0;
+  if (i == 0) {
+    d = 1;
+    l = n - 1;  // Q2
+  } else {
+    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
+    d = (diff > 0) - (diff < 0);
+    const int dmin = delta8(ci, codes[i - d]);
+    int lmax = 2;
+    while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
+    const int cutoff = (d == -1) ? i : n - i;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
+  }
+  const int j = i + l * d;
+  const int dnode = delta8(ci, codes[j]);
+  const int gamma = brt_split(codes, i, n, d, l, dnode, ci);
+  const bool ll = min(i, j) == gamma;
+  const bool lr = max(i, j) == gamma + 1;
+  a.prefix_n[sg.base + i] = static_cast<uint8_t>(dnode);
+  a.left_child[sg.base + i] = gamma;
+  a.has_leaf_left[sg.base + i] = ll ? 1 : 0;
+  a.has_leaf_right[sg.base + i] = lr ? 1 : 0;
+  if (i > 0) {
+    if (!ll) parent[gamma] = i;
+    if (!lr) parent[gamma + 1] = i;
+  } else {
+    parent[0] = 0;
+    // children of the root over the complete range [0, n]
+    const int dfull = delta8(ci, codes[n]);
+    const int g = brt_split(codes, 0, n + 1, 1, n, dfull, ci);
+    if (g != 0) parent[g] = 0;
+    if (g + 1 != n && g + 1 < n) parent[g + 1] = 0;
+  }
+}
+
+void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
+  const uint32_t grid = (n_index + kElemBlock - 1) / kElemBlock;
+  if (grid == 0) return;
+  seg.count_sub = 1;
+  hipLaunchKernelGGL(k_radix_tree, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
+}
+
+// ---------------------------------------------------------------------------
+// S5  edge count (reference: apps/tree/host/func_edge.hpp:9-16,
+// apps/tree/cuda/05_edge_count.cu:16-29): count[i] = prefix_n[i]/3 -
+// prefix_n[parent[i]]/3, count[0] = 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kElemBlock) void k_edge_count(const uint8_t* __restrict__ prefix_n,
+                                                           const int32_t* __restrict__ parent,
+                                                           int32_t* __restrict__ edge_count,
+                                                           SegArgs seg) {
+  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
+  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
+  const uint32_t i = gi - sg.base;
+  if (gi < sg.base || i >= sg.count) return;
+  const int p = parent[gi];
+  const int c = static_cast<int>(prefix_n[gi] / 3) - static_cast<int>(prefix_n[sg.base + p] / 3);
+  edge_count[gi] = (i == 0) ? 0 : c;
+}
+
+void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
+                       SegArgs seg, uint32_t n_index, hipStream_t stream) {
+  const uint32_t grid = (n_index + kElemBlock - 1) / kElemBlock;
+  if (grid == 0) return;
+  seg.count_sub = 1;
+  hipLaunchKernelGGL(k_edge_count, dim3(grid), dim3(kElemBlock), 0, stream, prefix_n, parent,
+                     edge_count, seg);
+}
+
+// ---------------------------------------------------------------------------
+// S6  exclusive prefix sum (reference: apps/tree/cuda/06_prefix_sum.cu:26-37,
+// a one-workgroup scan there; D4).  Single pass with decoupled look-back;
+// the segment's last tile writes the total (n_octree_nodes).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __restrict__ in,
+                                                           int32_t* __restrict__ out, SegArgs seg,
+                                                           uint32_t n_tiles, uint32_t* __restrict__ total,
+                                                           uint64_t* __restrict__ lookback,
+                                                           uint32_t* __restrict__ ctrl) {
+  constexpr int G = kScanKpt / 4;
+  __shared__ uint32_t s_w[kScanBlock / kWave];
+  __shared__ uint32_t s_bcast, s_tile;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t epoch = epoch_of(ctrl);
+  const uint32_t first = tile * kScanTile;
+  Seg sg{0, 0, 0};
+  uint32_t lim = 0;
+  if (tile < n_tiles) {
+    sg = seg_of(seg, first);
+    lim = sg.base + sg.count;
+  }
+  if (tile >= n_tiles || first >= lim) {
+    if (tid == 0 && tile < n_tiles && first == sg.base) total[sg.id] = 0;
+    if (tid == 0) finish_launch(ctrl, gridDim.x);
+    return;
+  }
+  const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
+  const uint32_t wave = tid / kWave, lane = tid % kWave;
+  const uint32_t wbase = first + wave * (kScanKpt * kWave);
+  uint32_t v[G][4];
+  uint32_t gsum[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+    if (i0 + 3 < lim) {
+      const int4 q = *reinterpret_cast<const int4*>(in + i0);
+      v[g][0] = q.x; v[g][1] = q.y; v[g][2] = q.z; v[g][3] = q.w;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[g][c] = (i0 + c < lim) ? static_cast<uint32_t>(in[i0 + c]) : 0u;
+    }
+    gsum[g] = v[g][0] + v[g][1] + v[g][2] + v[g][3];
+  }
+  uint32_t wave_run = 0;
+  uint32_t gexcl[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t incl = wave_incl_scan(gsum[g]);
+    gexcl[g] = wave_run + incl - gsum[g];
+    wave_run += __shfl(incl, kWave - 1, kWave);
+  }
+  uint32_t tile_total;
+  const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
+  const uint32_t wave_off = __shfl(wexcl, 0, kWave);
+  const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+  const uint32_t base = excl + wave_off;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+    uint32_t r = base + gexcl[g];
+    uint32_t o[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      o[c] = r;
+      r += v[g][c];
+    }
+    if (i0 + 3 < lim) {
+      *reinterpret_cast<int4*>(out + i0) = make_int4(static_cast<int>(o[0]), static_cast<int>(o[1]),
+                                                     static_cast<int>(o[2]), static_cast<int>(o[3]));
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (i0 + c < lim) out[i0 + c] = static_cast<int>(o[c]);
+    }
+  }
+  if (tid == 0 && lim - first <= kScanTile) total[sg.id] = excl + tile_total;
+  if (tid == 0) finish_launch(ctrl, gridDim.x);
+}
+
+void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
+                       uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
+  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
+  if (n_tiles == 0) return;
+  seg.count_sub = 1;
+  hipLaunchKernelGGL(k_prefix_sum, dim3(n_tiles), dim3(kScanBlock), 0, stream, in, out, seg,
+                     n_tiles, total, lookback, ctrl);
+}
+
+}  // namespace rw

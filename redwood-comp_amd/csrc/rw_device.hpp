@@ -1,0 +1,189 @@
+// Device-side building blocks shared by the octree-build kernels (gfx950,
+// wave64).  Everything here is header-only and inlined into each kernel TU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rw {
+
+constexpr int kWave = 64;
+
+// Segments.  A batch lays clouds out in one "point index space"; every cloud
+// (segment) starts at a multiple of kChunk, so a tile of at most kChunk items
+// that starts at a multiple of its size never straddles two clouds, and the
+// cloud of any index is chunk_seg[index >> kChunkShift].  A single build is the
+// one-segment case with base 0 and no table.
+constexpr uint32_t kChunkShift = 13;
+constexpr uint32_t kChunk = 1u << kChunkShift;  // 8192
+
+struct SegArgs {
+  uint32_t n_segs;            // 1: single segment at base 0 (tables unused)
+  uint32_t single_count;      // element count when count == nullptr (single segment)
+  const uint32_t* base;       // [n_segs] first index of each segment (nullptr: 0)
+  const uint32_t* count;      // [n_segs] live element count per segment (device)
+  const uint32_t* chunk_seg;  // [index >> kChunkShift] -> segment
+  uint32_t count_sub;         // live count = max(count - count_sub, 0)
+};
+
+struct Seg {
+  uint32_t id, base, count;
+};
+
+__device__ __forceinline__ Seg seg_of(const SegArgs& a, uint32_t index) {
+  Seg s;
+  if (a.n_segs == 1) {
+    s.id = 0;
+    s.base = a.base ? a.base[0] : 0u;
+    s.count = a.count ? a.count[0] : a.single_count;
+  } else {
+    s.id = a.chunk_seg[index >> kChunkShift];
+    s.base = a.base[s.id];
+    s.count = a.count[s.id];
+  }
+  s.count = s.count > a.count_sub ? s.count - a.count_sub : 0u;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of m in lanes below this lane.
+__device__ __forceinline__ uint32_t popc_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Inclusive wave scan (sum) of 32-bit values, Kogge-Stone over shuffles.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, kWave);
+    if (lane >= static_cast<uint32_t>(o)) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Decoupled look-back.  A tile publishes a 64-bit word:
+//   [63:34] epoch  [33:32] status (1 aggregate, 2 inclusive prefix)  [31:0] value
+// The epoch is the launch serial of the kernel using the table, so a table is
+// never cleared: words of older launches simply do not match.  The words are
+// read and written with device-scope (agent) relaxed atomics, which keeps them
+// coherent across the XCDs' separate L2s; the word carries all of its data.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kAgg = 1, kIncl = 2;
+
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t status, uint32_t value) {
+  return (static_cast<uint64_t>(epoch) << 34) | (status << 32) | value;
+}
+
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t w) {
+  __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Walks back from tile-1 over the n_prev predecessor tiles of the same
+// segment, accumulating aggregates until an inclusive prefix is found.
+// Every predecessor was handed out by the tile counter before this tile,
+// so it is resident and will publish: the spin ends.
+__device__ __forceinline__ uint32_t lb_exclusive(const uint64_t* table, uint32_t stride,
+                                                 uint32_t tile, uint32_t n_prev, uint32_t epoch) {
+  uint32_t sum = 0;
+  for (uint32_t k = 1; k <= n_prev; ++k) {
+    const uint64_t* p = table + static_cast<size_t>(tile - k) * stride;
+    uint64_t w;
+    do {
+      w = lb_load(p);
+    } while (static_cast<uint32_t>(w >> 34) != epoch || ((w >> 32) & 3u) == 0);
+    sum += static_cast<uint32_t>(w);
+    if (((w >> 32) & 3u) == kIncl) break;
+  }
+  return sum;
+}
+
+// Launch-serial bookkeeping of one look-back user: ctrl[0] tile counter,
+// ctrl[1] finished-workgroup counter, ctrl[2] epoch.  The last workgroup to
+// finish rewinds the counters and advances the epoch, so consecutive launches
+// (and graph replays) need no memset.
+constexpr int kCtrlWords = 4;
+
+__device__ __forceinline__ uint32_t epoch_of(const uint32_t* ctrl) {
+  return (ctrl[2] % 0x3fffffffu) + 1u;
+}
+
+__device__ __forceinline__ void finish_launch(uint32_t* ctrl, uint32_t n_groups) {
+  __threadfence();
+  const uint32_t done = atomicAdd(&ctrl[1], 1u);
+  if (done == n_groups - 1) {
+    ctrl[0] = 0;
+    ctrl[1] = 0;
+    ctrl[2] = ctrl[2] + 1;
+    __threadfence();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Morton code: 10 bits per axis, x in bit 0 (apps/tree/host/func_morton.hpp:15-39).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030c30c3u;
+  v = (v ^ (v >> 4)) & 0x0300f00fu;
+  v = (v ^ (v >> 8)) & 0x030000ffu;
+  v = (v ^ (v >> 16)) & 0x000003ffu;
+  return v;
+}
+
+// u32((v - min) / range * 1024): IEEE sub, div, mul (no contraction), then a
+// saturating truncation (NaN and negatives -> 0, >= 2^32 -> 2^32-1).
+__device__ __forceinline__ uint32_t quantize(float v, float min_coord, float range) {
+  const float t = __fdiv_rn(__fsub_rn(v, min_coord), range);
+  const float s = __fmul_rn(t, 1024.0f);
+  if (!(s > 0.0f)) return 0u;
+  if (s >= 4294967296.0f) return 0xffffffffu;
+  return static_cast<uint32_t>(s);
+}
+
+__device__ __forceinline__ uint32_t morton_encode(float x, float y, float z, float min_coord,
+                                                  float range) {
+  return spread3(quantize(x, min_coord, range)) | (spread3(quantize(y, min_coord, range)) << 1) |
+         (spread3(quantize(z, min_coord, range)) << 2);
+}
+
+// Corner of a cell: (float(d) / 1024) * range + min per axis, w = 1.
+__device__ __forceinline__ float4 morton_decode(uint32_t code, float min_coord, float range) {
+  float4 r;
+  r.x = __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(compact3(code)), 1024.0f), range), min_coord);
+  r.y = __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(compact3(code >> 1)), 1024.0f), range), min_coord);
+  r.z = __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(compact3(code >> 2)), 1024.0f), range), min_coord);
+  r.w = 1.0f;
+  return r;
+}
+
+}  // namespace rw

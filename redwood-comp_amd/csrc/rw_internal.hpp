@@ -1,0 +1,92 @@
+// Host-side internals of the HIP library: launch descriptors shared between
+// the kernel translation units and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rw_device.hpp"
+
+namespace rw {
+
+// Sort shape.  A tile is kSortBlock threads x kSortKpt keys; it divides kChunk.
+constexpr int kSortBlock = 256;
+constexpr int kSortKpt = 16;
+constexpr uint32_t kSortTile = kSortBlock * kSortKpt;  // 4096
+constexpr int kHistBlock = 512;
+constexpr uint32_t kHistCopies = 16;
+constexpr uint32_t kMaxPasses = 4;
+constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
+
+// Compaction / scan tiles.
+constexpr int kScanBlock = 256;
+constexpr int kScanKpt = 16;
+constexpr uint32_t kScanTile = kScanBlock * kScanKpt;  // 4096
+
+// Elementwise tiles.
+constexpr int kElemBlock = 256;
+
+struct SortLaunch {
+  const uint32_t* keys_in;
+  uint32_t* keys_out;
+  uint32_t* keys_tmp;
+  SegArgs seg;
+  uint32_t n_segs;
+  uint32_t n_tiles;        // tiles of kSortTile over the index space
+  uint32_t hist_unit_keys;  // keys per histogram workgroup
+  uint32_t n_hist_units;
+  uint32_t key_bits;       // significant key bits (30 for Morton codes, 32 generic)
+  uint32_t* hist;          // [segs][kHistCopies][kMaxPasses*256], zero between sorts
+  uint32_t* digit_start;   // [segs][passes][256]
+  uint64_t* lookback;      // [n_tiles][256]
+  uint32_t* ctrl;          // kCtrlWords
+};
+void launch_sort(const SortLaunch& L, hipStream_t stream);
+
+void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index,
+                   float min_coord, float range, hipStream_t stream);
+
+// Compaction of adjacent-distinct keys; n_unique[s] per segment.
+void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, uint32_t n_index,
+                   uint32_t* n_unique, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
+
+struct TreeArgs {
+  const uint32_t* codes;  // unique keys (index space)
+  uint8_t* prefix_n;
+  uint8_t* has_leaf_left;
+  uint8_t* has_leaf_right;
+  int32_t* left_child;
+  int32_t* parent;
+};
+// seg.count = n_unique per segment with count_sub = 1, i.e. the live count is n_brt
+void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
+void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
+                       SegArgs seg, uint32_t n_index, hipStream_t stream);
+// Exclusive scan of the live count (n_brt) per segment; total per segment.
+void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
+                       uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
+
+struct OctArgs {
+  const int32_t* edge_offset;
+  const int32_t* edge_count;
+  const uint32_t* codes;
+  const uint8_t* prefix_n;
+  const int32_t* parent;
+  const uint8_t* has_leaf_left;
+  const uint8_t* has_leaf_right;
+  const int32_t* left_child;
+  const uint32_t* n_oct;     // per segment
+  const uint32_t* oct_base;  // per segment (nullptr: 0)
+  const uint32_t* oct_cap;   // per segment (nullptr: single_cap)
+  uint32_t single_cap;
+  float min_coord, range;
+  int32_t* children;
+  float4* corner;
+  float* cell_size;
+  int32_t* child_node_mask;
+  int32_t* child_leaf_mask;
+};
+void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
+void launch_link_leaf(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
+
+}  // namespace rw
