@@ -1,0 +1,457 @@
+"""Python access to the gfx950 octree-build backend.
+
+ctypes bindings of the C ABI (include/rwhip.h, lib/librwhip.so) and of the
+host layer (lib/libredwood_tree.so: input generators, the drop-in
+Engine/AppData/run_stage path), plus small numpy helpers used by the tests and
+the benchmark.  Nothing here computes on the CPU: every stage runs in the HIP
+library, and loading fails loudly when it is missing.
+
+The directory name is not a Python identifier; load it with
+``load_package()`` of ``__graft_entry__`` (it registers the module as
+``redwood_comp_amd``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+RWHIP_PATH = os.path.join(LIB_DIR, "librwhip.so")
+HOST_PATH = os.path.join(LIB_DIR, "libredwood_tree.so")
+
+MIN_COORD = 0.0
+RANGE = 1024.0
+MEMORY_RATIO = np.float32(0.6)
+
+RWH_OK = 0
+RWH_ERR_INVALID_ARG = 1
+RWH_ERR_HIP = 2
+RWH_ERR_CAPACITY = 3
+RWH_ERR_NO_DEVICE = 4
+RWH_ERR_INJECTED = 5
+
+# Every function of include/rwhip.h: name -> (restype, argtypes).
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I = C.c_int
+_F = C.c_float
+_SZ = C.c_size_t
+ABI = {
+    "rwh_error_string": (C.c_char_p, [_I]),
+    "rwh_last_error": (C.c_char_p, []),
+    "rwh_version": (_I, []),
+    "rwh_device_count": (_I, [C.POINTER(_I)]),
+    "rwh_device_name": (_I, [C.c_char_p, _SZ]),
+    "rwh_malloc": (_I, [C.POINTER(_P), _SZ]),
+    "rwh_free": (_I, [_P]),
+    "rwh_host_alloc": (_I, [C.POINTER(_P), _SZ]),
+    "rwh_host_free": (_I, [_P]),
+    "rwh_memcpy_h2d": (_I, [_P, _P, _SZ, _P]),
+    "rwh_memcpy_d2h": (_I, [_P, _P, _SZ, _P]),
+    "rwh_memcpy_d2d": (_I, [_P, _P, _SZ, _P]),
+    "rwh_memset": (_I, [_P, _I, _SZ, _P]),
+    "rwh_stream_create": (_I, [C.POINTER(_P)]),
+    "rwh_stream_destroy": (_I, [_P]),
+    "rwh_stream_sync": (_I, [_P]),
+    "rwh_device_sync": (_I, []),
+    "rwh_event_create": (_I, [C.POINTER(_P)]),
+    "rwh_event_destroy": (_I, [_P]),
+    "rwh_event_record": (_I, [_P, _P]),
+    "rwh_event_sync": (_I, [_P]),
+    "rwh_event_query": (_I, [_P]),
+    "rwh_event_elapsed_ms": (_I, [C.POINTER(_F), _P, _P]),
+    "rwh_workspace_create": (_I, [_U32, _U32, C.POINTER(_P)]),
+    "rwh_workspace_destroy": (_I, [_P]),
+    "rwh_workspace_bytes": (_SZ, [_P]),
+    "rwh_workspace_counts": (_P, [_P]),
+    "rwh_workspace_max_segments": (_U32, [_P]),
+    "rwh_morton": (_I, [_P, _U32, _F, _F, _P, _P]),
+    "rwh_sort_u32": (_I, [_P, _P, _P, _U32, _U32, _P]),
+    "rwh_unique_u32": (_I, [_P, _P, _U32, _P, _P, _P]),
+    "rwh_radix_tree": (_I, [_P, _P, _U32, _P, _P, _P, _P, _P, _P]),
+    "rwh_edge_count": (_I, [_P, _P, _P, _U32, _P, _P]),
+    "rwh_prefix_sum": (_I, [_P, _P, _P, _U32, _P, _P, _P]),
+    "rwh_octree": (_I, [_P, _P, _P, _U32, _F, _F, _P]),
+    "rwh_link_leaf": (_I, [_P, _P, _P, _U32, _P]),
+    "rwh_build_octree": (_I, [_P, _P, _U32, _F, _F, _P]),
+    "rwh_read_counts": (_I, [_P, _U32, C.POINTER(_U32), _P]),
+    "rwh_batch_layout": (_I, [_P, _U32, _P, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "rwh_build_octree_batch": (_I, [_P, _P, _P, _P, _P, _P, _U32, _F, _F, _P]),
+    "rwh_read_batch_counts": (_I, [_P, _U32, _P, _P]),
+    "rwh_debug_fail_next": (None, [_I]),
+}
+
+HOST_ABI = {
+    "rwt_last_error": (C.c_char_p, []),
+    "rwt_bits32": (_U32, [C.c_uint64, C.c_uint64, C.c_uint64]),
+    "rwt_generate_uniform": (None, [C.c_uint64, C.c_uint64, _P]),
+    "rwt_generate_gmm": (None, [C.c_uint64, C.c_uint64, _U32, C.c_double, _P]),
+    "rwt_log_uniform_sizes": (None, [C.c_uint64, _U32, C.c_double, C.c_double, _P]),
+    "rwt_dropin_build": (_I, [_P, _U32, _I] + [_P] * 14 + [_P]),
+}
+
+
+class RwhError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(what)
+        self.status = status
+
+
+class TreeBuffers(C.Structure):
+    """rwh_tree_buffers of include/rwhip.h."""
+
+    _fields_ = [(name, _P) for name in (
+        "points", "morton_keys", "morton_keys_alt", "prefix_n", "has_leaf_left",
+        "has_leaf_right", "left_child", "parents", "edge_count", "edge_offset", "children",
+        "corner", "cell_size", "child_node_mask", "child_leaf_mask")] + [("oct_capacity", _U32)]
+
+
+_lib = None
+_host = None
+
+
+def _declare(lib, table):
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """The loaded librwhip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(RWHIP_PATH):
+            raise RuntimeError(f"{RWHIP_PATH} is missing: run __graft_entry__.build() first")
+        _lib = C.CDLL(RWHIP_PATH)
+        _declare(_lib, ABI)
+    return _lib
+
+
+def host():
+    """The loaded libredwood_tree.so (host layer)."""
+    global _host
+    if _host is None:
+        lib()
+        if not os.path.exists(HOST_PATH):
+            raise RuntimeError(f"{HOST_PATH} is missing: run __graft_entry__.build() first")
+        _host = C.CDLL(HOST_PATH)
+        _declare(_host, HOST_ABI)
+    return _host
+
+
+def check(status: int, what: str = "rwhip"):
+    if status != RWH_OK:
+        L = lib()
+        raise RwhError(status, f"{what}: {L.rwh_error_string(status).decode()} "
+                               f"({L.rwh_last_error().decode()})")
+
+
+def device_count() -> int:
+    n = _I(0)
+    check(lib().rwh_device_count(C.byref(n)), "rwh_device_count")
+    return n.value
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().rwh_device_name(buf, 256), "rwh_device_name")
+    return buf.value.decode()
+
+
+def octree_capacity(n: int) -> int:
+    """floor(n * 0.6f) in float arithmetic, as the reference sizes it."""
+    return int(np.float32(n) * MEMORY_RATIO)
+
+
+# ---------------------------------------------------------------------------
+# inputs (the counter-based generators of the host layer)
+# ---------------------------------------------------------------------------
+def uniform_points(n: int, seed: int = 114514) -> np.ndarray:
+    out = np.empty((n, 4), dtype=np.float32)
+    host().rwt_generate_uniform(seed, n, out.ctypes.data)
+    return out
+
+
+def gmm_points(n: int, seed: int, n_centers: int = 64, sigma: float = 20.48) -> np.ndarray:
+    out = np.empty((n, 4), dtype=np.float32)
+    host().rwt_generate_gmm(seed, n, n_centers, sigma, out.ctypes.data)
+    return out
+
+
+def log_uniform_sizes(seed: int, count: int, lo_log2: float, hi_log2: float) -> np.ndarray:
+    out = np.empty(count, dtype=np.uint32)
+    host().rwt_log_uniform_sizes(seed, count, lo_log2, hi_log2, out.ctypes.data)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# device memory
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """A device allocation with a numpy dtype and length."""
+
+    def __init__(self, count: int, dtype, zero: bool = False):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = _P()
+        check(lib().rwh_malloc(C.byref(p), max(self.nbytes, 16)), "rwh_malloc")
+        self.ptr = p.value
+        if zero:
+            check(lib().rwh_memset(self.ptr, 0, max(self.nbytes, 16), None), "rwh_memset")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, stream=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size <= self.count
+        check(lib().rwh_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, stream), "rwh_memcpy_h2d")
+        check(lib().rwh_stream_sync(stream), "rwh_stream_sync")
+
+    def numpy(self, count: int | None = None, stream=None) -> np.ndarray:
+        count = self.count if count is None else int(count)
+        out = np.empty(count, dtype=self.dtype)
+        if count:
+            check(lib().rwh_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes, stream), "rwh_memcpy_d2h")
+            check(lib().rwh_stream_sync(stream), "rwh_stream_sync")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().rwh_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Workspace:
+    def __init__(self, max_points: int, max_segments: int = 1):
+        p = _P()
+        check(lib().rwh_workspace_create(max_points, max_segments, C.byref(p)), "rwh_workspace_create")
+        self.ptr = p.value
+        self.max_segments = max_segments
+
+    @property
+    def counts_ptr(self) -> int:
+        return lib().rwh_workspace_counts(self.ptr)
+
+    def n_unique_ptr(self, s: int = 0) -> int:
+        return self.counts_ptr + 4 * s
+
+    def n_oct_ptr(self, s: int = 0) -> int:
+        return self.counts_ptr + 4 * (self.max_segments + s)
+
+    def counts(self, n_clouds: int = 1) -> tuple[np.ndarray, np.ndarray]:
+        out = np.empty(2 * self.max_segments, dtype=np.uint32)
+        check(lib().rwh_memcpy_d2h(out.ctypes.data, self.counts_ptr, out.nbytes, None), "rwh_memcpy_d2h")
+        check(lib().rwh_stream_sync(None), "rwh_stream_sync")
+        return out[:n_clouds], out[self.max_segments:self.max_segments + n_clouds]
+
+    def free(self):
+        if self.ptr:
+            lib().rwh_workspace_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = _P()
+        check(lib().rwh_stream_create(C.byref(p)), "rwh_stream_create")
+        self.ptr = p.value
+
+    def sync(self):
+        check(lib().rwh_stream_sync(self.ptr), "rwh_stream_sync")
+
+    def __del__(self):
+        try:
+            lib().rwh_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = _P()
+        check(lib().rwh_event_create(C.byref(p)), "rwh_event_create")
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        check(lib().rwh_event_record(self.ptr, stream), "rwh_event_record")
+
+    def sync(self):
+        check(lib().rwh_event_sync(self.ptr), "rwh_event_sync")
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = _F(0)
+        check(lib().rwh_event_elapsed_ms(C.byref(ms), self.ptr, stop.ptr), "rwh_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().rwh_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# one build's device buffers (the AppData layout)
+# ---------------------------------------------------------------------------
+TREE_FIELDS = {
+    # name: (dtype, elements per point or per octree slot, per-point?)
+    "morton_keys": (np.uint32, 1, True),
+    "morton_keys_alt": (np.uint32, 1, True),
+    "prefix_n": (np.uint8, 1, True),
+    "has_leaf_left": (np.uint8, 1, True),
+    "has_leaf_right": (np.uint8, 1, True),
+    "left_child": (np.int32, 1, True),
+    "parents": (np.int32, 1, True),
+    "edge_count": (np.int32, 1, True),
+    "edge_offset": (np.int32, 1, True),
+    "children": (np.int32, 8, False),
+    "corner": (np.float32, 4, False),
+    "cell_size": (np.float32, 1, False),
+    "child_node_mask": (np.int32, 1, False),
+    "child_leaf_mask": (np.int32, 1, False),
+}
+
+
+class TreeBuild:
+    """Device buffers of one build (or a batch index space) and its tree struct."""
+
+    def __init__(self, n_index: int, oct_space: int, points: DeviceArray | None = None):
+        self.n_index = n_index
+        self.oct_space = oct_space
+        self.points = points if points is not None else DeviceArray(4 * n_index, np.float32, zero=True)
+        self.arrays = {}
+        for name, (dt, k, per_point) in TREE_FIELDS.items():
+            self.arrays[name] = DeviceArray(k * (n_index if per_point else oct_space), dt, zero=True)
+        self.struct = TreeBuffers(points=self.points.ptr, oct_capacity=oct_space,
+                                  **{k: v.ptr for k, v in self.arrays.items()})
+
+    def ptr(self) -> int:
+        return C.addressof(self.struct)
+
+    def __getitem__(self, name) -> DeviceArray:
+        return self.arrays[name]
+
+    def read(self, n: int, n_rows: int) -> dict:
+        """Host copies: per-point arrays over [0, n), octree arrays over [0, n_rows)."""
+        out = {}
+        for name, (dt, k, per_point) in TREE_FIELDS.items():
+            cnt = k * (n if per_point else n_rows)
+            a = self.arrays[name].numpy(cnt)
+            out[name] = a.reshape(-1, k) if k > 1 else a
+        return out
+
+
+def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
+                 min_coord: float = MIN_COORD, range_: float = RANGE):
+    """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs)."""
+    n = points.shape[0]
+    cap = octree_capacity(n)
+    ws = ws or Workspace(max(n, 1))
+    tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
+    check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
+    counts = (_U32 * 3)()
+    check(lib().rwh_read_counts(ws.ptr, cap, counts, stream), "rwh_read_counts")
+    n_unique, n_brt, n_oct = counts[0], counts[1], counts[2]
+    out = tb.read(n, min(n_oct, cap))
+    return (n_unique, n_brt, n_oct), out
+
+
+# ---------------------------------------------------------------------------
+# batched build
+# ---------------------------------------------------------------------------
+def batch_layout(sizes):
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+    k = sizes.size
+    pb, ob, oc = (np.zeros(k, np.uint32) for _ in range(3))
+    space, ospace = _U32(0), _U32(0)
+    check(lib().rwh_batch_layout(sizes.ctypes.data, k, pb.ctypes.data, ob.ctypes.data,
+                                 oc.ctypes.data, C.byref(space), C.byref(ospace)), "rwh_batch_layout")
+    return pb, ob, oc, space.value, ospace.value
+
+
+class Batch:
+    """Device buffers and layout of a batch of clouds (one shared index space)."""
+
+    def __init__(self, clouds, ws: Workspace | None = None):
+        self.sizes = np.array([c.shape[0] for c in clouds], np.uint32)
+        self.pb, self.ob, self.oc, self.space, self.ospace = batch_layout(self.sizes)
+        pts = np.zeros((self.space, 4), np.float32)
+        for s, c in enumerate(clouds):
+            pts[self.pb[s]:self.pb[s] + c.shape[0]] = c
+        self.tb = TreeBuild(self.space, max(self.ospace, 1), DeviceArray.from_numpy(pts.reshape(-1)))
+        self.ws = ws or Workspace(self.space, len(clouds))
+
+    def run(self, stream=None, min_coord=MIN_COORD, range_=RANGE):
+        check(lib().rwh_build_octree_batch(
+            self.ws.ptr, self.tb.ptr(), self.sizes.ctypes.data, self.pb.ctypes.data,
+            self.ob.ctypes.data, self.oc.ctypes.data, self.sizes.size, min_coord, range_, stream),
+            "rwh_build_octree_batch")
+
+    def counts(self, stream=None):
+        k = self.sizes.size
+        out = np.zeros(2 * k, np.uint32)
+        check(lib().rwh_read_batch_counts(self.ws.ptr, k, out.ctypes.data, stream), "rwh_read_batch_counts")
+        return out[:k], out[k:]
+
+    def read(self):
+        """Per-cloud host outputs, cut to each cloud's counts."""
+        nu, no = self.counts()
+        full = self.tb.read(self.space, max(self.ospace, 1))
+        res = []
+        for s in range(self.sizes.size):
+            b, n, ob = int(self.pb[s]), int(self.sizes[s]), int(self.ob[s])
+            nb = max(int(nu[s]) - 1, 0)
+            rows = min(int(no[s]), int(self.oc[s]))
+            r = {"n_unique": int(nu[s]), "n_oct": int(no[s]),
+                 "morton_keys_alt": full["morton_keys_alt"][b:b + n],
+                 "unique": full["morton_keys"][b:b + int(nu[s])]}
+            for k in ("prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parents",
+                      "edge_count", "edge_offset"):
+                r[k] = full[k][b:b + nb]
+            for k in ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"):
+                r[k] = full[k][ob:ob + rows]
+            res.append(r)
+        return res
+
+
+def dropin_build(points: np.ndarray, fail_stage: int = 0):
+    """One build through the C++ drop-in path (Engine + AppData + run_stage1..7)."""
+    n = points.shape[0]
+    cap = octree_capacity(n)
+    m, c = max(n, 1), max(cap, 1)
+    o = {"morton_keys_alt": np.zeros(m, np.uint32), "unique": np.zeros(m, np.uint32),
+         "prefix_n": np.zeros(m, np.uint8), "has_leaf_left": np.zeros(m, np.uint8),
+         "has_leaf_right": np.zeros(m, np.uint8), "left_child": np.zeros(m, np.int32),
+         "parents": np.zeros(m, np.int32), "edge_count": np.zeros(m, np.int32),
+         "edge_offset": np.zeros(m, np.int32), "children": np.zeros((c, 8), np.int32),
+         "corner": np.zeros((c, 4), np.float32), "cell_size": np.zeros(c, np.float32),
+         "child_node_mask": np.zeros(c, np.int32), "child_leaf_mask": np.zeros(c, np.int32)}
+    counts = np.zeros(3, np.uint32)
+    pts = np.ascontiguousarray(points, np.float32)
+    rc = host().rwt_dropin_build(pts.ctypes.data, n, fail_stage,
+                                 *[o[k].ctypes.data for k in o], counts.ctypes.data)
+    if rc != 0:
+        raise RwhError(rc, host().rwt_last_error().decode())
+    return tuple(int(x) for x in counts), o

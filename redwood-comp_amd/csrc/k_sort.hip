@@ -29,7 +29,13 @@ namespace rw {
 __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __restrict__ keys,
                                                            SegArgs seg, uint32_t unit_keys,
                                                            uint32_t n_units, uint32_t n_passes,
+                                                           uint32_t last_bits,
                                                            uint32_t* __restrict__ hist) {
+  // the digit of pass p is masked exactly as the binning pass masks it, so
+  // the counts match the ranks whatever the key values
+  uint32_t mask[kMaxPasses];
+#pragma unroll
+  for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
   __shared__ uint32_t s_h[kMaxPasses * 256];
   for (uint32_t i = threadIdx.x; i < kMaxPasses * 256; i += kHistBlock) s_h[i] = 0;
   __syncthreads();
@@ -49,11 +55,11 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 #pragma unroll
       for (int c = 0; c < 4; ++c)
         for (uint32_t p = 0; p < n_passes; ++p)
-          atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & 255u)], 1u);
+          atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & mask[p])], 1u);
     }
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
       const uint32_t k = keys[i];
-      for (uint32_t p = 0; p < n_passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+      for (uint32_t p = 0; p < n_passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
     }
   }
   __syncthreads();
@@ -237,8 +243,9 @@ static_assert(kSortBlock * kSortKpt == kSortTile, "tile shape");
 
 void launch_sort(const SortLaunch& L, hipStream_t stream) {
   const uint32_t n_passes = (L.key_bits + 7) / 8;
+  const uint32_t last_bits = L.key_bits - 8 * (n_passes - 1);
   hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
-                     L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, L.hist);
+                     L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist);
   hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                      L.digit_start, n_passes);
   // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an

@@ -260,9 +260,9 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     parent[0] = 0;
     // children of the root over the complete range [0, n]
     const int dfull = delta8(ci, codes[n]);
-    const int g = brt_split(codes, 0, n + 1, 1, n, dfull, ci);
+    const int g = brt_split(codes, 0, n, 1, n, dfull, ci);
     if (g != 0) parent[g] = 0;
-    if (g + 1 != n && g + 1 < n) parent[g + 1] = 0;
+    if (g + 1 < n) parent[g + 1] = 0;
   }
 }
 

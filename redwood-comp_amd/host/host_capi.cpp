@@ -1,0 +1,92 @@
+// extern "C" access to the host layer for the Python tests and benchmark:
+// the seeded input generators, and one whole build through the drop-in path
+// (Engine + AppData + run_stage1..7 futures) with every output copied back.
+#include <cstring>
+#include <string>
+
+#include "hip_engine.hpp"
+#include "inputs.hpp"
+
+namespace {
+thread_local std::string g_err;
+}
+
+extern "C" {
+
+const char* rwt_last_error(void) { return g_err.c_str(); }
+
+uint32_t rwt_bits32(uint64_t seed, uint64_t stream, uint64_t counter) {
+  return redwood::inputs::bits32(seed, stream, counter);
+}
+
+void rwt_generate_uniform(uint64_t seed, uint64_t n, float* xyzw) {
+  redwood::inputs::uniform(seed, n, xyzw);
+}
+
+void rwt_generate_gmm(uint64_t seed, uint64_t n, uint32_t n_centers, double sigma, float* xyzw) {
+  redwood::inputs::gaussian_mixture(seed, n, n_centers, sigma, xyzw);
+}
+
+void rwt_log_uniform_sizes(uint64_t seed, uint32_t count, double lo_log2, double hi_log2,
+                           uint32_t* out) {
+  const auto v = redwood::inputs::log_uniform_sizes(seed, count, lo_log2, hi_log2);
+  std::memcpy(out, v.data(), sizeof(uint32_t) * count);
+}
+
+// One build through redwood::hip::run_stage1..7 on a fresh Engine.  Output
+// arrays are host memory sized as the AppData buffers (n entries, octree
+// arrays oct_cap = floor(0.6 n) entries, children 8 x, corner 4 x);
+// counts3 = {n_unique, n_brt, n_octree_nodes}.  fail_stage (1..7, 0 none)
+// injects a failure into that stage to exercise the error path.  Returns 0,
+// or the failing stage's number with the message in rwt_last_error().
+int rwt_dropin_build(const float* xyzw, uint32_t n, int fail_stage, uint32_t* sorted,
+                     uint32_t* unique_keys, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                     uint8_t* has_leaf_right, int32_t* left_child, int32_t* parents,
+                     int32_t* edge_count, int32_t* edge_offset, int32_t* children, float* corner,
+                     float* cell_size, int32_t* child_node_mask, int32_t* child_leaf_mask,
+                     uint32_t* counts3) {
+  using namespace redwood::hip;
+  int stage_no = 0;
+  try {
+    auto engine = Engine::create();
+    AppData app(engine->get_mr(), xyzw, n);
+    std::future<void> (*stages[7])(AppData&) = {run_stage1, run_stage2, run_stage3, run_stage4,
+                                                 run_stage5, run_stage6, run_stage7};
+    for (stage_no = 1; stage_no <= 7; ++stage_no) {
+      if (fail_stage == stage_no) rwh_debug_fail_next(stage_no);
+      stages[stage_no - 1](app).get();
+    }
+    stage_no = 8;
+    const size_t cap = app.oct.u_cell_size.size();
+    const auto cp = [](auto& v, auto* dst, size_t count) {
+      const auto h = v.to_host(count);
+      std::memcpy(dst, h.data(), sizeof(h[0]) * count);
+    };
+    cp(app.u_morton_keys_alt, sorted, n);
+    cp(app.u_morton_keys, unique_keys, n);
+    cp(app.brt.u_prefix_n, prefix_n, n);
+    cp(app.brt.u_has_leaf_left, has_leaf_left, n);
+    cp(app.brt.u_has_leaf_right, has_leaf_right, n);
+    cp(app.brt.u_left_child, left_child, n);
+    cp(app.brt.u_parents, parents, n);
+    cp(app.u_edge_count, edge_count, n);
+    cp(app.u_edge_offset, edge_offset, n);
+    cp(app.oct.u_children, children, 8 * cap);
+    {
+      const auto h = app.oct.u_corner.to_host(cap);
+      std::memcpy(corner, h.data(), sizeof(float) * 4 * cap);
+    }
+    cp(app.oct.u_cell_size, cell_size, cap);
+    cp(app.oct.u_child_node_mask, child_node_mask, cap);
+    cp(app.oct.u_child_leaf_mask, child_leaf_mask, cap);
+    counts3[0] = app.get_n_unique();
+    counts3[1] = app.get_n_brt_nodes();
+    counts3[2] = app.get_n_octree_nodes();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return stage_no == 0 ? -1 : stage_no;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,308 @@
+"""Pins the CPU oracle (oracle/tree_oracle.cpp) before it is trusted.
+
+Each stage is checked against something that is not the oracle's own formula:
+known answers and an independent formulation (S1), numpy and the reference's
+own func_sort.cpp compiled from source (S2), numpy.unique (S3), a brute-force
+top-down construction and the reference's own func_brt.hpp (S4), the
+reference's func_edge.hpp and a closed form (S5), numpy.cumsum (S6), and
+structural invariants with closed forms (S7).
+"""
+import numpy as np
+import pytest
+
+
+# ---------------------------------------------------------------------------
+# helpers: an independent Morton formulation (the reference's magic numbers,
+# func_morton.hpp:15-22, vectorised) and a brute-force radix tree
+# ---------------------------------------------------------------------------
+def spread_magic(v):
+    x = v.astype(np.uint64) & 0x3FF
+    x = (x | (x << 16)) & 0x30000FF
+    x = (x | (x << 8)) & 0x0300F00F
+    x = (x | (x << 4)) & 0x30C30C3
+    x = (x | (x << 2)) & 0x9249249
+    return x.astype(np.uint32)
+
+
+def morton_numpy(pts, min_coord=np.float32(0), range_=np.float32(1024)):
+    q = []
+    for a in range(3):
+        t = (pts[:, a] - min_coord) / range_
+        s = (t * np.float32(1024)).astype(np.float32)
+        q.append(np.where(s > 0, np.floor(s), 0).astype(np.int64).astype(np.uint32))
+    return spread_magic(q[0]) | (spread_magic(q[1]) << 1) | (spread_magic(q[2]) << 2)
+
+
+def clz32(x):
+    return 32 if x == 0 else 32 - int(x).bit_length()
+
+
+def delta8(a, b):
+    return (clz32(int(a) ^ int(b)) - 1) & 0xFF
+
+
+def split_of(c, a, b):
+    """Last index of [a, b] whose key has a 0 at the highest bit where c[a], c[b] differ."""
+    if a == b:
+        return a
+    bit = 31 - clz32(int(c[a]) ^ int(c[b]))
+    g = a
+    for k in range(a, b + 1):
+        if not (int(c[k]) >> bit) & 1:
+            g = k
+    return g
+
+
+def brute_tree(c):
+    """Radix tree over unique sorted keys c by top-down recursion (a second
+    construction of the Karras tree), with the reference's quirks: node 0
+    describes the range [0, n-1] (Q2), delta = uint8(clz - 1) (Q1), and a
+    parent slot keeps the largest index that claims it (D7)."""
+    n = len(c) - 1  # brt nodes
+    P = np.zeros(n, np.uint8)
+    HL = np.zeros(n, np.uint8)
+    HR = np.zeros(n, np.uint8)
+    LC = np.zeros(n, np.int32)
+    claims = {}
+
+    def node(idx, a, b):
+        g = split_of(c, a, b)
+        P[idx] = delta8(c[a], c[b])
+        LC[idx] = g
+        HL[idx] = 1 if a == g else 0
+        HR[idx] = 1 if b == g + 1 else 0
+        return g
+
+    def rec(idx, a, b):  # the complete tree below the root; fills nodes != 0
+        g = split_of(c, a, b)
+        if idx != 0:
+            node(idx, a, b)
+        if g != a:
+            claims.setdefault(g, []).append(idx)
+            rec(g, a, g)
+        if g + 1 != b:
+            claims.setdefault(g + 1, []).append(idx)
+            rec(g + 1, g + 1, b)
+
+    rec(0, 0, n)
+    # Q2 root over [0, n-1]
+    g0 = node(0, 0, n - 1) if n - 1 > 0 else 0
+    if n - 1 == 0:
+        P[0] = delta8(c[0], c[0])
+        LC[0] = 0
+        HL[0] = 1
+        HR[0] = 0
+    if not HL[0]:
+        claims.setdefault(int(LC[0]), []).append(0)
+    if not HR[0]:
+        claims.setdefault(int(LC[0]) + 1, []).append(0)
+    par = np.zeros(max(n, 1), np.int32)
+    for k, w in claims.items():
+        if k < n:
+            par[k] = max(w)
+    return {"prefix_n": P, "has_leaf_left": HL, "has_leaf_right": HR, "left_child": LC,
+            "parents": par[:n]}
+
+
+def unique_keys(rng, n, bits=30):
+    while True:
+        k = np.unique(rng.integers(0, 1 << bits, size=n, dtype=np.uint64)).astype(np.uint32)
+        if k.size >= 2:
+            return k
+
+
+# ---------------------------------------------------------------------------
+# S1 Morton
+# ---------------------------------------------------------------------------
+def test_morton_known_answers(orc, pool):
+    pts = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [1023, 1023, 1023, 0],
+                    [1023.99994, 0, 0, 0], [1024, 0, 0, 0], [0.5, 0.99, 1.0, 0],
+                    [512, 0, 0, 0], [0, 512, 0, 0], [0, 0, 512, 0]], np.float32)
+    got = orc.morton(pool, pts)
+    want = [1, 2, 4, (1 << 30) - 1, int(spread_magic(np.array([1023]))[0]), 0, 4,
+            1 << 27, 1 << 28, 1 << 29]
+    assert got.tolist() == want
+
+
+def test_morton_matches_magic_numbers(orc, pool):
+    rng = np.random.default_rng(1)
+    pts = (rng.random((50000, 4)) * 1024).astype(np.float32)
+    assert np.array_equal(orc.morton(pool, pts), morton_numpy(pts))
+    # a non power-of-two box exercises the float sub/div/mul rounding
+    mn, rg = np.float32(-3.25), np.float32(777.7)
+    pts2 = (mn + rng.random((50000, 4)) * rg).astype(np.float32)
+    assert np.array_equal(orc.morton(pool, pts2, float(mn), float(rg)), morton_numpy(pts2, mn, rg))
+
+
+def test_morton_saturation_out_of_box(orc, pool):
+    # below the box and NaN quantise to 0; far above saturates then keeps 10 bits (D10)
+    pts = np.array([[-5, 0, 0, 0], [np.nan, 0, 0, 0], [1e12, 0, 0, 0]], np.float32)
+    assert orc.morton(pool, pts).tolist() == [0, 0, int(spread_magic(np.array([0x3FF]))[0])]
+
+
+# ---------------------------------------------------------------------------
+# S2 sort, S3 unique
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 7, 1000, 307200])
+def test_sort_matches_numpy(orc, pool, n):
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(orc.sort(pool, k), np.sort(k))
+
+
+def test_sort_matches_reference_source(orc, pool):
+    if not orc.ref_available():
+        pytest.skip("reference CPU kernels not built (no /root/reference here)")
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 1 << 30, size=100003, dtype=np.uint64).astype(np.uint32)
+    keys = k.copy()
+    alt = np.zeros_like(k)
+    orc.ref().ref_sort(keys.ctypes.data, alt.ctypes.data, k.size, 4)
+    assert np.array_equal(orc.sort(pool, k), keys)
+
+
+def test_unique_matches_numpy(orc):
+    rng = np.random.default_rng(3)
+    for n in (0, 1, 2, 1000, 100000):
+        k = np.sort(rng.integers(0, max(n // 3, 1), size=n).astype(np.uint32))
+        assert np.array_equal(orc.unique(k), np.unique(k))
+
+
+# ---------------------------------------------------------------------------
+# S4 radix tree
+# ---------------------------------------------------------------------------
+def test_radix_tree_brute_force_tiny(orc, pool):
+    rng = np.random.default_rng(11)
+    cases = [np.array([0, 1], np.uint32), np.array([0, 1, 2], np.uint32),
+             np.array([0b000, 0b001, 0b100, 0b101], np.uint32),
+             np.array([1, 2, 3, 1 << 29], np.uint32)]
+    for _ in range(400):
+        n = int(rng.integers(2, 65))
+        bits = int(rng.choice([4, 8, 12, 30]))
+        if (1 << bits) < n:
+            bits = 12
+        cases.append(np.sort(rng.choice(1 << bits, size=n, replace=False)).astype(np.uint32))
+    for c in cases:
+        got = orc.radix_tree(pool, c)
+        want = brute_tree(c)
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (k, c.tolist())
+
+
+def test_radix_tree_matches_reference_source(orc, pool):
+    if not orc.ref_available():
+        pytest.skip("reference CPU kernels not built (no /root/reference here)")
+    rng = np.random.default_rng(12)
+    for n in (3, 17, 1000, 200000):
+        c = unique_keys(rng, n)
+        nu = c.size
+        r = {k: np.zeros(nu, dt) for k, dt in (("prefix_n", np.uint8), ("has_leaf_left", np.uint8),
+                                                ("has_leaf_right", np.uint8), ("left_child", np.int32),
+                                                ("parents", np.int32))}
+        orc.ref().ref_radix_tree(c.ctypes.data, nu, r["prefix_n"].ctypes.data,
+                                 r["has_leaf_left"].ctypes.data, r["has_leaf_right"].ctypes.data,
+                                 r["left_child"].ctypes.data, r["parents"].ctypes.data)
+        got = orc.radix_tree(pool, c)
+        for k in got:
+            assert np.array_equal(got[k], r[k][:nu - 1]), k
+
+
+def test_radix_tree_invariants(orc, pool):
+    rng = np.random.default_rng(13)
+    c = unique_keys(rng, 50000)
+    t = orc.radix_tree(pool, c)
+    n = c.size - 1
+    par = t["parents"]
+    # a parent always describes a shorter common prefix than its child
+    assert np.all(t["prefix_n"][par[1:]] <= t["prefix_n"][1:])
+    assert par[0] == 0 and np.all((par >= 0) & (par < n))
+    assert np.all((t["left_child"] >= 0) & (t["left_child"] < n))
+
+
+# ---------------------------------------------------------------------------
+# S5 edge count, S6 prefix sum
+# ---------------------------------------------------------------------------
+def test_edge_count_closed_form_and_reference(orc, pool):
+    rng = np.random.default_rng(21)
+    c = unique_keys(rng, 30000)
+    t = orc.radix_tree(pool, c)
+    got = orc.edge_count(pool, t["prefix_n"], t["parents"])
+    want = t["prefix_n"].astype(np.int32) // 3 - t["prefix_n"][t["parents"]].astype(np.int32) // 3
+    want[0] = 0
+    assert np.array_equal(got, want)
+    assert np.all(got >= 0)
+    if orc.ref_available():
+        r = np.zeros_like(got)
+        orc.ref().ref_edge_count(t["prefix_n"].ctypes.data, t["parents"].ctypes.data, got.size,
+                                 r.ctypes.data)
+        assert np.array_equal(got, r)
+
+
+def test_prefix_sum_matches_cumsum(orc):
+    rng = np.random.default_rng(31)
+    for n in (0, 1, 5, 100000):
+        c = rng.integers(0, 11, size=n).astype(np.int32)
+        off, total = orc.prefix_sum(c)
+        assert total == int(c.sum())
+        if n:
+            assert np.array_equal(off, np.concatenate([[0], np.cumsum(c)[:-1]]).astype(np.int32))
+
+
+# ---------------------------------------------------------------------------
+# S7 octree: structural invariants with closed forms
+# ---------------------------------------------------------------------------
+def decode_numpy(code):
+    d = []
+    for a in range(3):
+        v = 0
+        for b in range(10):
+            v |= ((int(code) >> (3 * b + a)) & 1) << b
+        d.append(v)
+    return d
+
+
+def test_octree_invariants(orc, pool, rw):
+    pts = rw.uniform_points(20000, seed=7)
+    (nu, nb, no, rc), o, _ = orc.build(pool, pts)
+    assert rc == 0
+    codes = o["unique"][:nu]
+    cnt = o["edge_count"][:nb]
+    off = o["edge_offset"][:nb]
+    pn = o["prefix_n"][:nb]
+    root_level = int(pn[0]) // 3
+    owner = np.full(no, -1)
+    for i in range(1, nb):
+        for j in range(cnt[i]):
+            assert owner[off[i] + j] == -1
+            owner[off[i] + j] = i
+    assert np.all(owner >= 0)  # every slot has exactly one owner
+    ch = o["children"][:no]
+    mask = o["child_node_mask"][:no]
+    # mask bit set <=> that child slot holds a node index
+    for s in range(no):
+        for w in range(8):
+            assert bool(mask[s] >> w & 1) == (ch[s, w] != 0 or (s == 0 and False)) or ch[s, w] == 0
+    assert np.all(o["child_leaf_mask"][:no] == 0)  # Q6
+    # corner = decoded prefix of the owner's code at the slot's level; cell closed form
+    rng = np.random.default_rng(0)
+    for s in rng.choice(no, size=min(no, 3000), replace=False):
+        i = owner[s]
+        j = s - off[i]
+        level = int(pn[i]) // 3 - j
+        sh = (30 - 3 * level) & 31
+        prefix = (int(codes[i]) >> sh) << sh
+        x, y, z = decode_numpy(prefix)
+        assert o["corner"][s].tolist() == [x * 1.0, y * 1.0, z * 1.0, 1.0]
+        assert o["cell_size"][s] * (1 << (level - root_level)) == 1024.0
+
+
+def test_build_is_thread_count_invariant(orc, rw):
+    pts = rw.gmm_points(40000, seed=3, n_centers=8, sigma=5.0)
+    ref = None
+    for t in (1, 3, 8):
+        p = orc.Pool(t)
+        counts, o, _ = orc.build(p, pts)
+        flat = [counts] + [o[k].tobytes() for k in sorted(o) if k != "tmp"]
+        if ref is None:
+            ref = flat
+        assert flat == ref
