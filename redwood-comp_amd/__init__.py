@@ -91,6 +91,12 @@ HOST_ABI = {
     "rwt_generate_gmm": (None, [C.c_uint64, C.c_uint64, _U32, C.c_double, _P]),
     "rwt_log_uniform_sizes": (None, [C.c_uint64, _U32, C.c_double, C.c_double, _P]),
     "rwt_dropin_build": (_I, [_P, _U32, _I] + [_P] * 14 + [_P]),
+    "rwt_plan_batch": (_U32, [C.c_uint64, _U32, _P, _U32]),
+    "rwt_server_create": (_P, [C.c_uint64, _U32, _U32, _U32, _I]),
+    "rwt_server_destroy": (None, [_P]),
+    "rwt_server_slots": (_U32, [_P]),
+    "rwt_serve_load": (_I, [_P, _U32, _P, _P, C.c_double, _U32, C.c_uint64, _P]),
+    "rwt_serve_mix": (_I, [_P, _U32, _P, _P] + [_P] * 9 + [_P, _P]),
 }
 
 
@@ -455,3 +461,78 @@ def dropin_build(points: np.ndarray, fail_stage: int = 0):
     if rc != 0:
         raise RwhError(rc, host().rwt_last_error().decode())
     return tuple(int(x) for x in counts), o
+
+
+# ---------------------------------------------------------------------------
+# serving (the C++ BuildServer of host/build_server.hpp)
+# ---------------------------------------------------------------------------
+SERVE_STATS = ("rate", "n_requests", "makespan_s", "throughput_rps", "mpts_per_s", "p50_ms",
+               "p99_ms", "mean_ms", "max_ms", "mean_batch", "rejected", "batches")
+
+
+def plan_batch(slot_points: int, max_clouds: int, sizes) -> int:
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    return int(host().rwt_plan_batch(slot_points, max_clouds, sizes.ctypes.data, sizes.size))
+
+
+class Server:
+    def __init__(self, pool_bytes=64 << 30, n_slots=3, slot_points=1 << 26, max_clouds=512,
+                 plain=False):
+        self.ptr = host().rwt_server_create(pool_bytes, n_slots, slot_points, max_clouds, int(plain))
+        if not self.ptr:
+            raise RwhError(-1, host().rwt_last_error().decode())
+        self.n_slots = host().rwt_server_slots(self.ptr)
+
+    def load(self, clouds, rate: float, n_requests: int, seed: int = 1) -> dict:
+        ptrs = (C.c_void_p * len(clouds))(*[c.ctypes.data for c in clouds])
+        sizes = np.array([c.shape[0] for c in clouds], np.uint32)
+        stats = np.zeros(12, np.float64)
+        rc = host().rwt_serve_load(self.ptr, len(clouds), ptrs, sizes.ctypes.data, rate,
+                                   n_requests, seed, stats.ctypes.data)
+        if rc != 0:
+            raise RwhError(rc, host().rwt_last_error().decode())
+        return dict(zip(SERVE_STATS, stats.tolist()))
+
+    def mix(self, clouds):
+        """All clouds submitted together, outputs kept; per-request results."""
+        k = len(clouds)
+        fields = {"morton_keys_alt": np.uint32, "unique": np.uint32, "prefix_n": np.uint8,
+                  "parents": np.int32, "edge_offset": np.int32, "children": np.int32,
+                  "corner": np.float32, "cell_size": np.float32, "child_node_mask": np.int32}
+        width = {"children": 8, "corner": 4}
+        outs = []
+        for c in clouds:
+            n = c.shape[0]
+            cap = max(octree_capacity(n), 1)
+            o = {}
+            for f, dt in fields.items():
+                per_slot = f in ("children", "corner", "cell_size", "child_node_mask")
+                o[f] = np.zeros((cap if per_slot else max(n, 1)) * width.get(f, 1), dt)
+            outs.append(o)
+        tables = [(C.c_void_p * k)(*[outs[r][f].ctypes.data for r in range(k)]) for f in fields]
+        ptrs = (C.c_void_p * k)(*[c.ctypes.data for c in clouds])
+        sizes = np.array([c.shape[0] for c in clouds], np.uint32)
+        counts = np.zeros(3 * k, np.uint32)
+        bsz = np.zeros(k, np.uint32)
+        rc = host().rwt_serve_mix(self.ptr, k, ptrs, sizes.ctypes.data, *tables,
+                                  counts.ctypes.data, bsz.ctypes.data)
+        if rc != 0:
+            raise RwhError(rc, host().rwt_last_error().decode())
+        res = []
+        for r in range(k):
+            o = outs[r]
+            o["children"] = o["children"].reshape(-1, 8)
+            o["corner"] = o["corner"].reshape(-1, 4)
+            res.append((tuple(int(x) for x in counts[3 * r:3 * r + 3]), o, int(bsz[r])))
+        return res
+
+    def close(self):
+        if self.ptr:
+            host().rwt_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

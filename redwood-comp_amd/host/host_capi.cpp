@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "build_server.hpp"
 #include "hip_engine.hpp"
 #include "inputs.hpp"
 
@@ -87,6 +88,101 @@ int rwt_dropin_build(const float* xyzw, uint32_t n, int fail_stage, uint32_t* so
     g_err = e.what();
     return stage_no == 0 ? -1 : stage_no;
   }
+}
+
+// ---- serving ------------------------------------------------------------------
+// Pure planning (no GPU): how many of the queued sizes the next batch of a
+// slot with slot_points index-space points and max_clouds clouds takes.
+uint32_t rwt_plan_batch(uint64_t slot_points, uint32_t max_clouds, const uint32_t* sizes,
+                        uint32_t n_sizes) {
+  redwood::serve::Planner p{slot_points, static_cast<uint64_t>(slot_points * 0.6) + 64ull * max_clouds,
+                            max_clouds};
+  return static_cast<uint32_t>(p.plan(std::vector<uint32_t>(sizes, sizes + n_sizes)));
+}
+
+void* rwt_server_create(uint64_t pool_bytes, uint32_t n_slots, uint32_t slot_points,
+                        uint32_t max_clouds, int plain) {
+  try {
+    redwood::serve::ServerConfig c;
+    c.pool_bytes = pool_bytes;
+    c.n_slots = n_slots;
+    c.slot_points = slot_points;
+    c.max_clouds = max_clouds;
+    c.plain = plain != 0;
+    return new redwood::serve::BuildServer(c);
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void rwt_server_destroy(void* server) { delete static_cast<redwood::serve::BuildServer*>(server); }
+
+uint32_t rwt_server_slots(void* server) {
+  return static_cast<redwood::serve::BuildServer*>(server)->n_slots();
+}
+
+// Open-loop Poisson load over clouds[k] (host points, sizes[k]);
+// stats[12] = rate, n, makespan_s, rps, Mpts/s, p50, p99, mean, max (ms),
+// mean batch, rejected, batches launched.
+int rwt_serve_load(void* server, uint32_t n_clouds, const float* const* clouds,
+                   const uint32_t* sizes, double rate, uint32_t n_requests, uint64_t seed,
+                   double* stats) {
+  try {
+    auto& s = *static_cast<redwood::serve::BuildServer*>(server);
+    const uint64_t b0 = s.batches_launched();
+    const auto st = redwood::serve::run_open_loop(
+        s, std::vector<const float*>(clouds, clouds + n_clouds),
+        std::vector<uint32_t>(sizes, sizes + n_clouds), rate, n_requests, seed);
+    const double v[12] = {st.rate, static_cast<double>(st.n_requests), st.makespan_s,
+                          st.throughput_rps, st.mpts_per_s, st.p50_ms, st.p99_ms, st.mean_ms,
+                          st.max_ms, st.mean_batch, static_cast<double>(st.rejected),
+                          static_cast<double>(s.batches_launched() - b0)};
+    for (int i = 0; i < 12; ++i) stats[i] = v[i];
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Submits every cloud at once with outputs kept, waits, and copies the
+// results of request r into the arrays at the r-th entry of each pointer
+// table (sizes as in rwt_dropin_build; counts3[r*3..]).  Returns 0, or 1 + r
+// for the first failed request.
+int rwt_serve_mix(void* server, uint32_t n_clouds, const float* const* clouds,
+                  const uint32_t* sizes, uint32_t* const* sorted, uint32_t* const* unique_keys,
+                  uint8_t* const* prefix_n, int32_t* const* parents, int32_t* const* edge_offset,
+                  int32_t* const* children, float* const* corner, float* const* cell_size,
+                  int32_t* const* child_node_mask, uint32_t* counts3, uint32_t* batch_sizes) {
+  auto& s = *static_cast<redwood::serve::BuildServer*>(server);
+  std::vector<std::future<redwood::serve::BuildResult>> f;
+  for (uint32_t r = 0; r < n_clouds; ++r) f.push_back(s.submit(clouds[r], sizes[r], true));
+  for (uint32_t r = 0; r < n_clouds; ++r) {
+    try {
+      const auto res = f[r].get();
+      const auto cp = [](const auto& v, auto* dst) {
+        if (!v.empty()) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
+      };
+      cp(res.sorted, sorted[r]);
+      cp(res.unique_keys, unique_keys[r]);
+      cp(res.prefix_n, prefix_n[r]);
+      cp(res.parents, parents[r]);
+      cp(res.edge_offset, edge_offset[r]);
+      cp(res.children, children[r]);
+      cp(res.corner, corner[r]);
+      cp(res.cell_size, cell_size[r]);
+      cp(res.child_node_mask, child_node_mask[r]);
+      counts3[3 * r] = res.n_unique;
+      counts3[3 * r + 1] = res.n_unique > 0 ? res.n_unique - 1 : 0;
+      counts3[3 * r + 2] = res.n_octree_nodes;
+      batch_sizes[r] = res.batch_size;
+    } catch (const std::exception& e) {
+      g_err = e.what();
+      return 1 + static_cast<int>(r);
+    }
+  }
+  return 0;
 }
 
 }  // extern "C"
