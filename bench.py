@@ -84,6 +84,105 @@ def cpu_baseline(pts, budget_s=12.0):
                       f"pthread oracle, median of {len(times)} runs after 1 warm-up"}
 
 
+def batched_section(rw, reps=5):
+    """512 clouds (sizes log-uniform in [2^10, 2^20], seed 7; even clouds uniform,
+    odd clouds 16-cluster Gaussian with sigma 10.24) built in one segmented pass,
+    against one plain build per cloud on the same resident buffers."""
+    import ctypes as C
+    L = rw.lib()
+    sizes = rw.log_uniform_sizes(7, 512, 10, 20)
+    clouds = [rw.uniform_points(int(n), (7 << 20) + s) if s % 2 == 0 else
+              rw.gmm_points(int(n), (7 << 20) + s, 16, 10.24) for s, n in enumerate(sizes)]
+    b = rw.Batch(clouds)
+    del clouds
+    total = int(sizes.sum())
+    stream = rw.Stream()
+    e0, e1 = rw.Event(), rw.Event()
+    rw.check(L.rwh_batch_prepare(b.ws.ptr, b.sizes.ctypes.data, b.pb.ctypes.data, b.ob.ctypes.data,
+                                 b.oc.ctypes.data, 512, stream.ptr), "prepare")
+
+    def timed(fn, n_reps):
+        fn()
+        stream.sync()
+        e0.record(stream.ptr)
+        for _ in range(n_reps):
+            fn()
+        e1.record(stream.ptr)
+        e1.sync()
+        return e0.elapsed_ms(e1) / n_reps
+
+    t_enqueue = timed(lambda: rw.check(L.rwh_build_octree_batch_enqueue(
+        b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr), "batch"), reps)
+    g = C.c_void_p()
+    rw.check(L.rwh_batch_graph_create(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr, C.byref(g)), "graph")
+    t_graph = timed(lambda: rw.check(L.rwh_graph_launch(g, stream.ptr), "graph launch"), reps)
+    batched = {k: v.copy() for k, v in zip(("n_unique", "n_oct"), b.ws.counts(512))}
+    snap = {k: b.tb[k].numpy() for k in ("morton_keys_alt", "parents", "edge_offset", "children", "corner")}
+
+    # plain: one rwh_build_octree per cloud, outputs written to the same places
+    ws1 = rw.Workspace(int(sizes.max()))
+    base = b.tb.struct
+    views = []
+    for s in range(512):
+        v = rw.TreeBuffers()
+        for name, (dt, k, per_point) in rw.TREE_FIELDS.items():
+            off = (int(b.pb[s]) if per_point else int(b.ob[s])) * k * np.dtype(dt).itemsize
+            setattr(v, name, getattr(base, name) + off)
+        v.points = base.points + int(b.pb[s]) * 16
+        v.oct_capacity = int(b.oc[s])
+        views.append(v)
+
+    def plain_all():
+        for s in range(512):
+            rw.check(L.rwh_build_octree(ws1.ptr, C.addressof(views[s]), int(sizes[s]), 0.0, 1024.0,
+                                        stream.ptr), "plain")
+
+    t_plain = timed(plain_all, max(1, reps // 2))
+    # equal output: the plain builds rewrote every array the batched build wrote
+    same = all(np.array_equal(snap[k], b.tb[k].numpy()) for k in snap)
+    rw.check(L.rwh_graph_destroy(g), "graph destroy")
+    return {"workload": "512 clouds, sizes log-uniform 2^10..2^20 (seed 7), half uniform, "
+                        "half 16-cluster Gaussian sigma 10.24, one segmented build",
+            "points": total, "ms_batched": round(t_enqueue, 3), "ms_batched_graph": round(t_graph, 3),
+            "ms_plain": round(t_plain, 3),
+            "mpts_per_s_batched": round(total / (min(t_enqueue, t_graph) * 1e-3) / 1e6, 1),
+            "mpts_per_s_plain": round(total / (t_plain * 1e-3) / 1e6, 1),
+            "speedup": round(t_plain / min(t_enqueue, t_graph), 2), "equal_output": bool(same),
+            "octree_nodes": int(batched["n_oct"].sum())}
+
+
+def serving_section(rw, rates=(200, 1000, 5000), seconds=2.0, max_requests=4000):
+    """Open-loop Poisson request streams against the continuous-batching server
+    and against one plain build per request."""
+    sizes = rw.log_uniform_sizes(11, 32, 12, 22)
+    pinned = []
+    for i, n in enumerate(sizes):
+        p = rw.PinnedArray((int(n), 4))
+        p.array[:] = (rw.uniform_points(int(n), 900 + i) if i % 2 == 0 else
+                      rw.gmm_points(int(n), 900 + i, 16, 10.24))
+        pinned.append(p)
+    clouds = [p.array for p in pinned]
+    out = {"workload": "Poisson arrivals; 32 clouds with sizes log-uniform 2^12..2^22 (seed 11), "
+                       "half uniform, half 16-cluster Gaussian; pinned host points uploaded per "
+                       "request; latency = arrival -> counts on host",
+           "pool_cap_GB": 64, "rates": {}}
+    srv = rw.Server(pool_bytes=64 << 30, n_slots=3, slot_points=1 << 26, max_clouds=512)
+    plain = rw.Server(pool_bytes=64 << 30, n_slots=1, slot_points=1 << 23, plain=True)
+    for r in rates:
+        n_req = int(min(max_requests, max(50, r * seconds)))
+        a = srv.load(clouds, float(r), n_req, seed=r)
+        p = plain.load(clouds, float(r), n_req, seed=r)
+        out["rates"][str(r)] = {
+            "requests": n_req,
+            "batched": {k: round(a[k], 3) for k in ("throughput_rps", "mpts_per_s", "p50_ms", "p99_ms",
+                                                    "mean_batch", "batches", "rejected")},
+            "plain": {k: round(p[k], 3) for k in ("throughput_rps", "mpts_per_s", "p50_ms", "p99_ms",
+                                                  "rejected")}}
+    srv.close()
+    plain.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +191,8 @@ def main():
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (clouds and requests "
@@ -165,6 +266,10 @@ def main():
         "stages": stages,
         "device": rw.device_name(),
     }
+    if not args.no_batched:
+        out["batched"] = batched_section(rw)
+    if not args.no_serving:
+        out["serving"] = serving_section(rw)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pts)
     print(json.dumps(out))

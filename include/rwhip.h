@@ -183,6 +183,26 @@ int rwh_build_octree_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, cons
                            const uint32_t* point_base, const uint32_t* oct_base,
                            const uint32_t* oct_cap, uint32_t n_clouds, float min_coord,
                            float range, rwh_stream_t stream);
+/* The same build split in two, for callers that replay it: rwh_batch_prepare
+ * validates the layout and uploads the per-cloud tables to the workspace
+ * (waiting for the copy); rwh_build_octree_batch_enqueue then enqueues only
+ * kernels (no copy, no host wait) for the last prepared layout, so it can be
+ * captured into a graph.  Replays see the tables' current contents. */
+int rwh_batch_prepare(rwh_workspace* ws, const uint32_t* n, const uint32_t* point_base,
+                      const uint32_t* oct_base, const uint32_t* oct_cap, uint32_t n_clouds,
+                      rwh_stream_t stream);
+int rwh_build_octree_batch_enqueue(rwh_workspace* ws, const rwh_tree_buffers* bufs,
+                                   float min_coord, float range, rwh_stream_t stream);
+
+/* HIP graph of rwh_build_octree_batch_enqueue for the prepared layout
+ * (captured on a non-default stream), its launch and its release.  The
+ * kernels reset their own look-back state, so the graph replays as is. */
+typedef void* rwh_graph_t; /* a hipGraphExec_t */
+int rwh_batch_graph_create(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coord,
+                           float range, rwh_stream_t stream, rwh_graph_t* graph);
+int rwh_graph_launch(rwh_graph_t graph, rwh_stream_t stream);
+int rwh_graph_destroy(rwh_graph_t graph);
+
 /* Waits and copies the counts of every cloud: n_unique to counts[s],
  * n_octree_nodes to counts[n_clouds + s]. */
 int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts,

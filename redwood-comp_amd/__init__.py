@@ -81,6 +81,11 @@ ABI = {
     "rwh_batch_layout": (_I, [_P, _U32, _P, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rwh_build_octree_batch": (_I, [_P, _P, _P, _P, _P, _P, _U32, _F, _F, _P]),
     "rwh_read_batch_counts": (_I, [_P, _U32, _P, _P]),
+    "rwh_batch_prepare": (_I, [_P, _P, _P, _P, _P, _U32, _P]),
+    "rwh_build_octree_batch_enqueue": (_I, [_P, _P, _F, _F, _P]),
+    "rwh_batch_graph_create": (_I, [_P, _P, _F, _F, _P, C.POINTER(_P)]),
+    "rwh_graph_launch": (_I, [_P, _P]),
+    "rwh_graph_destroy": (_I, [_P]),
     "rwh_debug_fail_next": (None, [_I]),
 }
 
@@ -369,6 +374,31 @@ class TreeBuild:
         return out
 
 
+class PinnedArray:
+    """Page-locked host memory viewed as a numpy array (fast H2D uploads)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        p = _P()
+        check(lib().rwh_host_alloc(C.byref(p), max(nbytes, 16)), "rwh_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * max(nbytes, 16)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().rwh_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
                  min_coord: float = MIN_COORD, range_: float = RANGE):
     """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs)."""
@@ -380,7 +410,8 @@ def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
     counts = (_U32 * 3)()
     check(lib().rwh_read_counts(ws.ptr, cap, counts, stream), "rwh_read_counts")
     n_unique, n_brt, n_oct = counts[0], counts[1], counts[2]
-    out = tb.read(n, min(n_oct, cap))
+    # octree rows [0, n_oct); slot 0 also when no node contributes (it holds the root cell)
+    out = tb.read(n, min(max(n_oct, 1), cap))
     return (n_unique, n_brt, n_oct), out
 
 

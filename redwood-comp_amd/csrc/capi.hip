@@ -99,6 +99,8 @@ struct rwh_workspace {
   uint32_t* h_counts = nullptr;  // pinned counts read-back
   hipEvent_t upload_done = nullptr;
   bool upload_pending = false;
+  uint32_t batch_clouds = 0;  // shape of the last staged batch
+  uint32_t batch_index = 0;
 
   uint32_t* seg_base() { return tables; }
   uint32_t* seg_n() { return tables + max_segs; }
@@ -570,53 +572,11 @@ int rwh_batch_layout(const uint32_t* n, uint32_t n_clouds, uint32_t* point_base,
   return RWH_OK;
 }
 
-int rwh_build_octree_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, const uint32_t* n,
-                           const uint32_t* point_base, const uint32_t* oct_base,
-                           const uint32_t* oct_cap, uint32_t n_clouds, float min_coord,
-                           float range, rwh_stream_t stream) {
-  RWH_CHECK(ws, "null workspace");
-  RWH_TRY(check_bufs(bufs));
-  RWH_CHECK(n && point_base && oct_base && oct_cap, "null layout array");
-  RWH_CHECK(n_clouds >= 1, "empty batch");
-  if (n_clouds > ws->max_segs)
-    return fail(RWH_ERR_CAPACITY, "batch of " + std::to_string(n_clouds) +
-                                      " clouds exceeds the workspace's " + std::to_string(ws->max_segs));
-  uint64_t space = 0;
-  for (uint32_t s = 0; s < n_clouds; ++s) {
-    RWH_CHECK(point_base[s] % kChunk == 0, "point_base must be a multiple of 8192");
-    RWH_CHECK(s == 0 || point_base[s] > point_base[s - 1], "point_base must increase");
-    const uint64_t chunks = std::max<uint64_t>(1, (n[s] + kChunk - 1) / kChunk);
-    space = std::max<uint64_t>(space, point_base[s] + chunks * kChunk);
-  }
-  if (space > ws->max_points)
-    return fail(RWH_ERR_CAPACITY, "batch index space " + std::to_string(space) +
-                                      " exceeds the workspace capacity " + std::to_string(ws->max_points));
-  const uint32_t n_index = static_cast<uint32_t>(space);
-  const hipStream_t st = static_cast<hipStream_t>(stream);
+namespace {
 
-  // Segment tables through the pinned staging area (reused only after the
-  // previous upload has completed).
-  if (ws->upload_pending) RWH_HIP(hipEventSynchronize(ws->upload_done));
-  uint32_t* h = ws->h_stage;
-  const uint32_t S = ws->max_segs;
-  for (uint32_t s = 0; s < n_clouds; ++s) {
-    h[s] = point_base[s];
-    h[S + s] = n[s];
-    h[2 * S + s] = oct_base[s];
-    h[3 * S + s] = oct_cap[s];
-  }
-  uint32_t* cs = h + 4 * S;
-  const uint32_t n_chunks = n_index / kChunk;
-  for (uint32_t s = 0; s < n_clouds; ++s) {
-    const uint32_t c0 = point_base[s] / kChunk;
-    const uint32_t c1 = (s + 1 < n_clouds) ? point_base[s + 1] / kChunk : n_chunks;
-    for (uint32_t c = c0; c < c1; ++c) cs[c] = s;
-  }
-  RWH_HIP(hipMemcpyAsync(ws->tables, h, sizeof(uint32_t) * 4 * S, hipMemcpyHostToDevice, st));
-  RWH_HIP(hipMemcpyAsync(ws->chunk_seg(), cs, sizeof(uint32_t) * n_chunks, hipMemcpyHostToDevice, st));
-  RWH_HIP(hipEventRecord(ws->upload_done, st));
-  ws->upload_pending = true;
-
+int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coord, float range,
+                  hipStream_t st) {
+  const uint32_t n_clouds = ws->batch_clouds, n_index = ws->batch_index;
   SegArgs pts{};
   pts.n_segs = n_clouds;
   pts.base = ws->seg_base();
@@ -645,6 +605,120 @@ int rwh_build_octree_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, cons
   oa.oct_cap = ws->oct_cap();
   launch_octree(oa, uni, n_index, st);
   return launched();
+}
+
+// Validates a batch layout and stages its tables; the copy to the device is
+// enqueued on st (sync: also waited for).
+int stage_batch(rwh_workspace* ws, const uint32_t* n, const uint32_t* point_base,
+                const uint32_t* oct_base, const uint32_t* oct_cap, uint32_t n_clouds,
+                hipStream_t st, bool sync) {
+  RWH_CHECK(ws, "null workspace");
+  RWH_CHECK(n && point_base && oct_base && oct_cap, "null layout array");
+  RWH_CHECK(n_clouds >= 1, "empty batch");
+  if (n_clouds > ws->max_segs)
+    return fail(RWH_ERR_CAPACITY, "batch of " + std::to_string(n_clouds) +
+                                      " clouds exceeds the workspace's " + std::to_string(ws->max_segs));
+  uint64_t space = 0;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    RWH_CHECK(point_base[s] % kChunk == 0, "point_base must be a multiple of 8192");
+    RWH_CHECK(s == 0 || point_base[s] > point_base[s - 1], "point_base must increase");
+    const uint64_t chunks = std::max<uint64_t>(1, (n[s] + kChunk - 1) / kChunk);
+    RWH_CHECK(s + 1 == n_clouds || point_base[s] + chunks * kChunk <= point_base[s + 1],
+              "clouds overlap in the index space");
+    space = std::max<uint64_t>(space, point_base[s] + chunks * kChunk);
+  }
+  if (space > ws->max_points)
+    return fail(RWH_ERR_CAPACITY, "batch index space " + std::to_string(space) +
+                                      " exceeds the workspace capacity " + std::to_string(ws->max_points));
+  const uint32_t n_index = static_cast<uint32_t>(space);
+  // the pinned staging area is reused only after the previous upload completed
+  if (ws->upload_pending) RWH_HIP(hipEventSynchronize(ws->upload_done));
+  uint32_t* h = ws->h_stage;
+  const uint32_t S = ws->max_segs;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    h[s] = point_base[s];
+    h[S + s] = n[s];
+    h[2 * S + s] = oct_base[s];
+    h[3 * S + s] = oct_cap[s];
+  }
+  uint32_t* cs = h + 4 * S;
+  const uint32_t n_chunks = n_index / kChunk;
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    const uint32_t c0 = point_base[s] / kChunk;
+    const uint32_t c1 = (s + 1 < n_clouds) ? point_base[s + 1] / kChunk : n_chunks;
+    for (uint32_t c = c0; c < c1; ++c) cs[c] = s;
+  }
+  RWH_HIP(hipMemcpyAsync(ws->tables, h, sizeof(uint32_t) * 4 * S, hipMemcpyHostToDevice, st));
+  RWH_HIP(hipMemcpyAsync(ws->chunk_seg(), cs, sizeof(uint32_t) * n_chunks, hipMemcpyHostToDevice, st));
+  RWH_HIP(hipEventRecord(ws->upload_done, st));
+  ws->upload_pending = true;
+  if (sync) RWH_HIP(hipEventSynchronize(ws->upload_done));
+  ws->batch_clouds = n_clouds;
+  ws->batch_index = n_index;
+  return RWH_OK;
+}
+
+}  // namespace
+
+int rwh_build_octree_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, const uint32_t* n,
+                           const uint32_t* point_base, const uint32_t* oct_base,
+                           const uint32_t* oct_cap, uint32_t n_clouds, float min_coord,
+                           float range, rwh_stream_t stream) {
+  RWH_TRY(check_bufs(bufs));
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  RWH_TRY(stage_batch(ws, n, point_base, oct_base, oct_cap, n_clouds, st, false));
+  return enqueue_batch(ws, bufs, min_coord, range, st);
+}
+
+int rwh_batch_prepare(rwh_workspace* ws, const uint32_t* n, const uint32_t* point_base,
+                      const uint32_t* oct_base, const uint32_t* oct_cap, uint32_t n_clouds,
+                      rwh_stream_t stream) {
+  return stage_batch(ws, n, point_base, oct_base, oct_cap, n_clouds,
+                     static_cast<hipStream_t>(stream), true);
+}
+
+int rwh_build_octree_batch_enqueue(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coord,
+                                   float range, rwh_stream_t stream) {
+  RWH_CHECK(ws, "null workspace");
+  RWH_TRY(check_bufs(bufs));
+  RWH_CHECK(ws->batch_clouds > 0, "no batch prepared on this workspace");
+  return enqueue_batch(ws, bufs, min_coord, range, static_cast<hipStream_t>(stream));
+}
+
+int rwh_batch_graph_create(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coord,
+                           float range, rwh_stream_t stream, rwh_graph_t* graph) {
+  RWH_CHECK(ws && graph, "null argument");
+  RWH_TRY(check_bufs(bufs));
+  RWH_CHECK(ws->batch_clouds > 0, "no batch prepared on this workspace");
+  *graph = nullptr;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  RWH_CHECK(st != nullptr, "graph capture needs a non-default stream");
+  RWH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_batch(ws, bufs, min_coord, range, st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != RWH_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  RWH_HIP(e);
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  RWH_HIP(e2);
+  *graph = ex;
+  return RWH_OK;
+}
+
+int rwh_graph_launch(rwh_graph_t graph, rwh_stream_t stream) {
+  RWH_CHECK(graph, "null graph");
+  RWH_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), static_cast<hipStream_t>(stream)));
+  return RWH_OK;
+}
+
+int rwh_graph_destroy(rwh_graph_t graph) {
+  if (graph) RWH_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+  return RWH_OK;
 }
 
 int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts,
