@@ -206,7 +206,9 @@ def main():
     n = 1 << args.log2n
     pts = rw.uniform_points(n, 114514)
     cap = rw.octree_capacity(n)
-    ws = rw.Workspace(n)
+    variant_ms = rw.measure_sort_variants(n, 5)
+    variant = int(np.argmin(variant_ms)) if variant_ms else 0
+    ws = rw.Workspace(n, sort_variant=variant)
     tb = rw.TreeBuild(n, cap, rw.DeviceArray.from_numpy(pts.reshape(-1)))
     stream = rw.Stream()
     L = rw.lib()
@@ -264,6 +266,8 @@ def main():
                      "achieved": sort["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sort["frac"], "traffic": None},
         "stages": stages,
+        "sort_variants": {f"{b}x{k}": round(ms, 4) for (b, k), ms in zip(rw.sort_variants(), variant_ms)},
+        "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
         "device": rw.device_name(),
     }
     if not args.no_batched:

@@ -80,6 +80,13 @@ size_t rwh_workspace_bytes(const rwh_workspace* ws);
 uint32_t* rwh_workspace_counts(rwh_workspace* ws);
 uint32_t rwh_workspace_max_segments(const rwh_workspace* ws);
 
+/* Tile shapes of the sort's digit passes: variant v runs workgroups of
+ * block threads with keys_per_thread keys each.  A workspace uses variant 0
+ * until told otherwise; the host layer's autotuner picks one per size. */
+int rwh_sort_variant_count(void);
+int rwh_sort_variant_shape(int variant, uint32_t* block, uint32_t* keys_per_thread);
+int rwh_workspace_set_sort_variant(rwh_workspace* ws, int variant);
+
 /* ---- S1  (reference: cpu::run_stage1 host_dispatchers.cpp:17-41,
  *          cuda::run_stage1 cu_dispatcher.cu:501-522)
  * codes[i] = Morton code of float4 point i (x,y,z used), 10 bits per axis.

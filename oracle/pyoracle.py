@@ -175,3 +175,13 @@ def build(pool, pts, min_coord=0.0, range_=1024.0):
                             _p(o["cell_size"]), _p(o["child_node_mask"]), _p(o["child_leaf_mask"]),
                             _p(counts), _p(ms))
     return (int(counts[0]), int(counts[1]), int(counts[2]), rc), o, ms
+
+
+def link_leaf(pool, o, counts):
+    """Runs oracle_link_leaf in place on the outputs o of build()."""
+    nu, nb, no = counts[:3]
+    lib().oracle_link_leaf(pool.ptr, _p(o["edge_offset"]), _p(o["edge_count"]), _p(o["unique"]),
+                           _p(o["has_leaf_left"]), _p(o["has_leaf_right"]), _p(o["prefix_n"]),
+                           _p(o["parents"]), _p(o["left_child"]), nb,
+                           min(no, o["cell_size"].shape[0]), _p(o["children"]), _p(o["child_leaf_mask"]))
+    return o

@@ -68,6 +68,9 @@ ABI = {
     "rwh_workspace_bytes": (_SZ, [_P]),
     "rwh_workspace_counts": (_P, [_P]),
     "rwh_workspace_max_segments": (_U32, [_P]),
+    "rwh_sort_variant_count": (_I, []),
+    "rwh_sort_variant_shape": (_I, [_I, C.POINTER(_U32), C.POINTER(_U32)]),
+    "rwh_workspace_set_sort_variant": (_I, [_P, _I]),
     "rwh_morton": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_sort_u32": (_I, [_P, _P, _P, _U32, _U32, _P]),
     "rwh_unique_u32": (_I, [_P, _P, _U32, _P, _P, _P]),
@@ -97,6 +100,8 @@ HOST_ABI = {
     "rwt_log_uniform_sizes": (None, [C.c_uint64, _U32, C.c_double, C.c_double, _P]),
     "rwt_dropin_build": (_I, [_P, _U32, _I] + [_P] * 14 + [_P]),
     "rwt_plan_batch": (_U32, [C.c_uint64, _U32, _P, _U32]),
+    "rwt_best_sort_variant": (_I, [_U32]),
+    "rwt_measure_sort_variants": (_I, [_U32, _I, _P, _I]),
     "rwt_server_create": (_P, [C.c_uint64, _U32, _U32, _U32, _I]),
     "rwt_server_destroy": (None, [_P]),
     "rwt_server_slots": (_U32, [_P]),
@@ -249,11 +254,16 @@ class DeviceArray:
 
 
 class Workspace:
-    def __init__(self, max_points: int, max_segments: int = 1):
+    def __init__(self, max_points: int, max_segments: int = 1, sort_variant: int | None = None):
         p = _P()
         check(lib().rwh_workspace_create(max_points, max_segments, C.byref(p)), "rwh_workspace_create")
         self.ptr = p.value
         self.max_segments = max_segments
+        if sort_variant is not None:
+            self.set_sort_variant(sort_variant)
+
+    def set_sort_variant(self, v: int):
+        check(lib().rwh_workspace_set_sort_variant(self.ptr, v), "rwh_workspace_set_sort_variant")
 
     @property
     def counts_ptr(self) -> int:
@@ -400,15 +410,23 @@ class PinnedArray:
 
 
 def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
-                 min_coord: float = MIN_COORD, range_: float = RANGE):
-    """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs)."""
+                 min_coord: float = MIN_COORD, range_: float = RANGE, link_leaf: bool = False,
+                 strict: bool = True):
+    """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs).
+    link_leaf also runs the opt-in leaf linking after S7.  strict=False returns the
+    (capacity-clamped) outputs when n_octree_nodes exceeds the 0.6 n capacity
+    instead of raising."""
     n = points.shape[0]
     cap = octree_capacity(n)
     ws = ws or Workspace(max(n, 1))
     tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
     check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
+    if link_leaf:
+        check(lib().rwh_link_leaf(tb.ptr(), ws.n_unique_ptr(), ws.n_oct_ptr(), n, stream), "rwh_link_leaf")
     counts = (_U32 * 3)()
-    check(lib().rwh_read_counts(ws.ptr, cap, counts, stream), "rwh_read_counts")
+    st = lib().rwh_read_counts(ws.ptr, cap, counts, stream)
+    if strict or st != RWH_ERR_CAPACITY:
+        check(st, "rwh_read_counts")
     n_unique, n_brt, n_oct = counts[0], counts[1], counts[2]
     # octree rows [0, n_oct); slot 0 also when no node contributes (it holds the root cell)
     out = tb.read(n, min(max(n_oct, 1), cap))
@@ -460,7 +478,8 @@ class Batch:
         for s in range(self.sizes.size):
             b, n, ob = int(self.pb[s]), int(self.sizes[s]), int(self.ob[s])
             nb = max(int(nu[s]) - 1, 0)
-            rows = min(int(no[s]), int(self.oc[s]))
+            # slot 0 too when no node contributes (it holds the root cell)
+            rows = min(max(int(no[s]), 1), int(self.oc[s]))
             r = {"n_unique": int(nu[s]), "n_oct": int(no[s]),
                  "morton_keys_alt": full["morton_keys_alt"][b:b + n],
                  "unique": full["morton_keys"][b:b + int(nu[s])]}
@@ -567,3 +586,23 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def sort_variants():
+    """(block, keys_per_thread) of every compiled sort tile shape."""
+    out = []
+    for v in range(lib().rwh_sort_variant_count()):
+        b, k = _U32(0), _U32(0)
+        check(lib().rwh_sort_variant_shape(v, C.byref(b), C.byref(k)), "rwh_sort_variant_shape")
+        out.append((b.value, k.value))
+    return out
+
+
+def best_sort_variant(n: int) -> int:
+    return int(host().rwt_best_sort_variant(n))
+
+
+def measure_sort_variants(n: int, reps: int = 5) -> list:
+    buf = np.zeros(16, np.float64)
+    k = host().rwt_measure_sort_variants(n, reps, buf.ctypes.data, 16)
+    return buf[:k].tolist()

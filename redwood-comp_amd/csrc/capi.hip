@@ -101,6 +101,7 @@ struct rwh_workspace {
   bool upload_pending = false;
   uint32_t batch_clouds = 0;  // shape of the last staged batch
   uint32_t batch_index = 0;
+  int sort_variant = 0;       // index into kSortVariants
 
   uint32_t* seg_base() { return tables; }
   uint32_t* seg_n() { return tables + max_segs; }
@@ -124,7 +125,8 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
   L.keys_tmp = ws->keys_tmp;
   L.seg = seg;
   L.n_segs = n_segs;
-  L.n_tiles = (n_index + kSortTile - 1) / kSortTile;
+  L.n_index = n_index;
+  L.variant = ws->sort_variant;
   if (plain) {
     L.hist_unit_keys = kPlainHistUnit;
   } else {
@@ -136,7 +138,7 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
   L.digit_start = ws->digit_start;
   L.lookback = ws->lb_sort;
   L.ctrl = ws->ctrl + 0 * kCtrlWords;
-  if (L.n_tiles == 0) return RWH_OK;
+  if (n_index == 0) return RWH_OK;
   launch_sort(L, st);
   return launched();
 }
@@ -337,7 +339,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   ws->max_segs = max_segments;
   const uint32_t mp = std::max<uint32_t>(max_points, 1);
   ws->max_chunks = (mp + kChunk - 1) / kChunk;
-  ws->max_sort_tiles = (mp + kSortTile - 1) / kSortTile;
+  ws->max_sort_tiles = (mp + kSortMinTile - 1) / kSortMinTile;
   ws->max_scan_tiles = (mp + kScanTile - 1) / kScanTile;
   // carve one allocation
   size_t off = 0;
@@ -402,6 +404,23 @@ int rwh_workspace_destroy(rwh_workspace* ws) {
 }
 
 size_t rwh_workspace_bytes(const rwh_workspace* ws) { return ws ? ws->bytes : 0; }
+
+int rwh_sort_variant_count(void) { return kNumSortVariants; }
+
+int rwh_sort_variant_shape(int variant, uint32_t* block, uint32_t* keys_per_thread) {
+  RWH_CHECK(variant >= 0 && variant < kNumSortVariants, "no such sort variant");
+  RWH_CHECK(block && keys_per_thread, "null argument");
+  *block = static_cast<uint32_t>(kSortVariants[variant].block);
+  *keys_per_thread = static_cast<uint32_t>(kSortVariants[variant].kpt);
+  return RWH_OK;
+}
+
+int rwh_workspace_set_sort_variant(rwh_workspace* ws, int variant) {
+  RWH_CHECK(ws, "null workspace");
+  RWH_CHECK(variant >= 0 && variant < kNumSortVariants, "no such sort variant");
+  ws->sort_variant = variant;
+  return RWH_OK;
+}
 uint32_t* rwh_workspace_counts(rwh_workspace* ws) { return ws ? ws->counts : nullptr; }
 uint32_t rwh_workspace_max_segments(const rwh_workspace* ws) { return ws ? ws->max_segs : 0; }
 

@@ -86,19 +86,16 @@ __global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg
     }
     // row of this slot: the bottom is cleared; others hold the child below
     if (slot >= 0 && slot < rows) {
-      int4 lo = make_int4(0, 0, 0, 0), hi = make_int4(0, 0, 0, 0);
-      int32_t m = 0;
-      if (j > 0) {
-        const int below_level = lvl - (j - 1);
-        const uint32_t which = (ci >> level_shift(below_level)) & 7u;
-        const int v = static_cast<int>(slot - 1);
-        int* e = (which < 4) ? &lo.x : &hi.x;
-        e[which & 3u] = v;
-        m = static_cast<int32_t>(1u << which);
-      }
-      rows8[slot * 2] = lo;
-      rows8[slot * 2 + 1] = hi;
-      mask[slot] = m;
+      // which = 8 marks "no child" (the bottom slot)
+      uint32_t which = 8u;
+      if (j > 0) which = (ci >> level_shift(lvl - (j - 1))) & 7u;
+      const int v = static_cast<int>(slot - 1);
+      int r[8];
+#pragma unroll
+      for (uint32_t c = 0; c < 8; ++c) r[c] = (which == c) ? v : 0;
+      rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
+      rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
+      mask[slot] = (which < 8u) ? static_cast<int32_t>(1u << which) : 0;
       leaf_mask[slot] = 0;
     }
   }

@@ -117,11 +117,11 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
 // tile + w*KPT*64 + k*64 + l), which is the order the ranking walks, so the
 // sort is stable.
 // ---------------------------------------------------------------------------
-template <int BLOCK, int KPT>
+template <int BLOCK, int KPT, int BITS>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg, uint32_t n_tiles,
     const uint32_t* __restrict__ digit_start, uint32_t pass, uint32_t n_passes, uint32_t shift,
-    uint32_t digit_bits, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
+    uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   }
   const uint32_t n_valid = min(TILE, lim - first);
   const uint32_t tile_in_seg = (first - sg.base) / TILE;
-  const uint32_t dmask = (1u << digit_bits) - 1u;
+  constexpr uint32_t dmask = (1u << BITS) - 1u;
 
   const uint32_t wave = tid / kWave;
   const uint32_t lane = tid % kWave;
@@ -167,19 +167,23 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   }
 
   // Wave-level multisplit: lanes holding the same digit find each other with
-  // digit_bits ballots; the lowest lane of each group advances the wave's
-  // counter for that digit.
+  // BITS ballots (a lane is a peer unless some ballot differs from this lane's
+  // own bit); the lowest lane of each group advances the wave's counter for
+  // that digit.
   uint32_t* wh = s_whist[wave];
   uint32_t rank[KPT];
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const uint32_t d = (keys[k] >> shift) & dmask;
-    uint64_t m = ~0ull;
-    for (uint32_t b = 0; b < digit_bits; ++b) {
-      const uint32_t bit = (d >> b) & 1u;
-      const uint64_t bal = ballot(bit != 0);
-      m &= bit ? bal : ~bal;
+    uint32_t mis_lo = 0, mis_hi = 0;  // lanes whose digit differs in some bit
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const uint64_t bal = ballot(((d >> b) & 1u) != 0);
+      const uint32_t own = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+      mis_lo |= static_cast<uint32_t>(bal) ^ own;
+      mis_hi |= static_cast<uint32_t>(bal >> 32) ^ own;
     }
+    const uint64_t m = ~((static_cast<uint64_t>(mis_hi) << 32) | mis_lo);
     const uint32_t below = popc_below(m);
     const uint32_t old = wh[d];
     if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t d = tid;
     uint32_t excl = 0;
     if (tile_in_seg > 0) {
-      excl = lb_exclusive(lookback + d, 256, tile, tile_in_seg, epoch);
+      excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
       lb_store(&lookback[static_cast<size_t>(tile) * 256 + d], lb_pack(epoch, kIncl, excl + cnt_valid));
     }
     s_gbase[d] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + d] +
@@ -239,7 +243,32 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-static_assert(kSortBlock * kSortKpt == kSortTile, "tile shape");
+template <int BLOCK, int KPT>
+void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
+  constexpr uint32_t TILE = BLOCK * KPT;
+  static_assert(TILE >= kSortMinTile, "look-back table sized for kSortMinTile");
+  const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
+  // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an
+  // odd count starts with in -> out)
+  const uint32_t* src = L.keys_in;
+  for (uint32_t p = 0; p < n_passes; ++p) {
+    const bool to_out = ((n_passes - p) % 2) == 1;
+    uint32_t* dst = to_out ? L.keys_out : L.keys_tmp;
+    const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
+#define RW_ONESWEEP(B)                                                                       \
+  case B:                                                                                    \
+    hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_tiles), dim3(BLOCK), 0, stream, src, \
+                       dst, L.seg, n_tiles, L.digit_start, p, n_passes, 8 * p, L.lookback,     \
+                       L.ctrl);                                                              \
+    break;
+    switch (bits) {
+      RW_ONESWEEP(1) RW_ONESWEEP(2) RW_ONESWEEP(3) RW_ONESWEEP(4)
+      RW_ONESWEEP(5) RW_ONESWEEP(6) RW_ONESWEEP(7) default: RW_ONESWEEP(8)
+    }
+#undef RW_ONESWEEP
+    src = dst;
+  }
+}
 
 void launch_sort(const SortLaunch& L, hipStream_t stream) {
   const uint32_t n_passes = (L.key_bits + 7) / 8;
@@ -248,17 +277,13 @@ void launch_sort(const SortLaunch& L, hipStream_t stream) {
                      L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist);
   hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                      L.digit_start, n_passes);
-  // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an
-  // odd count starts with in -> out)
-  const uint32_t* src = L.keys_in;
-  for (uint32_t p = 0; p < n_passes; ++p) {
-    const bool to_out = ((n_passes - p) % 2) == 1;
-    uint32_t* dst = to_out ? L.keys_out : L.keys_tmp;
-    const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
-    hipLaunchKernelGGL((k_onesweep<kSortBlock, kSortKpt>), dim3(L.n_tiles), dim3(kSortBlock), 0,
-                       stream, src, dst, L.seg, L.n_tiles, L.digit_start, p, n_passes, 8 * p, bits,
-                       L.lookback, L.ctrl);
-    src = dst;
+  static_assert(kNumSortVariants == 5, "one case per variant");
+  switch (L.variant) {
+    case 1: launch_passes<kSortVariants[1].block, kSortVariants[1].kpt>(L, n_passes, stream); break;
+    case 2: launch_passes<kSortVariants[2].block, kSortVariants[2].kpt>(L, n_passes, stream); break;
+    case 3: launch_passes<kSortVariants[3].block, kSortVariants[3].kpt>(L, n_passes, stream); break;
+    case 4: launch_passes<kSortVariants[4].block, kSortVariants[4].kpt>(L, n_passes, stream); break;
+    default: launch_passes<kSortVariants[0].block, kSortVariants[0].kpt>(L, n_passes, stream); break;
   }
 }
 

@@ -59,21 +59,22 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
   return off + incl - v;
 }
 
-// Tile prefix over the segment's earlier tiles by decoupled look-back; thread 0
-// publishes and looks back, the result is broadcast through LDS.
+// Tile prefix over the segment's earlier tiles by decoupled look-back: thread 0
+// publishes the tile's aggregate, wave 0 walks back 64 tiles per round trip,
+// thread 0 publishes the inclusive prefix; the result is broadcast via LDS.
+// All threads must call it.
 __device__ __forceinline__ uint32_t tile_prefix(uint32_t tile_total, uint32_t tile,
                                                 uint32_t tile_in_seg, uint32_t epoch,
                                                 uint64_t* lookback, uint32_t* s_bcast) {
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
+    lb_store(&lookback[tile], lb_pack(epoch, tile_in_seg == 0 ? kIncl : kAgg, tile_total));
+  if (threadIdx.x < kWave) {
     uint32_t excl = 0;
-    if (tile_in_seg == 0) {
-      lb_store(&lookback[tile], lb_pack(epoch, kIncl, tile_total));
-    } else {
-      lb_store(&lookback[tile], lb_pack(epoch, kAgg, tile_total));
-      excl = lb_exclusive(lookback, 1, tile, tile_in_seg, epoch);
-      lb_store(&lookback[tile], lb_pack(epoch, kIncl, excl + tile_total));
+    if (tile_in_seg > 0) {
+      excl = lb_exclusive_wave(lookback, tile, tile_in_seg, epoch);
+      if (threadIdx.x == 0) lb_store(&lookback[tile], lb_pack(epoch, kIncl, excl + tile_total));
     }
-    *s_bcast = excl;
+    if (threadIdx.x == 0) *s_bcast = excl;
   }
   __syncthreads();
   return *s_bcast;

@@ -30,7 +30,7 @@ struct Seg {
   uint32_t id, base, count;
 };
 
-__device__ __forceinline__ Seg seg_of(const SegArgs& a, uint32_t index) {
+__device__ __forceinline__ Seg seg_of(SegArgs a, uint32_t index) {
   Seg s;
   if (a.n_segs == 1) {
     s.id = 0;
@@ -99,21 +99,64 @@ __device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
   return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Walks back from tile-1 over the n_prev predecessor tiles of the same
-// segment, accumulating aggregates until an inclusive prefix is found.
-// Every predecessor was handed out by the tile counter before this tile,
-// so it is resident and will publish: the spin ends.
+__device__ __forceinline__ bool lb_ready(uint64_t w, uint32_t epoch) {
+  return static_cast<uint32_t>(w >> 34) == epoch && ((w >> 32) & 3u) != 0;
+}
+
+// Exclusive prefix of one lane's column of look-back words: walks back from
+// tile-1 over the n_prev predecessor tiles of the same segment, W words at a
+// time (independent loads in flight), summing aggregates until an inclusive
+// prefix is found.  Every predecessor was handed out by the tile counter
+// before this tile, so it is resident and will publish: the spin ends.  The
+// segment's first tile always publishes an inclusive prefix.
+template <int W>
 __device__ __forceinline__ uint32_t lb_exclusive(const uint64_t* table, uint32_t stride,
                                                  uint32_t tile, uint32_t n_prev, uint32_t epoch) {
   uint32_t sum = 0;
-  for (uint32_t k = 1; k <= n_prev; ++k) {
-    const uint64_t* p = table + static_cast<size_t>(tile - k) * stride;
-    uint64_t w;
-    do {
-      w = lb_load(p);
-    } while (static_cast<uint32_t>(w >> 34) != epoch || ((w >> 32) & 3u) == 0);
-    sum += static_cast<uint32_t>(w);
-    if (((w >> 32) & 3u) == kIncl) break;
+  uint32_t k = 0;  // predecessors consumed
+  while (k < n_prev) {
+    uint64_t w[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+      w[j] = (k + j < n_prev) ? lb_load(table + static_cast<size_t>(tile - 1 - k - j) * stride) : 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      if (k + j >= n_prev) return sum;
+      const uint64_t* p = table + static_cast<size_t>(tile - 1 - k - j) * stride;
+      while (!lb_ready(w[j], epoch)) w[j] = lb_load(p);
+      sum += static_cast<uint32_t>(w[j]);
+      if (((w[j] >> 32) & 3u) == kIncl) return sum;
+    }
+    k += W;
+  }
+  return sum;
+}
+
+// The same walk for a single column done by a whole wave: lane l inspects
+// predecessor tile-1-(k+l), 64 words per round trip.  All lanes must call it;
+// every lane returns the exclusive prefix.
+__device__ __forceinline__ uint32_t lb_exclusive_wave(const uint64_t* table, uint32_t tile,
+                                                      uint32_t n_prev, uint32_t epoch) {
+  const uint32_t lane = lane_id();
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < n_prev; k += kWave) {
+    const bool live = k + lane < n_prev;
+    const uint64_t* p = table + (tile - 1 - k - (live ? lane : 0));
+    uint64_t w = live ? lb_load(p) : 0;
+    for (;;) {
+      const bool ready = !live || lb_ready(w, epoch);
+      const uint64_t incl = ballot(live && ready && ((w >> 32) & 3u) == kIncl);
+      // lanes that count: up to and including the nearest inclusive prefix
+      const uint64_t upto = incl ? (((incl & (~incl + 1)) << 1) - 1) : ~0ull;
+      const uint64_t waiting = ballot(!ready) & upto;
+      if (waiting == 0) {
+        const uint32_t v = (live && ((upto >> lane) & 1u)) ? static_cast<uint32_t>(w) : 0u;
+        sum += wave_sum(v);
+        if (incl) return sum;
+        break;
+      }
+      if (!ready) w = lb_load(p);
+    }
   }
   return sum;
 }

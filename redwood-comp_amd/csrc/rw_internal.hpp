@@ -9,14 +9,20 @@
 
 namespace rw {
 
-// Sort shape.  A tile is kSortBlock threads x kSortKpt keys; it divides kChunk.
-constexpr int kSortBlock = 256;
-constexpr int kSortKpt = 16;
-constexpr uint32_t kSortTile = kSortBlock * kSortKpt;  // 4096
+// Sort shapes.  A digit-pass tile is BLOCK threads x KPT keys and divides
+// kChunk; the variants below are compiled, one is chosen per call (the
+// autotuner of the host layer picks per size bucket).
+struct SortVariant {
+  int block, kpt;
+};
+constexpr SortVariant kSortVariants[] = {{256, 16}, {512, 16}, {256, 32}, {1024, 8}, {512, 8}};
+constexpr int kNumSortVariants = sizeof(kSortVariants) / sizeof(kSortVariants[0]);
+constexpr uint32_t kSortMinTile = 4096;  // smallest tile: sizes the look-back table
 constexpr int kHistBlock = 512;
 constexpr uint32_t kHistCopies = 16;
 constexpr uint32_t kMaxPasses = 4;
 constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
+constexpr int kLookbackWindow = 8;           // look-back words loaded per round trip
 
 // Compaction / scan tiles.
 constexpr int kScanBlock = 256;
@@ -32,7 +38,8 @@ struct SortLaunch {
   uint32_t* keys_tmp;
   SegArgs seg;
   uint32_t n_segs;
-  uint32_t n_tiles;        // tiles of kSortTile over the index space
+  int variant;              // index into kSortVariants
+  uint32_t n_index;        // index-space size (tiles are derived per variant)
   uint32_t hist_unit_keys;  // keys per histogram workgroup
   uint32_t n_hist_units;
   uint32_t key_bits;       // significant key bits (30 for Morton codes, 32 generic)

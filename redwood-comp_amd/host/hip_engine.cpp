@@ -3,6 +3,7 @@
 #include <new>
 #include <vector>
 
+#include "autotune.hpp"
 #include "inputs.hpp"
 
 namespace redwood::tree {
@@ -165,6 +166,8 @@ DeviceState& state_of(AppData& a) {
   if (!a.backend_state) {
     auto s = std::make_shared<DeviceState>();
     check(rwh_workspace_create(a.n_input, 1, &s->ws), "rwh_workspace_create");
+    check(rwh_workspace_set_sort_variant(s->ws, redwood::tune::best_sort_variant(a.n_input)),
+          "rwh_workspace_set_sort_variant");
     void* h = nullptr;
     check(rwh_host_alloc(&h, 2 * sizeof(uint32_t)), "rwh_host_alloc");
     s->h_counts = static_cast<uint32_t*>(h);

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "autotune.hpp"
 #include "build_server.hpp"
 #include "hip_engine.hpp"
 #include "inputs.hpp"
@@ -88,6 +89,19 @@ int rwt_dropin_build(const float* xyzw, uint32_t n, int fail_stage, uint32_t* so
     g_err = e.what();
     return stage_no == 0 ? -1 : stage_no;
   }
+}
+
+// ---- autotuning ----------------------------------------------------------------
+// Best sort tile variant for n keys (measured or read from the tune cache).
+int rwt_best_sort_variant(uint32_t n) { return redwood::tune::best_sort_variant(n); }
+
+// ms per sort of every variant at size n; returns the number of variants written.
+int rwt_measure_sort_variants(uint32_t n, int reps, double* ms_out, int cap) {
+  const auto t = redwood::tune::measure_sort_variants(n, reps);
+  int k = 0;
+  for (const auto& x : t)
+    if (k < cap) ms_out[k++] = x.ms;
+  return k;
 }
 
 // ---- serving ------------------------------------------------------------------

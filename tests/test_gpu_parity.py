@@ -54,9 +54,11 @@ def test_build_matches_oracle(gpu, orc, pool, n, kind):
     rw = gpu
     pts = inputs(rw, kind, n, seed=n + 17)
     print(f"case build kind={kind} n={n}", flush=True)
-    g_counts, g = rw.build_octree(pts)
+    g_counts, g = rw.build_octree(pts, strict=False)
     (nu, nb, no, rc), o, _ = orc.build(pool, pts)
-    assert rc == 0
+    # sparse clouds can need more octree nodes than the reference's 0.6 n slots:
+    # both sides then report it and keep the first 0.6 n slots (D11)
+    assert rc == (-1 if no > rw.octree_capacity(n) else 0)
     compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}")
 
 
@@ -153,6 +155,7 @@ def test_batch_equals_oracle_per_cloud(gpu, orc, pool):
     res = b.read()
     for s, c in enumerate(clouds):
         (nu, nb, no, rc), o, _ = orc.build(pool, c)
+        assert rc == (-1 if no > rw.octree_capacity(c.shape[0]) else 0)
         r = res[s]
         g = dict(r, morton_keys=r["unique"])
         compare(rw, c.shape[0], (r["n_unique"], max(r["n_unique"] - 1, 0), r["n_oct"]), g,
@@ -174,3 +177,17 @@ def test_batch_repeat_and_reuse(gpu):
             assert np.array_equal(res[s]["parents"], g["parents"][:max(g_counts[0] - 1, 0)])
             rows = min(g_counts[2], rw.octree_capacity(c.shape[0]))
             assert np.array_equal(bits(res[s]["children"]), bits(g["children"][:rows]))
+
+
+@pytest.mark.parametrize("kind,n", [("uniform", 50000), ("gmm", 200000), ("dups", 30000)])
+def test_link_leaf_matches_oracle(gpu, orc, pool, kind, n):
+    """Opt-in leaf linking (the reference's never-launched k_LinkLeafNodes)."""
+    rw = gpu
+    pts = inputs(rw, kind, n, seed=n)
+    g_counts, g = rw.build_octree(pts, link_leaf=True)
+    (nu, nb, no, rc), o, _ = orc.build(pool, pts)
+    before = o["children"].copy()
+    orc.link_leaf(pool, o, (nu, nb, no))
+    rows = min(no, rw.octree_capacity(n))
+    assert not np.array_equal(before[:rows], o["children"][:rows])  # leaves were linked
+    compare(rw, n, g_counts, g, (nu, nb, no), o, f"leaf {kind}/{n}")
