@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -239,6 +241,11 @@ def main():
         evs[7].sync()
         acc += [evs[i].elapsed_ms(evs[i + 1]) for i in range(7)]
     stage_ms = acc / args.stage_reps
+    if args.calib:
+        a = rw.DeviceArray(64 << 20, np.uint32, zero=True)
+        b = rw.DeviceArray(64 << 20, np.uint32, zero=True)
+        rw.check(L.rwh_memcpy_d2d(b.ptr, a.ptr, 256 << 20, stream.ptr), "calib copy")
+        stream.sync()
     sb = stage_bytes(n, nu, nb, no)
     stages = {}
     for (name, b), ms in zip(sb.items(), stage_ms):

@@ -160,10 +160,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   const uint32_t wbase = first + wave * (KPT * kWave) + lane;
 
   uint32_t keys[KPT];
+  if (n_valid == TILE) {
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const uint32_t idx = wbase + k * kWave;
-    keys[k] = (idx < lim) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
+    for (int k = 0; k < KPT; ++k) keys[k] = in[wbase + k * kWave];
+  } else {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const uint32_t idx = wbase + k * kWave;
+      keys[k] = (idx < lim) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
+    }
   }
 
   // Wave-level multisplit: lanes holding the same digit find each other with

@@ -120,13 +120,16 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
   uint32_t k[G][4];
   uint32_t flags[G];  // 4 head bits per group
   uint32_t cnt[G];
+  if (first + kScanTile <= lim) {  // full tile: plain 16-byte loads
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
-    if (i0 + 3 < lim) {
-      const uint4 q = *reinterpret_cast<const uint4*>(in + i0);
+    for (int g = 0; g < G; ++g) {
+      const uint4 q = *reinterpret_cast<const uint4*>(in + wbase + 4 * (g * kWave + lane));
       k[g][0] = q.x; k[g][1] = q.y; k[g][2] = q.z; k[g][3] = q.w;
-    } else {
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint32_t i0 = wbase + 4 * (g * kWave + lane);
 #pragma unroll
       for (int c = 0; c < 4; ++c) k[g][c] = (i0 + c < lim) ? in[i0 + c] : 0u;
     }
@@ -205,10 +208,24 @@ __device__ __forceinline__ int log2_ceil_u32(uint32_t x) {
   return fl + (((1u << fl) < x) ? 1 : 0);
 }
 
+// Codes of a segment, read through an LDS copy of the window around the
+// workgroup's nodes (the searches of most nodes stay inside it) and from
+// global memory outside it.
+struct CodeWindow {
+  const uint32_t* g;  // segment's codes
+  const uint32_t* s;  // LDS copy of g[lo, hi)
+  int lo, hi;
+  __device__ __forceinline__ uint32_t operator[](int k) const {
+    return (k >= lo && k < hi) ? s[k - lo] : g[k];
+  }
+};
+
+constexpr int kTreeHalo = 768;  // codes cached on each side of the workgroup's nodes
+
 // Split search of the reference (func_brt.hpp:90-104) for node i with range
 // length l, direction d; returns gamma.
-__device__ __forceinline__ int brt_split(const uint32_t* __restrict__ codes, int i, int n, int d,
-                                         int l, int dnode, uint32_t ci) {
+__device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, int d, int l,
+                                         int dnode, uint32_t ci) {
   int s = 0;
   const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
   const int scut = (d == -1) ? i - 1 : n - i - 1;
@@ -222,12 +239,19 @@ __device__ __forceinline__ int brt_split(const uint32_t* __restrict__ codes, int
 }
 
 __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs seg) {
+  __shared__ uint32_t s_codes[kElemBlock + 2 * kTreeHalo];
   const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
   const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
-  const int n = static_cast<int>(sg.count);  // n_brt
+  const int n = static_cast<int>(sg.count);  // n_brt; codes hold n + 1 keys
+  const int b0 = static_cast<int>(blockIdx.x * kElemBlock - sg.base);
+  if (b0 >= n) return;  // whole workgroup past the segment's nodes
+  const uint32_t* __restrict__ gcodes = a.codes + sg.base;
+  CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kElemBlock + kTreeHalo, n + 1)};
+  for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kElemBlock)
+    s_codes[k - codes.lo] = gcodes[k];
+  __syncthreads();
   const int i = static_cast<int>(gi - sg.base);
-  if (gi < sg.base || i >= n) return;
-  const uint32_t* __restrict__ codes = a.codes + sg.base;
+  if (i >= n) return;
   int32_t* __restrict__ parent = a.parent + sg.base;
   const uint32_t ci = codes[i];
 
@@ -336,18 +360,23 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
   const uint32_t wbase = first + wave * (kScanKpt * kWave);
   uint32_t v[G][4];
   uint32_t gsum[G];
+  const bool full = first + kScanTile <= lim;
+  if (full) {  // plain 16-byte loads
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
-    if (i0 + 3 < lim) {
-      const int4 q = *reinterpret_cast<const int4*>(in + i0);
+    for (int g = 0; g < G; ++g) {
+      const int4 q = *reinterpret_cast<const int4*>(in + wbase + 4 * (g * kWave + lane));
       v[g][0] = q.x; v[g][1] = q.y; v[g][2] = q.z; v[g][3] = q.w;
-    } else {
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint32_t i0 = wbase + 4 * (g * kWave + lane);
 #pragma unroll
       for (int c = 0; c < 4; ++c) v[g][c] = (i0 + c < lim) ? static_cast<uint32_t>(in[i0 + c]) : 0u;
     }
-    gsum[g] = v[g][0] + v[g][1] + v[g][2] + v[g][3];
   }
+#pragma unroll
+  for (int g = 0; g < G; ++g) gsum[g] = v[g][0] + v[g][1] + v[g][2] + v[g][3];
   uint32_t wave_run = 0;
   uint32_t gexcl[G];
 #pragma unroll
@@ -371,7 +400,7 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
       o[c] = r;
       r += v[g][c];
     }
-    if (i0 + 3 < lim) {
+    if (full) {
       *reinterpret_cast<int4*>(out + i0) = make_int4(static_cast<int>(o[0]), static_cast<int>(o[1]),
                                                      static_cast<int>(o[2]), static_cast<int>(o[3]));
     } else {

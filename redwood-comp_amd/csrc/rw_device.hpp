@@ -164,7 +164,11 @@ __device__ __forceinline__ uint32_t lb_exclusive_wave(const uint64_t* table, uin
 // Launch-serial bookkeeping of one look-back user: ctrl[0] tile counter,
 // ctrl[1] finished-workgroup counter, ctrl[2] epoch.  The last workgroup to
 // finish rewinds the counters and advances the epoch, so consecutive launches
-// (and graph replays) need no memset.
+// (and graph replays) need no memset.  No fence is needed: within a launch
+// workgroups exchange data only through the look-back words (device-scope
+// atomics), and the reset is read by the next launch, after the kernel
+// boundary.  (A device-scope fence here would write back the XCD's whole L2
+// once per workgroup.)
 constexpr int kCtrlWords = 4;
 
 __device__ __forceinline__ uint32_t epoch_of(const uint32_t* ctrl) {
@@ -172,13 +176,11 @@ __device__ __forceinline__ uint32_t epoch_of(const uint32_t* ctrl) {
 }
 
 __device__ __forceinline__ void finish_launch(uint32_t* ctrl, uint32_t n_groups) {
-  __threadfence();
   const uint32_t done = atomicAdd(&ctrl[1], 1u);
   if (done == n_groups - 1) {
     ctrl[0] = 0;
     ctrl[1] = 0;
     ctrl[2] = ctrl[2] + 1;
-    __threadfence();
   }
 }
 

@@ -176,7 +176,7 @@ def test_batch_repeat_and_reuse(gpu):
             assert res[s]["n_unique"] == g_counts[0] and res[s]["n_oct"] == g_counts[2]
             assert np.array_equal(res[s]["parents"], g["parents"][:max(g_counts[0] - 1, 0)])
             rows = min(g_counts[2], rw.octree_capacity(c.shape[0]))
-            assert np.array_equal(bits(res[s]["children"]), bits(g["children"][:rows]))
+            assert np.array_equal(bits(res[s]["children"][:rows]), bits(g["children"][:rows]))
 
 
 @pytest.mark.parametrize("kind,n", [("uniform", 50000), ("gmm", 200000), ("dups", 30000)])
