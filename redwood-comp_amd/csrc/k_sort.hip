@@ -120,129 +120,126 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
 template <int BLOCK, int KPT, int BITS>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg, uint32_t n_tiles,
-    const uint32_t* __restrict__ digit_start, uint32_t pass, uint32_t n_passes, uint32_t shift,
-    uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
+    uint32_t tiles_per_grab, const uint32_t* __restrict__ digit_start, uint32_t pass,
+    uint32_t n_passes, uint32_t shift, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
   static_assert(kChunk % TILE == 0, "tiles must not straddle segments");
+  constexpr uint32_t dmask = (1u << BITS) - 1u;
 
   __shared__ uint32_t s_stage[TILE];
   __shared__ uint32_t s_whist[WAVES][256];
   __shared__ uint32_t s_tstart[256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_tmp[4];
-  __shared__ uint32_t s_tile;
+  __shared__ Grab s_grab;
 
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
-  for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t epoch = epoch_of(ctrl);
-  const uint32_t first = tile * TILE;
-  Seg sg{0, 0, 0};
-  uint32_t lim = 0;
-  if (tile < n_tiles) {
-    sg = seg_of(seg, first);
-    lim = sg.base + sg.count;
-  }
-  if (tile >= n_tiles || first >= lim) {  // past the end, or a segment's padding
-    if (tid == 0) finish_launch(ctrl, gridDim.x);
-    return;
-  }
-  const uint32_t n_valid = min(TILE, lim - first);
-  const uint32_t tile_in_seg = (first - sg.base) / TILE;
-  constexpr uint32_t dmask = (1u << BITS) - 1u;
-
   const uint32_t wave = tid / kWave;
   const uint32_t lane = tid % kWave;
-  const uint32_t wbase = first + wave * (KPT * kWave) + lane;
+  if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
+  __syncthreads();
+  const uint32_t epoch = s_grab.epoch;
+  const uint32_t t0 = s_grab.index * tiles_per_grab;
+  const uint32_t t1 = min(t0 + tiles_per_grab, n_tiles);
 
-  uint32_t keys[KPT];
-  if (n_valid == TILE) {
+  for (uint32_t tile = t0; tile < t1; ++tile) {
+    const uint32_t first = tile * TILE;
+    const Seg sg = seg_of(seg, first);
+    const uint32_t lim = sg.base + sg.count;
+    if (first >= lim) continue;  // a segment's padding (uniform across the workgroup)
+    const uint32_t n_valid = min(TILE, lim - first);
+    const uint32_t tile_in_seg = (first - sg.base) / TILE;
+    const uint32_t wbase = first + wave * (KPT * kWave) + lane;
+
+    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
+    uint32_t keys[KPT];
+    if (n_valid == TILE) {
 #pragma unroll
-    for (int k = 0; k < KPT; ++k) keys[k] = in[wbase + k * kWave];
-  } else {
+      for (int k = 0; k < KPT; ++k) keys[k] = in[wbase + k * kWave];
+    } else {
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) {
+        const uint32_t idx = wbase + k * kWave;
+        keys[k] = (idx < lim) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
+      }
+    }
+    __syncthreads();
+
+    // Wave-level multisplit: lanes holding the same digit find each other with
+    // BITS ballots (a lane is a peer unless some ballot differs from this
+    // lane's own bit); the lowest lane of each group advances the wave's
+    // counter for that digit.
+    uint32_t* wh = s_whist[wave];
+    uint32_t rank[KPT];
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
-      const uint32_t idx = wbase + k * kWave;
-      keys[k] = (idx < lim) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
-    }
-  }
-
-  // Wave-level multisplit: lanes holding the same digit find each other with
-  // BITS ballots (a lane is a peer unless some ballot differs from this lane's
-  // own bit); the lowest lane of each group advances the wave's counter for
-  // that digit.
-  uint32_t* wh = s_whist[wave];
-  uint32_t rank[KPT];
+      const uint32_t d = (keys[k] >> shift) & dmask;
+      uint32_t mis_lo = 0, mis_hi = 0;  // lanes whose digit differs in some bit
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const uint32_t d = (keys[k] >> shift) & dmask;
-    uint32_t mis_lo = 0, mis_hi = 0;  // lanes whose digit differs in some bit
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-      const uint64_t bal = ballot(((d >> b) & 1u) != 0);
-      const uint32_t own = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-      mis_lo |= static_cast<uint32_t>(bal) ^ own;
-      mis_hi |= static_cast<uint32_t>(bal >> 32) ^ own;
+      for (int b = 0; b < BITS; ++b) {
+        const uint64_t bal = ballot(((d >> b) & 1u) != 0);
+        const uint32_t own = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+        mis_lo |= static_cast<uint32_t>(bal) ^ own;
+        mis_hi |= static_cast<uint32_t>(bal >> 32) ^ own;
+      }
+      const uint64_t m = ~((static_cast<uint64_t>(mis_hi) << 32) | mis_lo);
+      const uint32_t below = popc_below(m);
+      const uint32_t old = wh[d];
+      if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
+      rank[k] = old + below;
     }
-    const uint64_t m = ~((static_cast<uint64_t>(mis_hi) << 32) | mis_lo);
-    const uint32_t below = popc_below(m);
-    const uint32_t old = wh[d];
-    if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
-    rank[k] = old + below;
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // Per digit: exclusive offsets of the waves, tile count, publish aggregate.
-  uint32_t cnt = 0, cnt_valid = 0;
-  if (tid < 256) {
-    const uint32_t d = tid;
-    uint32_t run = 0;
+    // Per digit: exclusive offsets of the waves, tile count, publish aggregate.
+    uint32_t cnt = 0, cnt_valid = 0;
+    if (tid < 256) {
+      const uint32_t d = tid;
+      uint32_t run = 0;
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-      const uint32_t t = s_whist[w][d];
-      s_whist[w][d] = run;
-      run += t;
+      for (int w = 0; w < WAVES; ++w) {
+        const uint32_t t = s_whist[w][d];
+        s_whist[w][d] = run;
+        run += t;
+      }
+      cnt = run;
+      cnt_valid = run - ((d == dmask) ? (TILE - n_valid) : 0u);
+      lb_store(&lookback[static_cast<size_t>(tile) * 256 + d],
+               lb_pack(epoch, tile_in_seg == 0 ? kIncl : kAgg, cnt_valid));
     }
-    cnt = run;
-    cnt_valid = run - ((d == dmask) ? (TILE - n_valid) : 0u);
-    lb_store(&lookback[static_cast<size_t>(tile) * 256 + d],
-             lb_pack(epoch, tile_in_seg == 0 ? kIncl : kAgg, cnt_valid));
-  }
-  uint32_t total;
-  const uint32_t tstart = scan256(cnt, s_tmp, &total);
-  if (tid < 256) s_tstart[tid] = tstart;
-  __syncthreads();
+    uint32_t total;
+    const uint32_t tstart = scan256(cnt, s_tmp, &total);
+    if (tid < 256) s_tstart[tid] = tstart;
+    __syncthreads();
 
-  // Stage the tile in digit order.
+    // Stage the tile in digit order.
 #pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const uint32_t d = (keys[k] >> shift) & dmask;
-    s_stage[s_tstart[d] + s_whist[wave][d] + rank[k]] = keys[k];
-  }
-
-  // Look back for this digit's exclusive prefix over the segment's earlier tiles.
-  if (tid < 256) {
-    const uint32_t d = tid;
-    uint32_t excl = 0;
-    if (tile_in_seg > 0) {
-      excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
-      lb_store(&lookback[static_cast<size_t>(tile) * 256 + d], lb_pack(epoch, kIncl, excl + cnt_valid));
+    for (int k = 0; k < KPT; ++k) {
+      const uint32_t d = (keys[k] >> shift) & dmask;
+      s_stage[s_tstart[d] + s_whist[wave][d] + rank[k]] = keys[k];
     }
-    s_gbase[d] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + d] +
-                 excl - tstart;
-  }
-  __syncthreads();
 
-  // Scatter: consecutive staged keys of one digit go to consecutive addresses.
-  for (uint32_t p = tid; p < n_valid; p += BLOCK) {
-    const uint32_t key = s_stage[p];
-    out[s_gbase[(key >> shift) & dmask] + p] = key;
+    // Look back for this digit's exclusive prefix over the segment's earlier tiles.
+    if (tid < 256) {
+      const uint32_t d = tid;
+      uint32_t excl = 0;
+      if (tile_in_seg > 0) {
+        excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
+        lb_store(&lookback[static_cast<size_t>(tile) * 256 + d], lb_pack(epoch, kIncl, excl + cnt_valid));
+      }
+      s_gbase[d] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + d] +
+                   excl - tstart;
+    }
+    __syncthreads();
+
+    // Scatter: consecutive staged keys of one digit go to consecutive addresses.
+    for (uint32_t p = tid; p < n_valid; p += BLOCK) {
+      const uint32_t key = s_stage[p];
+      out[s_gbase[(key >> shift) & dmask] + p] = key;
+    }
+    __syncthreads();  // s_stage / s_gbase / s_whist are reused by the next tile
   }
-  if (tid == 0) finish_launch(ctrl, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -253,6 +250,8 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(TILE >= kSortMinTile, "look-back table sized for kSortMinTile");
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
+  const uint32_t per = kSortTilesPerGrab;
+  const uint32_t n_grabs = (n_tiles + per - 1) / per;
   // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an
   // odd count starts with in -> out)
   const uint32_t* src = L.keys_in;
@@ -262,8 +261,8 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
     const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
 #define RW_ONESWEEP(B)                                                                       \
   case B:                                                                                    \
-    hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_tiles), dim3(BLOCK), 0, stream, src, \
-                       dst, L.seg, n_tiles, L.digit_start, p, n_passes, 8 * p, L.lookback,     \
+    hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_grabs), dim3(BLOCK), 0, stream, src, \
+                       dst, L.seg, n_tiles, per, L.digit_start, p, n_passes, 8 * p, L.lookback, \
                        L.ctrl);                                                              \
     break;
     switch (bits) {

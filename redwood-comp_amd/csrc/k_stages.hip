@@ -39,7 +39,7 @@ void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_i
 }
 
 // ---------------------------------------------------------------------------
-// Tile-wide exclusive scan for kScanBlock threads (4 waves).
+// Tile-wide exclusive scan for kScanBlock threads.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
   constexpr int W = kScanBlock / kWave;
@@ -89,100 +89,102 @@ __device__ __forceinline__ uint32_t tile_prefix(uint32_t tile_total, uint32_t ti
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, SegArgs seg,
-                                                       uint32_t n_tiles, uint32_t* __restrict__ n_unique,
+                                                       uint32_t n_tiles, uint32_t tiles_per_grab,
+                                                       uint32_t* __restrict__ n_unique,
                                                        uint64_t* __restrict__ lookback,
                                                        uint32_t* __restrict__ ctrl) {
   constexpr int G = kScanKpt / 4;
   __shared__ uint32_t s_w[kScanBlock / kWave];
-  __shared__ uint32_t s_bcast, s_tile;
+  __shared__ uint32_t s_bcast;
+  __shared__ Grab s_grab;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t epoch = epoch_of(ctrl);
-  const uint32_t first = tile * kScanTile;
-  Seg sg{0, 0, 0};
-  uint32_t lim = 0;
-  if (tile < n_tiles) {
-    sg = seg_of(seg, first);
-    lim = sg.base + sg.count;
-  }
-  if (tile >= n_tiles || first >= lim) {
-    // an empty segment still reports n_unique = 0 from its first tile
-    if (tid == 0 && tile < n_tiles && first == sg.base) n_unique[sg.id] = 0;
-    if (tid == 0) finish_launch(ctrl, gridDim.x);
-    return;
-  }
-  const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
   const uint32_t wave = tid / kWave, lane = tid % kWave;
-  const uint32_t wbase = first + wave * (kScanKpt * kWave);
+  if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
+  __syncthreads();
+  const uint32_t epoch = s_grab.epoch;
+  const uint32_t t0 = s_grab.index * tiles_per_grab;
+  const uint32_t t1 = min(t0 + tiles_per_grab, n_tiles);
 
-  uint32_t k[G][4];
-  uint32_t flags[G];  // 4 head bits per group
-  uint32_t cnt[G];
-  if (first + kScanTile <= lim) {  // full tile: plain 16-byte loads
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const uint4 q = *reinterpret_cast<const uint4*>(in + wbase + 4 * (g * kWave + lane));
-      k[g][0] = q.x; k[g][1] = q.y; k[g][2] = q.z; k[g][3] = q.w;
+  for (uint32_t tile = t0; tile < t1; ++tile) {
+    const uint32_t first = tile * kScanTile;
+    const Seg sg = seg_of(seg, first);
+    const uint32_t lim = sg.base + sg.count;
+    if (first >= lim) {
+      // an empty segment still reports n_unique = 0 from its first tile
+      if (tid == 0 && first == sg.base) n_unique[sg.id] = 0;
+      continue;
     }
-  } else {
+    const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
+    const uint32_t wbase = first + wave * (kScanKpt * kWave);
+
+    uint32_t k[G][4];
+    uint32_t flags[G];  // 4 head bits per group
+    uint32_t cnt[G];
+    if (first + kScanTile <= lim) {  // full tile: plain 16-byte loads
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const uint4 q = *reinterpret_cast<const uint4*>(in + wbase + 4 * (g * kWave + lane));
+        k[g][0] = q.x; k[g][1] = q.y; k[g][2] = q.z; k[g][3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) k[g][c] = (i0 + c < lim) ? in[i0 + c] : 0u;
+      }
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+      // predecessor of the group's first key: previous lane's last key, or memory
+      uint32_t prev = __shfl_up(k[g][3], 1, kWave);
+      if (lane == 0 && i0 > sg.base && i0 < lim) prev = in[i0 - 1];
+      uint32_t f = 0;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) k[g][c] = (i0 + c < lim) ? in[i0 + c] : 0u;
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t i = i0 + c;
+        const uint32_t pk = (c == 0) ? prev : k[g][c - 1];
+        const bool head = (i < lim) && (i == sg.base || k[g][c] != pk);
+        f |= (head ? 1u : 0u) << c;
+      }
+      flags[g] = f;
+      cnt[g] = __popc(f);
     }
-  }
+    // Order is (wave, group, lane, c): per group a wave scan over lanes.
+    uint32_t wave_run = 0;
+    uint32_t gexcl[G];
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
-    // predecessor of the group's first key: previous lane's last key, or memory
-    uint32_t prev = __shfl_up(k[g][3], 1, kWave);
-    if (lane == 0 && i0 > sg.base && i0 < lim) prev = in[i0 - 1];
-    uint32_t f = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const uint32_t i = i0 + c;
-      const uint32_t pk = (c == 0) ? prev : k[g][c - 1];
-      const bool head = (i < lim) && (i == sg.base || k[g][c] != pk);
-      f |= (head ? 1u : 0u) << c;
+    for (int g = 0; g < G; ++g) {
+      const uint32_t incl = wave_incl_scan(cnt[g]);
+      gexcl[g] = wave_run + incl - cnt[g];
+      wave_run += __shfl(incl, kWave - 1, kWave);
     }
-    flags[g] = f;
-    cnt[g] = __popc(f);
-  }
-  // Order is (wave, group, lane, c): per group a wave scan over lanes.
-  uint32_t wave_run = 0;
-  uint32_t gexcl[G];
+    uint32_t tile_total;
+    const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
+    const uint32_t wave_off = __shfl(wexcl, 0, kWave);
+    const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+    const uint32_t base = sg.base + excl + wave_off;
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t incl = wave_incl_scan(cnt[g]);
-    gexcl[g] = wave_run + incl - cnt[g];
-    wave_run += __shfl(incl, kWave - 1, kWave);
-  }
-  uint32_t tile_total;
-  const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
-  const uint32_t wave_off = __shfl(wexcl, 0, kWave);
-  const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
-  const uint32_t base = sg.base + excl + wave_off;
+    for (int g = 0; g < G; ++g) {
+      uint32_t pos = base + gexcl[g];
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    uint32_t pos = base + gexcl[g];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if ((flags[g] >> c) & 1u) out[pos++] = k[g][c];
+      for (int c = 0; c < 4; ++c)
+        if ((flags[g] >> c) & 1u) out[pos++] = k[g][c];
+    }
+    // the tile holding the segment's last key reports the count
+    if (tid == 0 && lim - first <= kScanTile) n_unique[sg.id] = excl + tile_total;
+    __syncthreads();  // s_bcast / s_w are reused by the next tile
   }
-  // the tile holding the segment's last key reports the count
-  if (tid == 0 && lim - first <= kScanTile) n_unique[sg.id] = excl + tile_total;
-  if (tid == 0) finish_launch(ctrl, gridDim.x);
 }
 
 void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, uint32_t n_index,
                    uint32_t* n_unique, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
   if (n_tiles == 0) return;
-  hipLaunchKernelGGL(k_unique, dim3(n_tiles), dim3(kScanBlock), 0, stream, sorted, unique_out, seg,
-                     n_tiles, n_unique, lookback, ctrl);
+  const uint32_t n_grabs = (n_tiles + kScanTilesPerGrab - 1) / kScanTilesPerGrab;
+  hipLaunchKernelGGL(k_unique, dim3(n_grabs), dim3(kScanBlock), 0, stream, sorted, unique_out, seg,
+                     n_tiles, kScanTilesPerGrab, n_unique, lookback, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,85 +334,86 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __restrict__ in,
                                                            int32_t* __restrict__ out, SegArgs seg,
-                                                           uint32_t n_tiles, uint32_t* __restrict__ total,
+                                                           uint32_t n_tiles, uint32_t tiles_per_grab,
+                                                           uint32_t* __restrict__ total,
                                                            uint64_t* __restrict__ lookback,
                                                            uint32_t* __restrict__ ctrl) {
   constexpr int G = kScanKpt / 4;
   __shared__ uint32_t s_w[kScanBlock / kWave];
-  __shared__ uint32_t s_bcast, s_tile;
+  __shared__ uint32_t s_bcast;
+  __shared__ Grab s_grab;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t epoch = epoch_of(ctrl);
-  const uint32_t first = tile * kScanTile;
-  Seg sg{0, 0, 0};
-  uint32_t lim = 0;
-  if (tile < n_tiles) {
-    sg = seg_of(seg, first);
-    lim = sg.base + sg.count;
-  }
-  if (tile >= n_tiles || first >= lim) {
-    if (tid == 0 && tile < n_tiles && first == sg.base) total[sg.id] = 0;
-    if (tid == 0) finish_launch(ctrl, gridDim.x);
-    return;
-  }
-  const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
   const uint32_t wave = tid / kWave, lane = tid % kWave;
-  const uint32_t wbase = first + wave * (kScanKpt * kWave);
-  uint32_t v[G][4];
-  uint32_t gsum[G];
-  const bool full = first + kScanTile <= lim;
-  if (full) {  // plain 16-byte loads
+  if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
+  __syncthreads();
+  const uint32_t epoch = s_grab.epoch;
+  const uint32_t t0 = s_grab.index * tiles_per_grab;
+  const uint32_t t1 = min(t0 + tiles_per_grab, n_tiles);
+
+  for (uint32_t tile = t0; tile < t1; ++tile) {
+    const uint32_t first = tile * kScanTile;
+    const Seg sg = seg_of(seg, first);
+    const uint32_t lim = sg.base + sg.count;
+    if (first >= lim) {
+      if (tid == 0 && first == sg.base) total[sg.id] = 0;
+      continue;
+    }
+    const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
+    const uint32_t wbase = first + wave * (kScanKpt * kWave);
+    uint32_t v[G][4];
+    uint32_t gsum[G];
+    const bool full = first + kScanTile <= lim;
+    if (full) {  // plain 16-byte loads
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int4 q = *reinterpret_cast<const int4*>(in + wbase + 4 * (g * kWave + lane));
+        v[g][0] = q.x; v[g][1] = q.y; v[g][2] = q.z; v[g][3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[g][c] = (i0 + c < lim) ? static_cast<uint32_t>(in[i0 + c]) : 0u;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) gsum[g] = v[g][0] + v[g][1] + v[g][2] + v[g][3];
+    uint32_t wave_run = 0;
+    uint32_t gexcl[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int4 q = *reinterpret_cast<const int4*>(in + wbase + 4 * (g * kWave + lane));
-      v[g][0] = q.x; v[g][1] = q.y; v[g][2] = q.z; v[g][3] = q.w;
+      const uint32_t incl = wave_incl_scan(gsum[g]);
+      gexcl[g] = wave_run + incl - gsum[g];
+      wave_run += __shfl(incl, kWave - 1, kWave);
     }
-  } else {
+    uint32_t tile_total;
+    const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
+    const uint32_t wave_off = __shfl(wexcl, 0, kWave);
+    const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+    const uint32_t base = excl + wave_off;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+      uint32_t r = base + gexcl[g];
+      uint32_t o[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) v[g][c] = (i0 + c < lim) ? static_cast<uint32_t>(in[i0 + c]) : 0u;
+      for (int c = 0; c < 4; ++c) {
+        o[c] = r;
+        r += v[g][c];
+      }
+      if (full) {
+        *reinterpret_cast<int4*>(out + i0) = make_int4(static_cast<int>(o[0]), static_cast<int>(o[1]),
+                                                       static_cast<int>(o[2]), static_cast<int>(o[3]));
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (i0 + c < lim) out[i0 + c] = static_cast<int>(o[c]);
+      }
     }
+    if (tid == 0 && lim - first <= kScanTile) total[sg.id] = excl + tile_total;
+    __syncthreads();
   }
-#pragma unroll
-  for (int g = 0; g < G; ++g) gsum[g] = v[g][0] + v[g][1] + v[g][2] + v[g][3];
-  uint32_t wave_run = 0;
-  uint32_t gexcl[G];
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t incl = wave_incl_scan(gsum[g]);
-    gexcl[g] = wave_run + incl - gsum[g];
-    wave_run += __shfl(incl, kWave - 1, kWave);
-  }
-  uint32_t tile_total;
-  const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
-  const uint32_t wave_off = __shfl(wexcl, 0, kWave);
-  const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
-  const uint32_t base = excl + wave_off;
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t i0 = wbase + 4 * (g * kWave + lane);
-    uint32_t r = base + gexcl[g];
-    uint32_t o[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      o[c] = r;
-      r += v[g][c];
-    }
-    if (full) {
-      *reinterpret_cast<int4*>(out + i0) = make_int4(static_cast<int>(o[0]), static_cast<int>(o[1]),
-                                                     static_cast<int>(o[2]), static_cast<int>(o[3]));
-    } else {
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (i0 + c < lim) out[i0 + c] = static_cast<int>(o[c]);
-    }
-  }
-  if (tid == 0 && lim - first <= kScanTile) total[sg.id] = excl + tile_total;
-  if (tid == 0) finish_launch(ctrl, gridDim.x);
 }
 
 void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
@@ -418,8 +421,9 @@ void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
   if (n_tiles == 0) return;
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_prefix_sum, dim3(n_tiles), dim3(kScanBlock), 0, stream, in, out, seg,
-                     n_tiles, total, lookback, ctrl);
+  const uint32_t n_grabs = (n_tiles + kScanTilesPerGrab - 1) / kScanTilesPerGrab;
+  hipLaunchKernelGGL(k_prefix_sum, dim3(n_grabs), dim3(kScanBlock), 0, stream, in, out, seg,
+                     n_tiles, kScanTilesPerGrab, total, lookback, ctrl);
 }
 
 }  // namespace rw

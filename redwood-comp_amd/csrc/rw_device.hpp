@@ -161,27 +161,34 @@ __device__ __forceinline__ uint32_t lb_exclusive_wave(const uint64_t* table, uin
   return sum;
 }
 
-// Launch-serial bookkeeping of one look-back user: ctrl[0] tile counter,
-// ctrl[1] finished-workgroup counter, ctrl[2] epoch.  The last workgroup to
-// finish rewinds the counters and advances the epoch, so consecutive launches
-// (and graph replays) need no memset.  No fence is needed: within a launch
-// workgroups exchange data only through the look-back words (device-scope
-// atomics), and the reset is read by the next launch, after the kernel
-// boundary.  (A device-scope fence here would write back the XCD's whole L2
-// once per workgroup.)
-constexpr int kCtrlWords = 4;
+// Launch bookkeeping of one look-back user: one 64-bit word, the launch
+// epoch in the high half and a grab counter in the low half.  A launch has
+// exactly n_grabs workgroups; each takes the next grab (a run of consecutive
+// tiles) with one atomic add, so tiles are handed out in order to resident
+// workgroups (the look-back spin always ends).  The epoch comes back with the
+// same atomic; the workgroup that takes the last grab stores (epoch + 1, 0),
+// which is safe because no other workgroup of the launch touches the word
+// again.  Consecutive launches and graph replays need no memset, and no
+// second atomic or fence per workgroup is spent on bookkeeping.
+constexpr int kCtrlWords = 4;  // 16 bytes per user (8 used)
 
-__device__ __forceinline__ uint32_t epoch_of(const uint32_t* ctrl) {
-  return (ctrl[2] % 0x3fffffffu) + 1u;
-}
+struct Grab {
+  uint32_t index;  // which run of tiles
+  uint32_t epoch;  // look-back epoch of this launch, in [1, 2^30)
+};
 
-__device__ __forceinline__ void finish_launch(uint32_t* ctrl, uint32_t n_groups) {
-  const uint32_t done = atomicAdd(&ctrl[1], 1u);
-  if (done == n_groups - 1) {
-    ctrl[0] = 0;
-    ctrl[1] = 0;
-    ctrl[2] = ctrl[2] + 1;
-  }
+// Thread 0 only.
+__device__ __forceinline__ Grab grab_tiles(uint32_t* ctrl, uint32_t n_grabs) {
+  unsigned long long* w = reinterpret_cast<unsigned long long*>(ctrl);
+  const unsigned long long v = atomicAdd(w, 1ull);
+  const uint32_t e = static_cast<uint32_t>(v >> 32);
+  Grab g;
+  g.index = static_cast<uint32_t>(v);
+  g.epoch = (e % 0x3fffffffu) + 1u;
+  if (g.index == n_grabs - 1)
+    __hip_atomic_store(w, static_cast<unsigned long long>(e + 1u) << 32, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return g;
 }
 
 // ---------------------------------------------------------------------------

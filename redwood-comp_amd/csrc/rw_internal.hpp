@@ -22,12 +22,14 @@ constexpr int kHistBlock = 512;
 constexpr uint32_t kHistCopies = 16;
 constexpr uint32_t kMaxPasses = 4;
 constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
-constexpr int kLookbackWindow = 8;           // look-back words loaded per round trip
+constexpr int kLookbackWindow = 4;           // look-back words loaded per round trip
+constexpr uint32_t kSortTilesPerGrab = 2;    // consecutive tiles per workgroup (one atomic)
+constexpr uint32_t kScanTilesPerGrab = 4;
 
 // Compaction / scan tiles.
-constexpr int kScanBlock = 256;
+constexpr int kScanBlock = 512;
 constexpr int kScanKpt = 16;
-constexpr uint32_t kScanTile = kScanBlock * kScanKpt;  // 4096
+constexpr uint32_t kScanTile = kScanBlock * kScanKpt;  // 8192
 
 // Elementwise tiles.
 constexpr int kElemBlock = 256;
