@@ -8,13 +8,28 @@
 // offset[i] + j is the node j levels above the deepest one.  Every slot but
 // the chain's first ("bottom") receives exactly one child, from the slot below
 // it in the same chain, so its whole row (8 children, node mask, leaf mask)
-// is written in one go.  Bottom slots receive children from other chains (the
-// top of every chain links to the bottom slot of its nearest contributing
-// ancestor, a chain start by construction of the exclusive offsets), so they
-// are cleared by their owner in the first kernel and filled with atomic max /
-// atomic OR in the second: the result is the same as the reference's
-// |= / = updates resolved as OR and max (D6), independent of scheduling.
-// Slots outside [0, min(n_oct, capacity)) are never written.
+// is written by the owner in one go.
+//
+// Bottom slots receive the tops of other chains: the reference pushes, from
+// every chain c, a link into offset[T(c)], where T(c) is found by walking up
+// parent[] while edge_count is 0 (at most 31 steps; Q5).  Here the owner of a
+// bottom pulls those links instead, so no slot has two writers and no atomic
+// is needed:
+//   * parent[] is the complete Karras tree (node 0 spans [0, n_brt]), so the
+//     nodes whose parent is p >= 1 are p's internal children (left_child[p]
+//     unless has_leaf_left[p], left_child[p] + 1 unless has_leaf_right[p]),
+//     and those of node 0 are the internal children of the complete-range
+//     root;
+//   * a node of edge count 0 has the same octree level as its parent and a
+//     longer prefix, so at most two of them follow each other on a path: the
+//     31-step cap can only bite at node 0 (count 0, its own parent), where
+//     the walk stays.  T(c) is thus the nearest ancestor with a non-zero count,
+//     or node 0;
+//   * the owner p pulls every chain reached from its children through nodes of
+//     count 0, and the owner of slot 0 (the first chain, offset 0 = offset[0])
+//     also pulls for node 0.
+// Child slots keep the largest top and the mask ORs all links (D6).  Slots
+// outside [0, min(n_oct, capacity)) are never written.
 #include "rw_device.hpp"
 #include "rw_internal.hpp"
 
@@ -43,13 +58,64 @@ __device__ __forceinline__ float cell_of(float range, int level_diff) {
   return __fdiv_rn(range, static_cast<float>(1u << (static_cast<uint32_t>(level_diff) & 31u)));
 }
 
-// Kernel A: corners, cell sizes, chain-internal rows, cleared bottom rows, and
-// the root cell when no node contributes (D8).
+// One incoming link: chain c's top slot at its top level's octant.
+__device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c, int32_t (&row)[8],
+                                         int32_t& mask) {
+  const int cnt = a.edge_count[base + c];
+  const int32_t top = a.edge_offset[base + c] + cnt - 1;
+  const int top_level = a.prefix_n[base + c] / 3 - cnt + 1;
+  const uint32_t which = (a.codes[base + c] >> level_shift(top_level)) & 7u;
+#pragma unroll
+  for (uint32_t w = 0; w < 8; ++w) row[w] = (w == which) ? max(row[w], top) : row[w];
+  mask |= static_cast<int32_t>(1u << which);
+}
+
+// Pulls every chain reached from the given roots through nodes of count 0.
+__device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int* stack, int sp,
+                                           int32_t (&row)[8], int32_t& mask) {
+  constexpr int kStack = 16;
+  while (sp > 0) {
+    const int q = stack[--sp];
+    if (a.edge_count[base + q] > 0) {
+      add_link(a, base, q, row, mask);
+      continue;
+    }
+    const int lc = a.left_child[base + q];
+    if (!a.has_leaf_left[base + q] && sp < kStack) stack[sp++] = lc;
+    if (!a.has_leaf_right[base + q] && sp < kStack) stack[sp++] = lc + 1;
+  }
+}
+
+__device__ __forceinline__ int delta8o(uint32_t x, uint32_t y) {
+  return static_cast<uint8_t>(__clz(x ^ y) - 1);
+}
+
+// Split of the complete range [0, n] (the root of the complete tree), by the
+// reference's split search (func_brt.hpp:90-104) with l = n, d = +1.
+__device__ int full_root_split(const uint32_t* codes, int n) {
+  const uint32_t c0 = codes[0];
+  const int dnode = delta8o(c0, codes[n]);
+  int l2 = 0;  // log2_ceil(n)
+  while ((1u << l2) < static_cast<uint32_t>(n)) ++l2;
+  const uint32_t max_div = 1u << l2;
+  int s = 0;
+  uint32_t t = (static_cast<uint32_t>(n) + 1) / 2;
+  for (uint32_t div = 2; div <= max_div; div <<= 1) {
+    if (s + static_cast<int>(t) <= n - 1 && delta8o(c0, codes[s + static_cast<int>(t)]) > dnode)
+      s += static_cast<int>(t);
+    t = (static_cast<uint32_t>(n) + (div << 1) - 1) / (div << 1);
+  }
+  return s;
+}
+
+// Corners, cell sizes and every row of the chain; the bottom row pulls its
+// links.  Node 0 writes the root cell when no node contributes (D8).
 __global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg) {
   const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
   const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
   const int i = static_cast<int>(gi - sg.base);
-  if (gi < sg.base || i >= static_cast<int>(sg.count)) return;
+  const int n = static_cast<int>(sg.count);  // n_brt
+  if (gi < sg.base || i >= n) return;
   const OctSeg os = oct_seg(a, sg);
   const uint32_t* codes = a.codes + sg.base;
   const uint8_t* prefix_n = a.prefix_n + sg.base;
@@ -77,56 +143,38 @@ __global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg
   const int64_t rows = os.rows;
   for (int j = 0; j < count; ++j) {
     const int64_t slot = off + j;
+    if (slot < 0 || slot >= rows) continue;
     const int level = lvl - j;
     const uint32_t sh = level_shift(level);
-    const uint32_t node_prefix = ci >> sh;
-    if (slot >= 0 && slot < rows) {
-      corner[slot] = morton_decode(node_prefix << sh, a.min_coord, a.range);
-      cell[slot] = cell_of(a.range, level - root_level);
-    }
-    // row of this slot: the bottom is cleared; others hold the child below
-    if (slot >= 0 && slot < rows) {
-      // which = 8 marks "no child" (the bottom slot)
-      uint32_t which = 8u;
-      if (j > 0) which = (ci >> level_shift(lvl - (j - 1))) & 7u;
-      const int v = static_cast<int>(slot - 1);
-      int r[8];
+    corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
+    cell[slot] = cell_of(a.range, level - root_level);
+    int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t m = 0;
+    if (j > 0) {  // the child below, in the same chain
+      const uint32_t which = (ci >> level_shift(lvl - (j - 1))) & 7u;
 #pragma unroll
-      for (uint32_t c = 0; c < 8; ++c) r[c] = (which == c) ? v : 0;
-      rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
-      rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
-      mask[slot] = (which < 8u) ? static_cast<int32_t>(1u << which) : 0;
-      leaf_mask[slot] = 0;
+      for (uint32_t w = 0; w < 8; ++w) r[w] = (w == which) ? static_cast<int32_t>(slot - 1) : 0;
+      m = static_cast<int32_t>(1u << which);
+    } else {  // the bottom: pull the chains that link here
+      int stack[16];
+      int sp = 0;
+      const int lc = a.left_child[sg.base + i];
+      if (!a.has_leaf_left[sg.base + i]) stack[sp++] = lc;
+      if (!a.has_leaf_right[sg.base + i]) stack[sp++] = lc + 1;
+      pull_links(a, sg.base, stack, sp, r, m);
+      if (off == 0) {  // slot 0 is also offset[0]: node 0's links land here
+        const int g = full_root_split(codes, n);
+        sp = 0;
+        if (g != 0) stack[sp++] = g;
+        if (g + 1 < n) stack[sp++] = g + 1;
+        pull_links(a, sg.base, stack, sp, r, m);
+      }
     }
+    rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
+    rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
+    mask[slot] = m;
+    leaf_mask[slot] = 0;
   }
-}
-
-// Kernel B: the top of each chain links to its nearest contributing ancestor
-// (walk capped at 31 steps, Q5).
-__global__ __launch_bounds__(kElemBlock) void k_oct_links(OctArgs a, SegArgs seg) {
-  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
-  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
-  const int i = static_cast<int>(gi - sg.base);
-  if (gi < sg.base || i < 1 || i >= static_cast<int>(sg.count)) return;
-  const int32_t* counts = a.edge_count + sg.base;
-  const int count = counts[i];
-  if (count <= 0) return;
-  const OctSeg os = oct_seg(a, sg);
-  const int32_t* parent = a.parent + sg.base;
-  int rt = parent[i];
-  int steps = 0;
-  while (counts[rt] == 0) {
-    rt = parent[rt];
-    if (++steps > 30) break;
-  }
-  const int64_t oct_parent = a.edge_offset[sg.base + rt];
-  const int64_t top = static_cast<int64_t>(a.edge_offset[sg.base + i]) + count - 1;
-  const int top_level = a.prefix_n[sg.base + i] / 3 - count + 1;
-  const uint32_t which = (a.codes[sg.base + i] >> level_shift(top_level)) & 7u;
-  if (oct_parent < 0 || oct_parent >= static_cast<int64_t>(os.rows)) return;
-  int32_t* children = a.children + static_cast<size_t>(os.base) * 8;
-  atomicMax(&children[oct_parent * 8 + which], static_cast<int32_t>(top));
-  atomicOr(&a.child_node_mask[os.base + oct_parent], static_cast<int32_t>(1u << which));
 }
 
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
@@ -134,7 +182,6 @@ void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t 
   if (grid == 0) return;
   seg.count_sub = 1;
   hipLaunchKernelGGL(k_oct_nodes, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
-  hipLaunchKernelGGL(k_oct_links, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
 }
 
 // Leaf linking (opt-in): each leaf child of brt node i goes into the bottom

@@ -141,10 +141,11 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
   __syncthreads();
   const uint32_t epoch = s_grab.epoch;
-  const uint32_t t0 = s_grab.index * tiles_per_grab;
-  const uint32_t t1 = min(t0 + tiles_per_grab, n_tiles);
+  // grab g owns tiles g, g + G, g + 2G, ... (G = number of grabs): each
+  // workgroup's next tile depends on tiles the other grabs reach at the same
+  // pace, so the tiles stay in flight together
 
-  for (uint32_t tile = t0; tile < t1; ++tile) {
+  for (uint32_t tile = s_grab.index; tile < n_tiles; tile += gridDim.x) {
     const uint32_t first = tile * TILE;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;

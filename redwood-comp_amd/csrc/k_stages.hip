@@ -102,10 +102,11 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
   __syncthreads();
   const uint32_t epoch = s_grab.epoch;
-  const uint32_t t0 = s_grab.index * tiles_per_grab;
-  const uint32_t t1 = min(t0 + tiles_per_grab, n_tiles);
+  // grab g owns tiles g, g + G, g + 2G, ... (G = number of grabs): each
+  // workgroup's next tile depends on tiles the other grabs reach at the same
+  // pace, so the tiles stay in flight together
 
-  for (uint32_t tile = t0; tile < t1; ++tile) {
+  for (uint32_t tile = s_grab.index; tile < n_tiles; tile += gridDim.x) {
     const uint32_t first = tile * kScanTile;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
@@ -347,10 +348,11 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
   __syncthreads();
   const uint32_t epoch = s_grab.epoch;
-  const uint32_t t0 = s_grab.index * tiles_per_grab;
-  const uint32_t t1 = min(t0 + tiles_per_grab, n_tiles);
+  // grab g owns tiles g, g + G, g + 2G, ... (G = number of grabs): each
+  // workgroup's next tile depends on tiles the other grabs reach at the same
+  // pace, so the tiles stay in flight together
 
-  for (uint32_t tile = t0; tile < t1; ++tile) {
+  for (uint32_t tile = s_grab.index; tile < n_tiles; tile += gridDim.x) {
     const uint32_t first = tile * kScanTile;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;

@@ -45,11 +45,19 @@ def inputs(rw, kind, n, seed):
         p = rw.uniform_points(n, seed)
         p[:, :3] = np.floor(p[:, :3] / 256) * 256
         return p
+    if kind in ("lastfar", "firstfar"):
+        # all keys share a long prefix except one isolated at the top split
+        # (the last key for lastfar: the case where the root's shortened
+        # range (Q2) differs from the complete root)
+        p = rw.uniform_points(n, seed)
+        p[:, :3] = p[:, :3] / 64.0 + (0.0 if kind == "lastfar" else 900.0)
+        p[-1, :3] = 1000.0 if kind == "lastfar" else 0.5
+        return p
     raise ValueError(kind)
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 100, 1000, 4095, 4096, 4097, 8193, 65537, 307200])
-@pytest.mark.parametrize("kind", ["uniform", "gmm", "dups"])
+@pytest.mark.parametrize("kind", ["uniform", "gmm", "dups", "lastfar", "firstfar"])
 def test_build_matches_oracle(gpu, orc, pool, n, kind):
     rw = gpu
     pts = inputs(rw, kind, n, seed=n + 17)

@@ -306,3 +306,95 @@ def test_build_is_thread_count_invariant(orc, rw):
         if ref is None:
             ref = flat
         assert flat == ref
+
+
+def pull_rows(o, nu, nb, no):
+    """Second formulation of the octree links: every chain bottom pulls the
+    chains reached from its radix-tree children through nodes of edge count 0
+    (and slot 0 also pulls for node 0, the complete-range root).  Must equal the
+    reference's push along parent[] (the oracle)."""
+    codes = o["unique"][:nu].astype(np.int64)
+    cnt = o["edge_count"][:nb].astype(np.int64)
+    off = o["edge_offset"][:nb].astype(np.int64)
+    pn = o["prefix_n"][:nb].astype(np.int64)
+    lc = o["left_child"][:nb].astype(np.int64)
+    hl, hr = o["has_leaf_left"][:nb], o["has_leaf_right"][:nb]
+    n = nb
+    rows = {}
+
+    def link(c, row):
+        top_level = pn[c] // 3 - cnt[c] + 1
+        which = (int(codes[c]) >> ((30 - 3 * top_level) & 31)) & 7
+        row[0][which] = max(row[0][which], off[c] + cnt[c] - 1)
+        row[1] |= 1 << which
+
+    def pull(stack, row):
+        while stack:
+            q = stack.pop()
+            if cnt[q] > 0:
+                link(q, row)
+                continue
+            if not hl[q]:
+                stack.append(lc[q])
+            if not hr[q]:
+                stack.append(lc[q] + 1)
+
+    def full_root_split():
+        if n == 0:
+            return 0
+        bit = 31 - clz32(int(codes[0]) ^ int(codes[n]))
+        return max(k for k in range(n + 1) if not (int(codes[k]) >> bit) & 1)
+
+    for p in range(1, n):
+        if cnt[p] <= 0 or off[p] >= no:
+            continue
+        row = [[0] * 8, 0]
+        st = []
+        if not hl[p]:
+            st.append(lc[p])
+        if not hr[p]:
+            st.append(lc[p] + 1)
+        pull(st, row)
+        if off[p] == 0:
+            g = full_root_split()
+            st = ([g] if g != 0 else []) + ([g + 1] if g + 1 < n else [])
+            pull(st, row)
+        rows[off[p]] = row
+    return rows
+
+
+def test_octree_pull_equals_push(orc, pool, rw):
+    """The GPU builds chain-bottom rows by pulling; the oracle pushes as the
+    reference does.  Checked here on the CPU over random, clustered and
+    adversarial (isolated first / last key) clouds."""
+    rng = np.random.default_rng(77)
+    clouds = []
+    for s in range(60):
+        n = int(rng.integers(2, 3000))
+        kind = s % 5
+        if kind == 0:
+            p = rw.uniform_points(n, 1000 + s)
+        elif kind == 1:
+            p = rw.gmm_points(n, 1000 + s, 4, 3.0)
+        elif kind == 2:
+            p = rw.uniform_points(n, 1000 + s)
+            p[:, :3] = np.floor(p[:, :3] / 128) * 128
+        else:
+            p = rw.uniform_points(n, 1000 + s)
+            p[:, :3] = p[:, :3] / 64.0 + (0.0 if kind == 3 else 900.0)
+            p[-1, :3] = 1000.0 if kind == 3 else 0.5
+        clouds.append(p)
+    checked = 0
+    for p in clouds:
+        (nu, nb, no, rc), o, _ = orc.build(pool, p)
+        rows_n = min(no, o["cell_size"].shape[0])
+        got = pull_rows(o, nu, nb, rows_n)
+        # chain bottoms of the push result
+        bottoms = {int(o["edge_offset"][i]) for i in range(1, nb)
+                   if o["edge_count"][i] > 0 and o["edge_offset"][i] < rows_n}
+        assert set(got) == bottoms
+        for slot, (row, m) in got.items():
+            assert o["children"][slot].tolist() == row, (slot, row)
+            assert int(o["child_node_mask"][slot]) == m
+            checked += 1
+    assert checked > 1000
