@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
 // ---------------------------------------------------------------------------
 template <int BLOCK, int KPT, int BITS>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(
-    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg, uint32_t n_tiles,
-    uint32_t tiles_per_grab, const uint32_t* __restrict__ digit_start, uint32_t pass,
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg,
+    const uint32_t* __restrict__ digit_start, uint32_t pass,
     uint32_t n_passes, uint32_t shift, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
@@ -141,15 +141,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
   __syncthreads();
   const uint32_t epoch = s_grab.epoch;
-  // grab g owns tiles g, g + G, g + 2G, ... (G = number of grabs): each
-  // workgroup's next tile depends on tiles the other grabs reach at the same
-  // pace, so the tiles stay in flight together
-
-  for (uint32_t tile = s_grab.index; tile < n_tiles; tile += gridDim.x) {
+  // One tile per workgroup, in grab order: a tile only ever waits on tiles
+  // whose workgroups took their grab earlier and are therefore running, so
+  // the look-back spin ends whatever the residency of the grid.
+  {
+    const uint32_t tile = s_grab.index;
     const uint32_t first = tile * TILE;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
-    if (first >= lim) continue;  // a segment's padding (uniform across the workgroup)
+    if (first >= lim) return;  // a segment's padding (uniform across the workgroup)
     const uint32_t n_valid = min(TILE, lim - first);
     const uint32_t tile_in_seg = (first - sg.base) / TILE;
     const uint32_t wbase = first + wave * (KPT * kWave) + lane;
@@ -239,7 +239,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       const uint32_t key = s_stage[p];
       out[s_gbase[(key >> shift) & dmask] + p] = key;
     }
-    __syncthreads();  // s_stage / s_gbase / s_whist are reused by the next tile
   }
 }
 
@@ -251,8 +250,6 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(TILE >= kSortMinTile, "look-back table sized for kSortMinTile");
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
-  const uint32_t per = kSortTilesPerGrab;
-  const uint32_t n_grabs = (n_tiles + per - 1) / per;
   // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an
   // odd count starts with in -> out)
   const uint32_t* src = L.keys_in;
@@ -262,9 +259,8 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
     const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
 #define RW_ONESWEEP(B)                                                                       \
   case B:                                                                                    \
-    hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_grabs), dim3(BLOCK), 0, stream, src, \
-                       dst, L.seg, n_tiles, per, L.digit_start, p, n_passes, 8 * p, L.lookback, \
-                       L.ctrl);                                                              \
+    hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_tiles), dim3(BLOCK), 0, stream, src, \
+                       dst, L.seg, L.digit_start, p, n_passes, 8 * p, L.lookback, L.ctrl);  \
     break;
     switch (bits) {
       RW_ONESWEEP(1) RW_ONESWEEP(2) RW_ONESWEEP(3) RW_ONESWEEP(4)

@@ -89,7 +89,6 @@ __device__ __forceinline__ uint32_t tile_prefix(uint32_t tile_total, uint32_t ti
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, SegArgs seg,
-                                                       uint32_t n_tiles, uint32_t tiles_per_grab,
                                                        uint32_t* __restrict__ n_unique,
                                                        uint64_t* __restrict__ lookback,
                                                        uint32_t* __restrict__ ctrl) {
@@ -102,18 +101,18 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
   __syncthreads();
   const uint32_t epoch = s_grab.epoch;
-  // grab g owns tiles g, g + G, g + 2G, ... (G = number of grabs): each
-  // workgroup's next tile depends on tiles the other grabs reach at the same
-  // pace, so the tiles stay in flight together
-
-  for (uint32_t tile = s_grab.index; tile < n_tiles; tile += gridDim.x) {
+  // One tile per workgroup, in grab order: a tile only ever waits on tiles
+  // whose workgroups took their grab earlier and are therefore running, so
+  // the look-back spin ends whatever the residency of the grid.
+  {
+    const uint32_t tile = s_grab.index;
     const uint32_t first = tile * kScanTile;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
     if (first >= lim) {
       // an empty segment still reports n_unique = 0 from its first tile
       if (tid == 0 && first == sg.base) n_unique[sg.id] = 0;
-      continue;
+      return;
     }
     const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
     const uint32_t wbase = first + wave * (kScanKpt * kWave);
@@ -175,7 +174,6 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
     }
     // the tile holding the segment's last key reports the count
     if (tid == 0 && lim - first <= kScanTile) n_unique[sg.id] = excl + tile_total;
-    __syncthreads();  // s_bcast / s_w are reused by the next tile
   }
 }
 
@@ -183,9 +181,8 @@ void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, ui
                    uint32_t* n_unique, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
   if (n_tiles == 0) return;
-  const uint32_t n_grabs = (n_tiles + kScanTilesPerGrab - 1) / kScanTilesPerGrab;
-  hipLaunchKernelGGL(k_unique, dim3(n_grabs), dim3(kScanBlock), 0, stream, sorted, unique_out, seg,
-                     n_tiles, kScanTilesPerGrab, n_unique, lookback, ctrl);
+  hipLaunchKernelGGL(k_unique, dim3(n_tiles), dim3(kScanBlock), 0, stream, sorted, unique_out, seg,
+                     n_unique, lookback, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -335,7 +332,6 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __restrict__ in,
                                                            int32_t* __restrict__ out, SegArgs seg,
-                                                           uint32_t n_tiles, uint32_t tiles_per_grab,
                                                            uint32_t* __restrict__ total,
                                                            uint64_t* __restrict__ lookback,
                                                            uint32_t* __restrict__ ctrl) {
@@ -348,17 +344,17 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
   __syncthreads();
   const uint32_t epoch = s_grab.epoch;
-  // grab g owns tiles g, g + G, g + 2G, ... (G = number of grabs): each
-  // workgroup's next tile depends on tiles the other grabs reach at the same
-  // pace, so the tiles stay in flight together
-
-  for (uint32_t tile = s_grab.index; tile < n_tiles; tile += gridDim.x) {
+  // One tile per workgroup, in grab order: a tile only ever waits on tiles
+  // whose workgroups took their grab earlier and are therefore running, so
+  // the look-back spin ends whatever the residency of the grid.
+  {
+    const uint32_t tile = s_grab.index;
     const uint32_t first = tile * kScanTile;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
     if (first >= lim) {
       if (tid == 0 && first == sg.base) total[sg.id] = 0;
-      continue;
+      return;
     }
     const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
     const uint32_t wbase = first + wave * (kScanKpt * kWave);
@@ -414,7 +410,6 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
       }
     }
     if (tid == 0 && lim - first <= kScanTile) total[sg.id] = excl + tile_total;
-    __syncthreads();
   }
 }
 
@@ -423,9 +418,8 @@ void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
   if (n_tiles == 0) return;
   seg.count_sub = 1;
-  const uint32_t n_grabs = (n_tiles + kScanTilesPerGrab - 1) / kScanTilesPerGrab;
-  hipLaunchKernelGGL(k_prefix_sum, dim3(n_grabs), dim3(kScanBlock), 0, stream, in, out, seg,
-                     n_tiles, kScanTilesPerGrab, total, lookback, ctrl);
+  hipLaunchKernelGGL(k_prefix_sum, dim3(n_tiles), dim3(kScanBlock), 0, stream, in, out, seg,
+                     total, lookback, ctrl);
 }
 
 }  // namespace rw

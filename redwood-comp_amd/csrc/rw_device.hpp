@@ -163,9 +163,10 @@ __device__ __forceinline__ uint32_t lb_exclusive_wave(const uint64_t* table, uin
 
 // Launch bookkeeping of one look-back user: one 64-bit word, the launch
 // epoch in the high half and a grab counter in the low half.  A launch has
-// exactly n_grabs workgroups; each takes the next grab (a run of consecutive
-// tiles) with one atomic add, so tiles are handed out in order to resident
-// workgroups (the look-back spin always ends).  The epoch comes back with the
+// exactly n_grabs workgroups; each takes the next grab (its tile) with one
+// atomic add, so tiles are handed out in order to running workgroups and a
+// tile's predecessors all belong to workgroups that are already running (the
+// look-back spin always ends, however few workgroups fit on the chip).  The epoch comes back with the
 // same atomic; the workgroup that takes the last grab stores (epoch + 1, 0),
 // which is safe because no other workgroup of the launch touches the word
 // again.  Consecutive launches and graph replays need no memset, and no

@@ -23,8 +23,6 @@ constexpr uint32_t kHistCopies = 16;
 constexpr uint32_t kMaxPasses = 4;
 constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
 constexpr int kLookbackWindow = 4;           // look-back words loaded per round trip
-constexpr uint32_t kSortTilesPerGrab = 2;    // tiles per workgroup (one atomic per workgroup)
-constexpr uint32_t kScanTilesPerGrab = 4;
 
 // Compaction / scan tiles.
 constexpr int kScanBlock = 512;
