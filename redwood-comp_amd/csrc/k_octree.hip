@@ -58,19 +58,26 @@ __device__ __forceinline__ float cell_of(float range, int level_diff) {
   return __fdiv_rn(range, static_cast<float>(1u << (static_cast<uint32_t>(level_diff) & 31u)));
 }
 
-// One incoming link: chain c's top slot at its top level's octant.
-__device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c, int32_t (&row)[8],
-                                         int32_t& mask) {
-  const int cnt = a.edge_count[base + c];
-  const int32_t top = a.edge_offset[base + c] + cnt - 1;
-  const int top_level = a.prefix_n[base + c] / 3 - cnt + 1;
-  const uint32_t which = (a.codes[base + c] >> level_shift(top_level)) & 7u;
+// One incoming link: chain c's top slot at its top level's octant (the
+// reference's set_child arguments, func_octree.hpp:96-105).
+__device__ __forceinline__ void link_into(int cnt, int off, uint32_t code, int pfx, int32_t (&row)[8],
+                                          int32_t& mask) {
+  const int32_t top = off + cnt - 1;
+  const int top_level = pfx / 3 - cnt + 1;
+  const uint32_t which = (code >> level_shift(top_level)) & 7u;
 #pragma unroll
   for (uint32_t w = 0; w < 8; ++w) row[w] = (w == which) ? max(row[w], top) : row[w];
   mask |= static_cast<int32_t>(1u << which);
 }
 
-// Pulls every chain reached from the given roots through nodes of count 0.
+__device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c, int32_t (&row)[8],
+                                         int32_t& mask) {
+  link_into(a.edge_count[base + c], a.edge_offset[base + c], a.codes[base + c],
+            a.prefix_n[base + c], row, mask);
+}
+
+// Pulls every chain reached from the given roots through nodes of count 0
+// (depth-first, one node per step).
 __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int* stack, int sp,
                                            int32_t (&row)[8], int32_t& mask) {
   constexpr int kStack = 16;
@@ -83,6 +90,85 @@ __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int*
     const int lc = a.left_child[base + q];
     if (!a.has_leaf_left[base + q] && sp < kStack) stack[sp++] = lc;
     if (!a.has_leaf_right[base + q] && sp < kStack) stack[sp++] = lc + 1;
+  }
+}
+
+// What the pull needs of one radix node, loaded in one round trip.
+struct PullNode {
+  int cnt, off, lc;
+  uint32_t code;
+  int pfx;
+  bool leaf_l, leaf_r;
+};
+
+template <bool kWithChildren>
+__device__ __forceinline__ PullNode load_pull(const OctArgs& a, uint32_t base, int q, bool live) {
+  PullNode n{0, 0, 0, 0u, 0, true, true};
+  if (live) {
+    n.cnt = a.edge_count[base + q];
+    n.off = a.edge_offset[base + q];
+    n.code = a.codes[base + q];
+    n.pfx = a.prefix_n[base + q];
+    if (kWithChildren) {
+      n.lc = a.left_child[base + q];
+      n.leaf_l = a.has_leaf_left[base + q] != 0;
+      n.leaf_r = a.has_leaf_right[base + q] != 0;
+    }
+  }
+  return n;
+}
+
+// The pull of a bottom slot owned by brt node p >= 1, breadth-first: the
+// nodes of one level are loaded together, so the walk costs one memory round
+// trip per level instead of one per node.  Below p (count > 0) at most two
+// nodes of count 0 follow each other on a path (prefix lengths increase
+// strictly and a count-0 node has its parent's level), so every internal node
+// three levels down has a non-zero count; any that does not is still handled,
+// by the depth-first walk.
+__device__ __forceinline__ void pull_below(const OctArgs& a, uint32_t base, int lc, bool leaf_l,
+                                           bool leaf_r, int32_t (&row)[8], int32_t& mask) {
+  PullNode n1[2];
+  n1[0] = load_pull<true>(a, base, lc, !leaf_l);
+  n1[1] = load_pull<true>(a, base, lc + 1, !leaf_r);
+  int q2[4];
+  bool v2[4];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const bool live = (k == 0) ? !leaf_l : !leaf_r;
+    if (live && n1[k].cnt > 0) link_into(n1[k].cnt, n1[k].off, n1[k].code, n1[k].pfx, row, mask);
+    const bool down = live && n1[k].cnt <= 0;
+    q2[2 * k] = n1[k].lc;
+    v2[2 * k] = down && !n1[k].leaf_l;
+    q2[2 * k + 1] = n1[k].lc + 1;
+    v2[2 * k + 1] = down && !n1[k].leaf_r;
+  }
+  PullNode n2[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) n2[k] = load_pull<true>(a, base, q2[k], v2[k]);
+  int q3[8];
+  bool v3[8];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (v2[k] && n2[k].cnt > 0) link_into(n2[k].cnt, n2[k].off, n2[k].code, n2[k].pfx, row, mask);
+    const bool down = v2[k] && n2[k].cnt <= 0;
+    q3[2 * k] = n2[k].lc;
+    v3[2 * k] = down && !n2[k].leaf_l;
+    q3[2 * k + 1] = n2[k].lc + 1;
+    v3[2 * k + 1] = down && !n2[k].leaf_r;
+  }
+  PullNode n3[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) n3[k] = load_pull<false>(a, base, q3[k], v3[k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (!v3[k]) continue;
+    if (n3[k].cnt > 0) {
+      link_into(n3[k].cnt, n3[k].off, n3[k].code, n3[k].pfx, row, mask);
+    } else {
+      int stack[16];
+      stack[0] = q3[k];
+      pull_links(a, base, stack, 1, row, mask);
+    }
   }
 }
 
@@ -136,10 +222,14 @@ __global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg
     return;
   }
   const int count = a.edge_count[sg.base + i];
-  if (count <= 0) return;
   const int64_t off = a.edge_offset[sg.base + i];
   const uint32_t ci = codes[i];
   const int lvl = prefix_n[i] / 3;
+  // the bottom's own children, loaded with the rest of the node
+  const int lc = a.left_child[sg.base + i];
+  const bool leaf_l = a.has_leaf_left[sg.base + i] != 0;
+  const bool leaf_r = a.has_leaf_right[sg.base + i] != 0;
+  if (count <= 0) return;
   const int64_t rows = os.rows;
   for (int j = 0; j < count; ++j) {
     const int64_t slot = off + j;
@@ -156,15 +246,11 @@ __global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg
       for (uint32_t w = 0; w < 8; ++w) r[w] = (w == which) ? static_cast<int32_t>(slot - 1) : 0;
       m = static_cast<int32_t>(1u << which);
     } else {  // the bottom: pull the chains that link here
-      int stack[16];
-      int sp = 0;
-      const int lc = a.left_child[sg.base + i];
-      if (!a.has_leaf_left[sg.base + i]) stack[sp++] = lc;
-      if (!a.has_leaf_right[sg.base + i]) stack[sp++] = lc + 1;
-      pull_links(a, sg.base, stack, sp, r, m);
+      pull_below(a, sg.base, lc, leaf_l, leaf_r, r, m);
       if (off == 0) {  // slot 0 is also offset[0]: node 0's links land here
         const int g = full_root_split(codes, n);
-        sp = 0;
+        int stack[16];
+        int sp = 0;
         if (g != 0) stack[sp++] = g;
         if (g + 1 < n) stack[sp++] = g + 1;
         pull_links(a, sg.base, stack, sp, r, m);

@@ -328,16 +328,20 @@ def pull_rows(o, nu, nb, no):
         row[0][which] = max(row[0][which], off[c] + cnt[c] - 1)
         row[1] |= 1 << which
 
-    def pull(stack, row):
+    def pull(stack, row, bounded):
+        # stack entries: (node, depth below the pulling bottom; children are 1)
         while stack:
-            q = stack.pop()
+            q, d = stack.pop()
             if cnt[q] > 0:
                 link(q, row)
                 continue
+            # below a node of count > 0, internal nodes three levels down always
+            # have count > 0 (the GPU walks those three levels breadth-first)
+            assert not (bounded and d >= 3), (q, d)
             if not hl[q]:
-                stack.append(lc[q])
+                stack.append((lc[q], d + 1))
             if not hr[q]:
-                stack.append(lc[q] + 1)
+                stack.append((lc[q] + 1, d + 1))
 
     def full_root_split():
         if n == 0:
@@ -351,14 +355,14 @@ def pull_rows(o, nu, nb, no):
         row = [[0] * 8, 0]
         st = []
         if not hl[p]:
-            st.append(lc[p])
+            st.append((lc[p], 1))
         if not hr[p]:
-            st.append(lc[p] + 1)
-        pull(st, row)
+            st.append((lc[p] + 1, 1))
+        pull(st, row, True)
         if off[p] == 0:
             g = full_root_split()
-            st = ([g] if g != 0 else []) + ([g + 1] if g + 1 < n else [])
-            pull(st, row)
+            st = ([(g, 1)] if g != 0 else []) + ([(g + 1, 1)] if g + 1 < n else [])
+            pull(st, row, False)
         rows[off[p]] = row
     return rows
 
