@@ -132,7 +132,8 @@ int rwh_prefix_sum(rwh_workspace* ws, const int32_t* d_edge_count, const uint32_
                    rwh_stream_t stream);
 
 /* Device buffers of one build, laid out as the reference's AppData
- * (apps/tree/app_data.hpp:48-96).  oct_capacity = floor(0.6 * n) there. */
+ * (apps/tree/app_data.hpp:48-96).  oct_capacity = floor(0.6 * n) there.
+ * Every array is device memory and 16-byte aligned (rwh_malloc gives 256). */
 typedef struct rwh_tree_buffers {
   const float* points;        /* n x (x,y,z,w) */
   uint32_t* morton_keys;      /* n: S1 codes, then S3 unique keys */

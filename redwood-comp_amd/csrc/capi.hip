@@ -180,6 +180,8 @@ int check_bufs(const rwh_tree_buffers* b) {
                 b->child_node_mask && b->child_leaf_mask,
             "a tree buffer is null");
   RWH_CHECK(aligned16(b->points) && aligned16(b->morton_keys) && aligned16(b->morton_keys_alt) &&
+                aligned16(b->prefix_n) && aligned16(b->has_leaf_left) &&
+                aligned16(b->has_leaf_right) && aligned16(b->left_child) && aligned16(b->parents) &&
                 aligned16(b->edge_count) && aligned16(b->edge_offset) && aligned16(b->children) &&
                 aligned16(b->corner),
             "tree buffers must be 16-byte aligned");

@@ -93,85 +93,6 @@ __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int*
   }
 }
 
-// What the pull needs of one radix node, loaded in one round trip.
-struct PullNode {
-  int cnt, off, lc;
-  uint32_t code;
-  int pfx;
-  bool leaf_l, leaf_r;
-};
-
-template <bool kWithChildren>
-__device__ __forceinline__ PullNode load_pull(const OctArgs& a, uint32_t base, int q, bool live) {
-  PullNode n{0, 0, 0, 0u, 0, true, true};
-  if (live) {
-    n.cnt = a.edge_count[base + q];
-    n.off = a.edge_offset[base + q];
-    n.code = a.codes[base + q];
-    n.pfx = a.prefix_n[base + q];
-    if (kWithChildren) {
-      n.lc = a.left_child[base + q];
-      n.leaf_l = a.has_leaf_left[base + q] != 0;
-      n.leaf_r = a.has_leaf_right[base + q] != 0;
-    }
-  }
-  return n;
-}
-
-// The pull of a bottom slot owned by brt node p >= 1, breadth-first: the
-// nodes of one level are loaded together, so the walk costs one memory round
-// trip per level instead of one per node.  Below p (count > 0) at most two
-// nodes of count 0 follow each other on a path (prefix lengths increase
-// strictly and a count-0 node has its parent's level), so every internal node
-// three levels down has a non-zero count; any that does not is still handled,
-// by the depth-first walk.
-__device__ __forceinline__ void pull_below(const OctArgs& a, uint32_t base, int lc, bool leaf_l,
-                                           bool leaf_r, int32_t (&row)[8], int32_t& mask) {
-  PullNode n1[2];
-  n1[0] = load_pull<true>(a, base, lc, !leaf_l);
-  n1[1] = load_pull<true>(a, base, lc + 1, !leaf_r);
-  int q2[4];
-  bool v2[4];
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const bool live = (k == 0) ? !leaf_l : !leaf_r;
-    if (live && n1[k].cnt > 0) link_into(n1[k].cnt, n1[k].off, n1[k].code, n1[k].pfx, row, mask);
-    const bool down = live && n1[k].cnt <= 0;
-    q2[2 * k] = n1[k].lc;
-    v2[2 * k] = down && !n1[k].leaf_l;
-    q2[2 * k + 1] = n1[k].lc + 1;
-    v2[2 * k + 1] = down && !n1[k].leaf_r;
-  }
-  PullNode n2[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) n2[k] = load_pull<true>(a, base, q2[k], v2[k]);
-  int q3[8];
-  bool v3[8];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (v2[k] && n2[k].cnt > 0) link_into(n2[k].cnt, n2[k].off, n2[k].code, n2[k].pfx, row, mask);
-    const bool down = v2[k] && n2[k].cnt <= 0;
-    q3[2 * k] = n2[k].lc;
-    v3[2 * k] = down && !n2[k].leaf_l;
-    q3[2 * k + 1] = n2[k].lc + 1;
-    v3[2 * k + 1] = down && !n2[k].leaf_r;
-  }
-  PullNode n3[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) n3[k] = load_pull<false>(a, base, q3[k], v3[k]);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (!v3[k]) continue;
-    if (n3[k].cnt > 0) {
-      link_into(n3[k].cnt, n3[k].off, n3[k].code, n3[k].pfx, row, mask);
-    } else {
-      int stack[16];
-      stack[0] = q3[k];
-      pull_links(a, base, stack, 1, row, mask);
-    }
-  }
-}
-
 __device__ __forceinline__ int delta8o(uint32_t x, uint32_t y) {
   return static_cast<uint8_t>(__clz(x ^ y) - 1);
 }
@@ -194,67 +115,290 @@ __device__ int full_root_split(const uint32_t* codes, int n) {
   return s;
 }
 
-// Corners, cell sizes and every row of the chain; the bottom row pulls its
-// links.  Node 0 writes the root cell when no node contributes (D8).
-__global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg) {
-  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
-  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
-  const int i = static_cast<int>(gi - sg.base);
+// ---------------------------------------------------------------------------
+// The node kernel.  A workgroup owns kOctSpan consecutive radix nodes (a
+// "tile"; thread t owns the four nodes starting at 4t) and first builds, in
+// LDS, a pull descriptor for every node of a window reaching kOctHalo nodes
+// beyond the tile on either side:
+//   count > 0:  x = top slot (offset + count - 1), y = kLinkBit | octant of
+//               the chain's top level (the reference's set_child arguments,
+//               func_octree.hpp:96-105);
+//   otherwise:  x = left_child, y = has_leaf_left | has_leaf_right << 1.
+// All descendants of a node lie inside its key range, so the pull of a bottom
+// whose range fits the window never leaves LDS; the rest read the same
+// descriptor from global memory.  The node arrays are read with 16-byte
+// (int) and 4-byte (byte arrays) loads, four nodes per thread.
+// ---------------------------------------------------------------------------
+constexpr int kOctBlock = 256;
+constexpr int kOctSpan = kOctBlock * 4;  // radix nodes per workgroup
+constexpr int kOctHalo = 128;            // window margin on each side (multiple of 4)
+constexpr int kOctWin = kOctSpan + 2 * kOctHalo;
+constexpr uint32_t kLinkBit = 0x80000000u;
+constexpr int kXcds = 8;  // MI355X deals workgroups to its 8 XCDs round robin
+static_assert(kChunk % kOctSpan == 0, "tiles must not straddle segments");
+
+__device__ __forceinline__ uint2 make_desc(int cnt, int off, uint32_t code, int pfx, int lc,
+                                           bool leaf_l, bool leaf_r) {
+  uint2 d;
+  if (cnt > 0) {
+    const int top_level = pfx / 3 - cnt + 1;
+    d.x = static_cast<uint32_t>(off + cnt - 1);
+    d.y = kLinkBit | ((code >> level_shift(top_level)) & 7u);
+  } else {
+    d.x = static_cast<uint32_t>(lc);
+    d.y = (leaf_l ? 1u : 0u) | (leaf_r ? 2u : 0u);
+  }
+  return d;
+}
+
+// Four consecutive radix nodes q0 .. q0+3 of a segment (q0 a multiple of 4);
+// nodes at or beyond n read as count 0.
+struct Nodes4 {
+  int cnt[4], off[4], lc[4];
+  uint32_t code[4];
+  uint32_t pfx, leaf_l, leaf_r;  // one byte per node
+  __device__ int prefix(int e) const { return static_cast<int>((pfx >> (8 * e)) & 0xffu); }
+  __device__ bool ll(int e) const { return ((leaf_l >> (8 * e)) & 0xffu) != 0; }
+  __device__ bool lr(int e) const { return ((leaf_r >> (8 * e)) & 0xffu) != 0; }
+  __device__ uint2 desc(int e) const {
+    return make_desc(cnt[e], off[e], code[e], prefix(e), lc[e], ll(e), lr(e));
+  }
+};
+
+__device__ __forceinline__ Nodes4 load_nodes4(const OctArgs& a, uint32_t base, int q0, int n) {
+  Nodes4 g;
+  const uint32_t gq = base + static_cast<uint32_t>(q0);
+  if (q0 + 4 <= n) {
+    const int4 c = *reinterpret_cast<const int4*>(a.edge_count + gq);
+    const int4 o = *reinterpret_cast<const int4*>(a.edge_offset + gq);
+    const int4 l = *reinterpret_cast<const int4*>(a.left_child + gq);
+    const uint4 k = *reinterpret_cast<const uint4*>(a.codes + gq);
+    g.cnt[0] = c.x; g.cnt[1] = c.y; g.cnt[2] = c.z; g.cnt[3] = c.w;
+    g.off[0] = o.x; g.off[1] = o.y; g.off[2] = o.z; g.off[3] = o.w;
+    g.lc[0] = l.x; g.lc[1] = l.y; g.lc[2] = l.z; g.lc[3] = l.w;
+    g.code[0] = k.x; g.code[1] = k.y; g.code[2] = k.z; g.code[3] = k.w;
+    g.pfx = *reinterpret_cast<const uint32_t*>(a.prefix_n + gq);
+    g.leaf_l = *reinterpret_cast<const uint32_t*>(a.has_leaf_left + gq);
+    g.leaf_r = *reinterpret_cast<const uint32_t*>(a.has_leaf_right + gq);
+  } else {
+    g.pfx = g.leaf_l = g.leaf_r = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool in = q0 + e < n;
+      g.cnt[e] = in ? a.edge_count[gq + e] : 0;
+      g.off[e] = in ? a.edge_offset[gq + e] : 0;
+      g.lc[e] = in ? a.left_child[gq + e] : 0;
+      g.code[e] = in ? a.codes[gq + e] : 0u;
+      if (in) {
+        g.pfx |= static_cast<uint32_t>(a.prefix_n[gq + e]) << (8 * e);
+        g.leaf_l |= static_cast<uint32_t>(a.has_leaf_left[gq + e]) << (8 * e);
+        g.leaf_r |= static_cast<uint32_t>(a.has_leaf_right[gq + e]) << (8 * e);
+      }
+    }
+  }
+  return g;
+}
+
+struct Window {
+  const uint2* desc;  // LDS, nodes [w0, w1)
+  int w0, w1;
+};
+
+__device__ __forceinline__ uint2 fetch_desc(const OctArgs& a, uint32_t base, const Window& w, int q) {
+  if (q >= w.w0 && q < w.w1) return w.desc[q - w.w0];
+  const uint32_t gq = base + static_cast<uint32_t>(q);
+  return make_desc(a.edge_count[gq], a.edge_offset[gq], a.codes[gq], a.prefix_n[gq],
+                   a.left_child[gq], a.has_leaf_left[gq] != 0, a.has_leaf_right[gq] != 0);
+}
+
+__device__ __forceinline__ void link_desc(uint2 d, int32_t (&row)[8], int32_t& mask) {
+  const uint32_t which = d.y & 7u;
+  const int32_t top = static_cast<int32_t>(d.x);
+#pragma unroll
+  for (uint32_t w = 0; w < 8; ++w) row[w] = (w == which) ? max(row[w], top) : row[w];
+  mask |= static_cast<int32_t>(1u << which);
+}
+
+// The pull of a bottom slot owned by radix node p >= 1 (children lc, lc + 1
+// unless leaves), breadth-first so that each level costs one round trip.
+// Below p (count > 0) at most two nodes of count 0 follow each other on a
+// path (prefix lengths increase strictly and a count-0 node has its parent's
+// level), so the internal nodes three levels down all have a non-zero count;
+// any that does not is still handled, by the depth-first walk.
+__device__ __forceinline__ void pull_below(const OctArgs& a, uint32_t base, const Window& w, int n,
+                                           int lc, bool leaf_l, bool leaf_r, int32_t (&row)[8],
+                                           int32_t& mask) {
+  int q1[2] = {lc, lc + 1};
+  bool v1[2] = {!leaf_l && lc < n, !leaf_r && lc + 1 < n};
+  uint2 d1[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) d1[k] = v1[k] ? fetch_desc(a, base, w, q1[k]) : make_uint2(0u, 3u);
+  int q2[4];
+  bool v2[4];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const bool link = v1[k] && (d1[k].y & kLinkBit);
+    if (link) link_desc(d1[k], row, mask);
+    const bool down = v1[k] && !link;
+    const int c = static_cast<int>(d1[k].x);
+    q2[2 * k] = c;
+    q2[2 * k + 1] = c + 1;
+    v2[2 * k] = down && !(d1[k].y & 1u) && c < n;
+    v2[2 * k + 1] = down && !(d1[k].y & 2u) && c + 1 < n;
+  }
+  uint2 d2[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d2[k] = v2[k] ? fetch_desc(a, base, w, q2[k]) : make_uint2(0u, 3u);
+  int q3[8];
+  bool v3[8];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool link = v2[k] && (d2[k].y & kLinkBit);
+    if (link) link_desc(d2[k], row, mask);
+    const bool down = v2[k] && !link;
+    const int c = static_cast<int>(d2[k].x);
+    q3[2 * k] = c;
+    q3[2 * k + 1] = c + 1;
+    v3[2 * k] = down && !(d2[k].y & 1u) && c < n;
+    v3[2 * k + 1] = down && !(d2[k].y & 2u) && c + 1 < n;
+  }
+  uint2 d3[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) d3[k] = v3[k] ? fetch_desc(a, base, w, q3[k]) : make_uint2(0u, 3u);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (!v3[k]) continue;
+    if (d3[k].y & kLinkBit) {
+      link_desc(d3[k], row, mask);
+    } else {
+      int stack[16];
+      stack[0] = q3[k];
+      pull_links(a, base, stack, 1, row, mask);
+    }
+  }
+}
+
+// Corners, cell sizes and every row of each chain; the bottom rows pull
+// their links.  Node 0 writes the root cell when no node contributes (D8).
+__global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
+  __shared__ uint2 s_desc[kOctWin];
+  __shared__ uint2 s_bottom[kOctSpan];  // (slot, left_child | leaf bits << 30)
+  __shared__ uint32_t s_nbottom;
+
+  // Tiles that share window margins are neighbours: keep runs of them on
+  // one XCD (one L2) by mapping workgroup b to tile (b % 8) * G/8 + b / 8.
+  const uint32_t tile = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+  if (tile >= n_tiles) return;
+  const uint32_t first = tile * kOctSpan;
+  const Seg sg = seg_of(seg, first);
+  if (first < sg.base) return;
+  const int b = static_cast<int>(first - sg.base);
   const int n = static_cast<int>(sg.count);  // n_brt
-  if (gi < sg.base || i >= n) return;
+  if (b >= n) return;  // a segment's padding (uniform across the workgroup)
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid % kWave;
   const OctSeg os = oct_seg(a, sg);
+  const int64_t rows = os.rows;
   const uint32_t* codes = a.codes + sg.base;
-  const uint8_t* prefix_n = a.prefix_n + sg.base;
-  const int root_level = prefix_n[0] / 3;
   int4* rows8 = reinterpret_cast<int4*>(a.children + static_cast<size_t>(os.base) * 8);
   float4* corner = a.corner + os.base;
   float* cell = a.cell_size + os.base;
   int32_t* mask = a.child_node_mask + os.base;
   int32_t* leaf_mask = a.child_leaf_mask + os.base;
 
-  if (i == 0) {
-    const uint32_t cap = a.oct_cap ? a.oct_cap[sg.id] : a.single_cap;
-    if (a.n_oct[sg.id] == 0 && cap > 0) {
-      const uint32_t sh = level_shift(root_level);
-      corner[0] = morton_decode((codes[0] >> sh) << sh, a.min_coord, a.range);
-      cell[0] = a.range;
+  Window w;
+  w.desc = s_desc;
+  w.w0 = max(b - kOctHalo, 0);
+  w.w1 = min(b + kOctSpan + kOctHalo, n);
+
+  if (tid == 0) s_nbottom = 0;
+  const int q0 = b + 4 * static_cast<int>(tid);
+  const Nodes4 g = load_nodes4(a, sg.base, q0, n);
+  constexpr int kHaloGroups = kOctHalo / 4;
+  if (tid < 2 * kHaloGroups) {
+    const int h = (tid < kHaloGroups) ? b - kOctHalo + 4 * static_cast<int>(tid)
+                                      : b + kOctSpan + 4 * static_cast<int>(tid - kHaloGroups);
+    if (h >= w.w0 && h < w.w1) {
+      const Nodes4 hg = load_nodes4(a, sg.base, h, n);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (h + e < w.w1) s_desc[h + e - w.w0] = hg.desc(e);
     }
-    return;
   }
-  const int count = a.edge_count[sg.base + i];
-  const int64_t off = a.edge_offset[sg.base + i];
-  const uint32_t ci = codes[i];
-  const int lvl = prefix_n[i] / 3;
-  // the bottom's own children, loaded with the rest of the node
-  const int lc = a.left_child[sg.base + i];
-  const bool leaf_l = a.has_leaf_left[sg.base + i] != 0;
-  const bool leaf_r = a.has_leaf_right[sg.base + i] != 0;
-  if (count <= 0) return;
-  const int64_t rows = os.rows;
-  for (int j = 0; j < count; ++j) {
-    const int64_t slot = off + j;
-    if (slot < 0 || slot >= rows) continue;
-    const int level = lvl - j;
-    const uint32_t sh = level_shift(level);
-    corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
-    cell[slot] = cell_of(a.range, level - root_level);
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    if (q0 + e < n) s_desc[q0 + e - w.w0] = g.desc(e);
+  __syncthreads();
+
+  const int root_level = a.prefix_n[sg.base] / 3;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int i = q0 + e;
+    const int count = g.cnt[e];
+    const int64_t off = g.off[e];
+    const bool live = i > 0 && i < n && count > 0;
+    if (live) {
+      const uint32_t ci = g.code[e];
+      const int lvl = g.prefix(e) / 3;
+      for (int j = 0; j < count; ++j) {
+        const int64_t slot = off + j;
+        if (slot < 0 || slot >= rows) continue;
+        const int level = lvl - j;
+        const uint32_t sh = level_shift(level);
+        corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
+        cell[slot] = cell_of(a.range, level - root_level);
+        if (j > 0) {  // the child below, in the same chain
+          const uint32_t which = (ci >> level_shift(lvl - (j - 1))) & 7u;
+          int32_t r[8];
+#pragma unroll
+          for (uint32_t k = 0; k < 8; ++k) r[k] = (k == which) ? static_cast<int32_t>(slot - 1) : 0;
+          rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
+          rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
+          mask[slot] = static_cast<int32_t>(1u << which);
+          leaf_mask[slot] = 0;
+        }
+      }
+    } else if (i == 0) {
+      const uint32_t cap = a.oct_cap ? a.oct_cap[sg.id] : a.single_cap;
+      if (a.n_oct[sg.id] == 0 && cap > 0) {
+        const uint32_t sh = level_shift(root_level);
+        corner[0] = morton_decode((codes[0] >> sh) << sh, a.min_coord, a.range);
+        cell[0] = a.range;
+      }
+    }
+    // queue the chain bottom for the pull (wave-aggregated LDS append)
+    const bool bottom = live && off >= 0 && off < rows;
+    const uint64_t bal = ballot(bottom);
+    if (bal) {
+      const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(bal)) - 1);
+      uint32_t at = 0;
+      if (lane == leader) at = atomicAdd(&s_nbottom, static_cast<uint32_t>(__popcll(bal)));
+      at = __shfl(at, leader, kWave);
+      if (bottom)
+        s_bottom[at + popc_below(bal)] =
+            make_uint2(static_cast<uint32_t>(off),
+                       (static_cast<uint32_t>(g.lc[e]) & 0x3fffffffu) | (g.ll(e) ? 1u << 30 : 0u) |
+                           (g.lr(e) ? 1u << 31 : 0u));
+    }
+  }
+  __syncthreads();
+
+  // The bottoms, one per thread: pull, then write the row.
+  const uint32_t n_bottom = s_nbottom;
+  for (uint32_t e = tid; e < n_bottom; e += kOctBlock) {
+    const uint2 ent = s_bottom[e];
+    const int64_t slot = ent.x;
     int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int32_t m = 0;
-    if (j > 0) {  // the child below, in the same chain
-      const uint32_t which = (ci >> level_shift(lvl - (j - 1))) & 7u;
-#pragma unroll
-      for (uint32_t w = 0; w < 8; ++w) r[w] = (w == which) ? static_cast<int32_t>(slot - 1) : 0;
-      m = static_cast<int32_t>(1u << which);
-    } else {  // the bottom: pull the chains that link here
-      pull_below(a, sg.base, lc, leaf_l, leaf_r, r, m);
-      if (off == 0) {  // slot 0 is also offset[0]: node 0's links land here
-        const int g = full_root_split(codes, n);
-        int stack[16];
-        int sp = 0;
-        if (g != 0) stack[sp++] = g;
-        if (g + 1 < n) stack[sp++] = g + 1;
-        pull_links(a, sg.base, stack, sp, r, m);
-      }
+    pull_below(a, sg.base, w, n, static_cast<int>(ent.y & 0x3fffffffu), (ent.y >> 30) & 1u,
+               (ent.y >> 31) & 1u, r, m);
+    if (slot == 0) {  // slot 0 is also offset[0]: node 0's links land here
+      const int gsplit = full_root_split(codes, n);
+      int stack[16];
+      int sp = 0;
+      if (gsplit != 0) stack[sp++] = gsplit;
+      if (gsplit + 1 < n) stack[sp++] = gsplit + 1;
+      pull_links(a, sg.base, stack, sp, r, m);
     }
     rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
     rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
@@ -264,10 +408,11 @@ __global__ __launch_bounds__(kElemBlock) void k_oct_nodes(OctArgs a, SegArgs seg
 }
 
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
-  const uint32_t grid = (n_index + kElemBlock - 1) / kElemBlock;
-  if (grid == 0) return;
+  const uint32_t n_tiles = (n_index + kOctSpan - 1) / kOctSpan;
+  if (n_tiles == 0) return;
+  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_oct_nodes, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
+  hipLaunchKernelGGL(k_oct_nodes, dim3(grid), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
 }
 
 // Leaf linking (opt-in): each leaf child of brt node i goes into the bottom
