@@ -278,12 +278,21 @@ __device__ __forceinline__ void pull_below(const OctArgs& a, uint32_t base, cons
   }
 }
 
-// Corners, cell sizes and every row of each chain; the bottom rows pull
-// their links.  Node 0 writes the root cell when no node contributes (D8).
+// Corners, cell sizes and rows.  After the window, the workgroup walks the
+// contiguous slot range its nodes own ([offset of its first node, offset +
+// count of its last node)), one slot per thread, so every store is
+// coalesced: the owner of slot s is the last node of the tile whose offset is
+// <= s (offsets are an exclusive scan of the counts), found by a binary search
+// in LDS.  Slot offset + j is the node j levels above the deepest one; j = 0
+// is the chain's bottom, whose row is pulled.  Node 0 writes the root cell
+// when no node contributes (D8).
 __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
   __shared__ uint2 s_desc[kOctWin];
-  __shared__ uint2 s_bottom[kOctSpan];  // (slot, left_child | leaf bits << 30)
-  __shared__ uint32_t s_nbottom;
+  __shared__ int32_t s_off[kOctSpan];
+  __shared__ uint32_t s_code[kOctSpan];
+  __shared__ uint32_t s_lcf[kOctSpan];  // left_child | has_leaf_left << 30 | has_leaf_right << 31
+  __shared__ uint32_t s_lvl[kOctSpan / 4];  // prefix_n / 3, one byte per node
+  __shared__ int64_t s_end;
 
   // Tiles that share window margins are neighbours: keep runs of them on
   // one XCD (one L2) by mapping workgroup b to tile (b % 8) * G/8 + b / 8.
@@ -296,7 +305,6 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   const int n = static_cast<int>(sg.count);  // n_brt
   if (b >= n) return;  // a segment's padding (uniform across the workgroup)
   const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid % kWave;
   const OctSeg os = oct_seg(a, sg);
   const int64_t rows = os.rows;
   const uint32_t* codes = a.codes + sg.base;
@@ -305,13 +313,14 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   float* cell = a.cell_size + os.base;
   int32_t* mask = a.child_node_mask + os.base;
   int32_t* leaf_mask = a.child_leaf_mask + os.base;
+  const int nv = min(kOctSpan, n - b);  // valid nodes of the tile
 
   Window w;
   w.desc = s_desc;
   w.w0 = max(b - kOctHalo, 0);
   w.w1 = min(b + kOctSpan + kOctHalo, n);
 
-  if (tid == 0) s_nbottom = 0;
+  // 1. the tile's nodes (four per thread) and the window's descriptors
   const int q0 = b + 4 * static_cast<int>(tid);
   const Nodes4 g = load_nodes4(a, sg.base, q0, n);
   constexpr int kHaloGroups = kOctHalo / 4;
@@ -325,80 +334,68 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
         if (h + e < w.w1) s_desc[h + e - w.w0] = hg.desc(e);
     }
   }
+  if (q0 < n) {
+    uint32_t lcf[4], lv = 0;
 #pragma unroll
-  for (int e = 0; e < 4; ++e)
-    if (q0 + e < n) s_desc[q0 + e - w.w0] = g.desc(e);
-  __syncthreads();
-
-  const int root_level = a.prefix_n[sg.base] / 3;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int i = q0 + e;
-    const int count = g.cnt[e];
-    const int64_t off = g.off[e];
-    const bool live = i > 0 && i < n && count > 0;
-    if (live) {
-      const uint32_t ci = g.code[e];
-      const int lvl = g.prefix(e) / 3;
-      for (int j = 0; j < count; ++j) {
-        const int64_t slot = off + j;
-        if (slot < 0 || slot >= rows) continue;
-        const int level = lvl - j;
-        const uint32_t sh = level_shift(level);
-        corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
-        cell[slot] = cell_of(a.range, level - root_level);
-        if (j > 0) {  // the child below, in the same chain
-          const uint32_t which = (ci >> level_shift(lvl - (j - 1))) & 7u;
-          int32_t r[8];
-#pragma unroll
-          for (uint32_t k = 0; k < 8; ++k) r[k] = (k == which) ? static_cast<int32_t>(slot - 1) : 0;
-          rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
-          rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
-          mask[slot] = static_cast<int32_t>(1u << which);
-          leaf_mask[slot] = 0;
-        }
-      }
-    } else if (i == 0) {
-      const uint32_t cap = a.oct_cap ? a.oct_cap[sg.id] : a.single_cap;
-      if (a.n_oct[sg.id] == 0 && cap > 0) {
-        const uint32_t sh = level_shift(root_level);
-        corner[0] = morton_decode((codes[0] >> sh) << sh, a.min_coord, a.range);
-        cell[0] = a.range;
-      }
+    for (int e = 0; e < 4; ++e) {
+      if (q0 + e < n) s_desc[q0 + e - w.w0] = g.desc(e);
+      lcf[e] = (static_cast<uint32_t>(g.lc[e]) & 0x3fffffffu) | (g.ll(e) ? 1u << 30 : 0u) |
+               (g.lr(e) ? 1u << 31 : 0u);
+      lv |= static_cast<uint32_t>(g.prefix(e) / 3) << (8 * e);
+      // the last valid node closes the tile's slot range
+      if (q0 + e == b + nv - 1) s_end = static_cast<int64_t>(g.off[e]) + max(g.cnt[e], 0);
     }
-    // queue the chain bottom for the pull (wave-aggregated LDS append)
-    const bool bottom = live && off >= 0 && off < rows;
-    const uint64_t bal = ballot(bottom);
-    if (bal) {
-      const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(bal)) - 1);
-      uint32_t at = 0;
-      if (lane == leader) at = atomicAdd(&s_nbottom, static_cast<uint32_t>(__popcll(bal)));
-      at = __shfl(at, leader, kWave);
-      if (bottom)
-        s_bottom[at + popc_below(bal)] =
-            make_uint2(static_cast<uint32_t>(off),
-                       (static_cast<uint32_t>(g.lc[e]) & 0x3fffffffu) | (g.ll(e) ? 1u << 30 : 0u) |
-                           (g.lr(e) ? 1u << 31 : 0u));
+    reinterpret_cast<int4*>(s_off)[tid] = make_int4(g.off[0], g.off[1], g.off[2], g.off[3]);
+    reinterpret_cast<uint4*>(s_code)[tid] = make_uint4(g.code[0], g.code[1], g.code[2], g.code[3]);
+    reinterpret_cast<uint4*>(s_lcf)[tid] = make_uint4(lcf[0], lcf[1], lcf[2], lcf[3]);
+    s_lvl[tid] = lv;
+  }
+  const int root_level = a.prefix_n[sg.base] / 3;
+  if (b == 0 && tid == 0) {
+    const uint32_t cap = a.oct_cap ? a.oct_cap[sg.id] : a.single_cap;
+    if (a.n_oct[sg.id] == 0 && cap > 0) {
+      const uint32_t sh = level_shift(root_level);
+      corner[0] = morton_decode((codes[0] >> sh) << sh, a.min_coord, a.range);
+      cell[0] = a.range;
     }
   }
   __syncthreads();
 
-  // The bottoms, one per thread: pull, then write the row.
-  const uint32_t n_bottom = s_nbottom;
-  for (uint32_t e = tid; e < n_bottom; e += kOctBlock) {
-    const uint2 ent = s_bottom[e];
-    const int64_t slot = ent.x;
+  // 2. the slots, one per thread
+  const int64_t lo = max<int64_t>(s_off[0], 0);
+  const int64_t hi = min<int64_t>(s_end, rows);
+  for (int64_t slot = lo + tid; slot < hi; slot += kOctBlock) {
+    int k = 0;  // last node with offset <= slot
+#pragma unroll
+    for (int step = kOctSpan / 2; step >= 1; step >>= 1)
+      if (k + step < nv && s_off[k + step] <= slot) k += step;
+    if (b + k == 0) continue;  // node 0 owns no chain (the reference starts at 1)
+    const int j = static_cast<int>(slot - s_off[k]);
+    const uint32_t ci = s_code[k];
+    const int lvl = static_cast<int>((s_lvl[k >> 2] >> (8 * (k & 3))) & 0xffu);
+    const int level = lvl - j;
+    const uint32_t sh = level_shift(level);
+    corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
+    cell[slot] = cell_of(a.range, level - root_level);
     int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int32_t m = 0;
-    pull_below(a, sg.base, w, n, static_cast<int>(ent.y & 0x3fffffffu), (ent.y >> 30) & 1u,
-               (ent.y >> 31) & 1u, r, m);
-    if (slot == 0) {  // slot 0 is also offset[0]: node 0's links land here
-      const int gsplit = full_root_split(codes, n);
-      int stack[16];
-      int sp = 0;
-      if (gsplit != 0) stack[sp++] = gsplit;
-      if (gsplit + 1 < n) stack[sp++] = gsplit + 1;
-      pull_links(a, sg.base, stack, sp, r, m);
+    if (j > 0) {  // the child below, in the same chain
+      const uint32_t which = (ci >> level_shift(lvl - (j - 1))) & 7u;
+#pragma unroll
+      for (uint32_t c = 0; c < 8; ++c) r[c] = (c == which) ? static_cast<int32_t>(slot - 1) : 0;
+      m = static_cast<int32_t>(1u << which);
+    } else {  // the bottom: pull the chains that link here
+      const uint32_t lcf = s_lcf[k];
+      pull_below(a, sg.base, w, n, static_cast<int>(lcf & 0x3fffffffu), (lcf >> 30) & 1u,
+                 (lcf >> 31) & 1u, r, m);
+      if (slot == 0) {  // slot 0 is also offset[0]: node 0's links land here
+        const int gsplit = full_root_split(codes, n);
+        int stack[16];
+        int sp = 0;
+        if (gsplit != 0) stack[sp++] = gsplit;
+        if (gsplit + 1 < n) stack[sp++] = gsplit + 1;
+        pull_links(a, sg.base, stack, sp, r, m);
+      }
     }
     rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
     rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
