@@ -134,7 +134,6 @@ constexpr int kOctSpan = kOctBlock * 4;  // radix nodes per workgroup
 constexpr int kOctHalo = 128;            // window margin on each side (multiple of 4)
 constexpr int kOctWin = kOctSpan + 2 * kOctHalo;
 constexpr uint32_t kLinkBit = 0x80000000u;
-constexpr int kXcds = 8;  // MI355X deals workgroups to its 8 XCDs round robin
 static_assert(kChunk % kOctSpan == 0, "tiles must not straddle segments");
 
 __device__ __forceinline__ uint2 make_desc(int cnt, int off, uint32_t code, int pfx, int lc,
@@ -288,15 +287,14 @@ __device__ __forceinline__ void pull_below(const OctArgs& a, uint32_t base, cons
 // when no node contributes (D8).
 __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
   __shared__ uint2 s_desc[kOctWin];
-  __shared__ int32_t s_off[kOctSpan];
-  __shared__ uint32_t s_code[kOctSpan];
-  __shared__ uint32_t s_lcf[kOctSpan];  // left_child | has_leaf_left << 30 | has_leaf_right << 31
+  __shared__ __align__(16) int32_t s_off[kOctSpan];
+  __shared__ __align__(16) uint32_t s_code[kOctSpan];
+  __shared__ __align__(16) uint32_t s_lcf[kOctSpan];  // left_child | leaf_l << 30 | leaf_r << 31
   __shared__ uint32_t s_lvl[kOctSpan / 4];  // prefix_n / 3, one byte per node
   __shared__ int64_t s_end;
 
-  // Tiles that share window margins are neighbours: keep runs of them on
-  // one XCD (one L2) by mapping workgroup b to tile (b % 8) * G/8 + b / 8.
-  const uint32_t tile = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+  // neighbouring tiles share window margins: XCD-grouped order
+  const uint32_t tile = xcd_tile();
   if (tile >= n_tiles) return;
   const uint32_t first = tile * kOctSpan;
   const Seg sg = seg_of(seg, first);
@@ -407,7 +405,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
   const uint32_t n_tiles = (n_index + kOctSpan - 1) / kOctSpan;
   if (n_tiles == 0) return;
-  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+  const uint32_t grid = xcd_grid(n_tiles);
   seg.count_sub = 1;
   hipLaunchKernelGGL(k_oct_nodes, dim3(grid), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
 }

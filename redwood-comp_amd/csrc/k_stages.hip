@@ -220,7 +220,9 @@ struct CodeWindow {
   }
 };
 
-constexpr int kTreeHalo = 768;  // codes cached on each side of the workgroup's nodes
+constexpr int kTreeBlock = 1024;  // radix nodes per workgroup, one per thread
+constexpr int kTreeHalo = 768;    // codes cached on each side of the workgroup's nodes
+static_assert(kChunk % kTreeBlock == 0, "tiles must not straddle segments");
 
 // Split search of the reference (func_brt.hpp:90-104) for node i with range
 // length l, direction d; returns gamma.
@@ -238,64 +240,96 @@ __device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, 
   return i + s * d + min(d, 0);
 }
 
-__global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs seg) {
-  __shared__ uint32_t s_codes[kElemBlock + 2 * kTreeHalo];
-  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
-  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
-  const int n = static_cast<int>(sg.count);  // n_brt; codes hold n + 1 keys
-  const int b0 = static_cast<int>(blockIdx.x * kElemBlock - sg.base);
-  if (b0 >= n) return;  // whole workgroup past the segment's nodes
-  const uint32_t* __restrict__ gcodes = a.codes + sg.base;
-  CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kElemBlock + kTreeHalo, n + 1)};
-  for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kElemBlock)
-    s_codes[k - codes.lo] = gcodes[k];
-  __syncthreads();
-  const int i = static_cast<int>(gi - sg.base);
-  if (i >= n) return;
-  int32_t* __restrict__ parent = a.parent + sg.base;
-  const uint32_t ci = codes[i];
-
-  int d, l = 0;
-  if (i == 0) {
-    d = 1;
-    l = n - 1;  // Q2
-  } else {
-    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
-    d = (diff > 0) - (diff < 0);
-    const int dmin = delta8(ci, codes[i - d]);
-    int lmax = 2;
-    while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
-    const int cutoff = (d == -1) ? i : n - i;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-      if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
-  }
-  const int j = i + l * d;
-  const int dnode = delta8(ci, codes[j]);
-  const int gamma = brt_split(codes, i, n, d, l, dnode, ci);
-  const bool ll = min(i, j) == gamma;
-  const bool lr = max(i, j) == gamma + 1;
-  a.prefix_n[sg.base + i] = static_cast<uint8_t>(dnode);
-  a.left_child[sg.base + i] = gamma;
-  a.has_leaf_left[sg.base + i] = ll ? 1 : 0;
-  a.has_leaf_right[sg.base + i] = lr ? 1 : 0;
-  if (i > 0) {
-    if (!ll) parent[gamma] = i;
-    if (!lr) parent[gamma + 1] = i;
-  } else {
-    parent[0] = 0;
-    // children of the root over the complete range [0, n]
-    const int dfull = delta8(ci, codes[n]);
-    const int g = brt_split(codes, 0, n, 1, n, dfull, ci);
-    if (g != 0) parent[g] = 0;
-    if (g + 1 < n) parent[g + 1] = 0;
+// Writes the tile's nv leading entries of an LDS array to global memory, 16
+// bytes per lane where the group is whole (dst 16-byte aligned).
+template <typename T>
+__device__ __forceinline__ void flush_tile(T* __restrict__ dst, const T* src, int nv, int lane_id,
+                                           int lanes) {
+  constexpr int kPer = 16 / sizeof(T);
+  for (int g = lane_id; g * kPer < nv; g += lanes) {
+    if ((g + 1) * kPer <= nv) {
+      *reinterpret_cast<uint4*>(dst + g * kPer) = reinterpret_cast<const uint4*>(src)[g];
+    } else {
+      for (int e = g * kPer; e < nv; ++e) dst[e] = src[e];
+    }
   }
 }
 
+__global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs seg, uint32_t n_tiles) {
+  __shared__ uint32_t s_codes[kTreeBlock + 2 * kTreeHalo];
+  __shared__ __align__(16) int32_t s_lc[kTreeBlock];
+  __shared__ __align__(16) uint8_t s_pfx[kTreeBlock];
+  __shared__ __align__(16) uint8_t s_ll[kTreeBlock];
+  __shared__ __align__(16) uint8_t s_lr[kTreeBlock];
+  // neighbouring tiles share window codes: XCD-grouped order
+  const uint32_t tile = xcd_tile();
+  if (tile >= n_tiles) return;
+  const uint32_t first = tile * kTreeBlock;
+  const Seg sg = seg_of(seg, first);
+  const int n = static_cast<int>(sg.count);  // n_brt; codes hold n + 1 keys
+  const int b0 = static_cast<int>(first - sg.base);
+  if (first < sg.base || b0 >= n) return;  // whole workgroup past the segment's nodes
+  const uint32_t* __restrict__ gcodes = a.codes + sg.base;
+  CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kTreeBlock + kTreeHalo, n + 1)};
+  for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kTreeBlock)
+    s_codes[k - codes.lo] = gcodes[k];
+  __syncthreads();
+  const int i = b0 + static_cast<int>(threadIdx.x);
+  if (i < n) {
+    int32_t* __restrict__ parent = a.parent + sg.base;
+    const uint32_t ci = codes[i];
+    int d, l = 0;
+    if (i == 0) {
+      d = 1;
+      l = n - 1;  // Q2
+    } else {
+      const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
+      d = (diff > 0) - (diff < 0);
+      const int dmin = delta8(ci, codes[i - d]);
+      int lmax = 2;
+      while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
+      const int cutoff = (d == -1) ? i : n - i;
+      for (int t = lmax / 2; t >= 1; t /= 2)
+        if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
+    }
+    const int j = i + l * d;
+    const int dnode = delta8(ci, codes[j]);
+    const int gamma = brt_split(codes, i, n, d, l, dnode, ci);
+    const bool ll = min(i, j) == gamma;
+    const bool lr = max(i, j) == gamma + 1;
+    s_pfx[threadIdx.x] = static_cast<uint8_t>(dnode);
+    s_lc[threadIdx.x] = gamma;
+    s_ll[threadIdx.x] = ll ? 1 : 0;
+    s_lr[threadIdx.x] = lr ? 1 : 0;
+    if (i > 0) {
+      if (!ll) parent[gamma] = i;
+      if (!lr) parent[gamma + 1] = i;
+    } else {
+      parent[0] = 0;
+      // children of the root over the complete range [0, n]
+      const int dfull = delta8(ci, codes[n]);
+      const int g = brt_split(codes, 0, n, 1, n, dfull, ci);
+      if (g != 0) parent[g] = 0;
+      if (g + 1 < n) parent[g + 1] = 0;
+    }
+  }
+  __syncthreads();
+  // the tile's per-node outputs, 16 bytes per lane
+  const int nv = min(kTreeBlock, n - b0);
+  const uint32_t at = sg.base + static_cast<uint32_t>(b0);
+  const int t = static_cast<int>(threadIdx.x);
+  if (t < 64) flush_tile(a.prefix_n + at, s_pfx, nv, t, 64);
+  else if (t < 128) flush_tile(a.has_leaf_left + at, s_ll, nv, t - 64, 64);
+  else if (t < 192) flush_tile(a.has_leaf_right + at, s_lr, nv, t - 128, 64);
+  else if (t >= 256 && t < 512) flush_tile(a.left_child + at, s_lc, nv, t - 256, 256);
+}
+
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
-  const uint32_t grid = (n_index + kElemBlock - 1) / kElemBlock;
-  if (grid == 0) return;
+  const uint32_t n_tiles = (n_index + kTreeBlock - 1) / kTreeBlock;
+  if (n_tiles == 0) return;
+  const uint32_t grid = xcd_grid(n_tiles);
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_radix_tree, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
+  hipLaunchKernelGGL(k_radix_tree, dim3(grid), dim3(kTreeBlock), 0, stream, a, seg, n_tiles);
 }
 
 // ---------------------------------------------------------------------------

@@ -193,6 +193,18 @@ __device__ __forceinline__ Grab grab_tiles(uint32_t* ctrl, uint32_t n_grabs) {
 }
 
 // ---------------------------------------------------------------------------
+// XCD-grouped tile order.  MI355X deals the workgroups of a launch to its 8
+// XCDs round robin, and each XCD has its own L2.  Kernels whose neighbouring
+// tiles read overlapping input launch a grid padded to a multiple of 8
+// (xcd_grid) and map workgroup b to tile (b % 8) * (G / 8) + b / 8, so each
+// XCD walks a contiguous run of tiles and the overlap hits its L2.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_tile() {
+  return (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+}
+
 // Morton code: 10 bits per axis, x in bit 0 (apps/tree/host/func_morton.hpp:15-39).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {

@@ -32,6 +32,9 @@ constexpr uint32_t kScanTile = kScanBlock * kScanKpt;  // 8192
 // Elementwise tiles.
 constexpr int kElemBlock = 256;
 
+// Grid of n_tiles workgroups padded for xcd_tile().
+inline uint32_t xcd_grid(uint32_t n_tiles) { return (n_tiles + kXcds - 1) / kXcds * kXcds; }
+
 struct SortLaunch {
   const uint32_t* keys_in;
   uint32_t* keys_out;
