@@ -240,13 +240,38 @@ __device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, 
   return i + s * d + min(d, 0);
 }
 
+// Largest k in [0, K] with delta8(ci, codes[i + k*d]) > ref, for a predicate
+// that holds at k = 0 and is monotone in k (true, then false).  Sorted unique
+// keys below 2^31 make delta8(ci, codes[i + k*d]) non-increasing in k (no
+// uint8 wrap), and then the reference's doubling + binary search and its
+// split search both return exactly this boundary.  The whole wave searches
+// for one lane: 64 probes per round trip, so a range of 2^24 takes 4-5 round
+// trips instead of 24 dependent ones.  Wave-uniform arguments; every lane
+// must call it.
+__device__ int wave_boundary(const CodeWindow& codes, int i, int d, uint32_t ci, int ref, int K) {
+  const uint32_t lane = lane_id();
+  int lo = 0, hi = K;
+  while (lo < hi) {
+    const uint64_t span = static_cast<uint64_t>(hi - lo);
+    const int k = lo + static_cast<int>(((lane + 1) * span + 63) / 64);  // lane 63 probes hi
+    const bool p = delta8(ci, codes[i + k * d]) > ref;
+    const uint32_t c = static_cast<uint32_t>(__popcll(ballot(p)));  // probes that hold
+    if (c == 64) return hi;
+    const int kc = lo + static_cast<int>(((c + 1) * span + 63) / 64);  // first probe that fails
+    const int kp = (c == 0) ? lo : lo + static_cast<int>((c * span + 63) / 64);
+    lo = kp;
+    hi = kc - 1;
+  }
+  return lo;
+}
+
 // Writes the tile's nv leading entries of an LDS array to global memory, 16
 // bytes per lane where the group is whole (dst 16-byte aligned).
 template <typename T>
-__device__ __forceinline__ void flush_tile(T* __restrict__ dst, const T* src, int nv, int lane_id,
+__device__ __forceinline__ void flush_tile(T* __restrict__ dst, const T* src, int nv, int first_group,
                                            int lanes) {
   constexpr int kPer = 16 / sizeof(T);
-  for (int g = lane_id; g * kPer < nv; g += lanes) {
+  for (int g = first_group; g * kPer < nv; g += lanes) {
     if ((g + 1) * kPer <= nv) {
       *reinterpret_cast<uint4*>(dst + g * kPer) = reinterpret_cast<const uint4*>(src)[g];
     } else {
@@ -275,26 +300,73 @@ __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs s
     s_codes[k - codes.lo] = gcodes[k];
   __syncthreads();
   const int i = b0 + static_cast<int>(threadIdx.x);
-  if (i < n) {
-    int32_t* __restrict__ parent = a.parent + sg.base;
-    const uint32_t ci = codes[i];
-    int d, l = 0;
-    if (i == 0) {
-      d = 1;
-      l = n - 1;  // Q2
-    } else {
-      const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
-      d = (diff > 0) - (diff < 0);
-      const int dmin = delta8(ci, codes[i - d]);
-      int lmax = 2;
+  const bool valid = i < n;
+  const uint32_t lane = threadIdx.x % kWave;
+  // keys below 2^31 (the largest is the last): searches are boundary searches
+  const bool mono = gcodes[n] < 0x80000000u;
+  const uint32_t ci = valid ? codes[i] : 0u;
+
+  // Range: direction d and length l (func_brt.hpp:57-88).  A lane is "big"
+  // when its doubling probe leaves the LDS window.
+  int d = 1, l = 0, dmin = 0, cutoff = 0, lmax = 2;
+  bool big = false;
+  if (valid && i == 0) {
+    l = n - 1;  // Q2
+  } else if (valid) {
+    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
+    d = (diff > 0) - (diff < 0);
+    dmin = delta8(ci, codes[i - d]);
+    cutoff = (d == -1) ? i : n - i;
+    for (;;) {
+      const int pos = i + lmax * d;
+      if (pos < 0 || pos > n) break;
+      if (pos < codes.lo || pos >= codes.hi) {
+        big = true;
+        break;
+      }
+      if (!(delta8(ci, s_codes[pos - codes.lo]) > dmin)) break;
+      lmax *= 2;
+    }
+    if (!big || !mono) {  // the reference's own search
       while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
-      const int cutoff = (d == -1) ? i : n - i;
       for (int t = lmax / 2; t >= 1; t /= 2)
         if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
     }
-    const int j = i + l * d;
-    const int dnode = delta8(ci, codes[j]);
-    const int gamma = brt_split(codes, i, n, d, l, dnode, ci);
+  }
+  for (uint64_t m = ballot(valid && big && mono); m; m &= m - 1) {
+    const int o = __ffsll(static_cast<long long>(m)) - 1;
+    const int ol = wave_boundary(codes, __shfl(i, o, kWave), __shfl(d, o, kWave),
+                                 __shfl(ci, o, kWave), __shfl(dmin, o, kWave),
+                                 __shfl(cutoff, o, kWave));
+    if (static_cast<int>(lane) == o) l = ol;
+  }
+
+  // Split (func_brt.hpp:90-104); big when the far end is outside the window.
+  const int j = i + l * d;
+  const int dnode = valid ? delta8(ci, codes[j]) : 0;
+  const bool sbig = valid && (j < codes.lo || j >= codes.hi);
+  int gamma = 0;
+  if (valid && (!sbig || !mono)) gamma = brt_split(codes, i, n, d, l, dnode, ci);
+  for (uint64_t m = ballot(sbig && mono); m; m &= m - 1) {
+    const int o = __ffsll(static_cast<long long>(m)) - 1;
+    const int oi = __shfl(i, o, kWave), od = __shfl(d, o, kWave);
+    const int os = wave_boundary(codes, oi, od, __shfl(ci, o, kWave), __shfl(dnode, o, kWave),
+                                 max(__shfl(l, o, kWave) - 1, 0));
+    if (static_cast<int>(lane) == o) gamma = oi + os * od + min(od, 0);
+  }
+
+  // Node 0 also parents the internal children of the root over the complete
+  // range [0, n] (split with l = n); its wave searches together.
+  int groot = 0;
+  if (b0 == 0 && threadIdx.x < kWave) {
+    const uint32_t c0 = __shfl(ci, 0, kWave);
+    const int dfull = delta8(c0, codes[n]);
+    if (mono) groot = wave_boundary(codes, 0, 1, c0, dfull, max(n - 1, 0));
+    else if (lane == 0) groot = brt_split(codes, 0, n, 1, n, dfull, c0);
+  }
+
+  if (valid) {
+    int32_t* __restrict__ parent = a.parent + sg.base;
     const bool ll = min(i, j) == gamma;
     const bool lr = max(i, j) == gamma + 1;
     s_pfx[threadIdx.x] = static_cast<uint8_t>(dnode);
@@ -306,11 +378,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs s
       if (!lr) parent[gamma + 1] = i;
     } else {
       parent[0] = 0;
-      // children of the root over the complete range [0, n]
-      const int dfull = delta8(ci, codes[n]);
-      const int g = brt_split(codes, 0, n, 1, n, dfull, ci);
-      if (g != 0) parent[g] = 0;
-      if (g + 1 < n) parent[g + 1] = 0;
+      if (groot != 0) parent[groot] = 0;
+      if (groot + 1 < n) parent[groot + 1] = 0;
     }
   }
   __syncthreads();
