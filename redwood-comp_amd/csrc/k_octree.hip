@@ -54,10 +54,6 @@ __device__ __forceinline__ uint32_t level_shift(int level) {
   return static_cast<uint32_t>(kMortonBits - 3 * level) & 31u;
 }
 
-__device__ __forceinline__ float cell_of(float range, int level_diff) {
-  return __fdiv_rn(range, static_cast<float>(1u << (static_cast<uint32_t>(level_diff) & 31u)));
-}
-
 // One incoming link: chain c's top slot at its top level's octant (the
 // reference's set_child arguments, func_octree.hpp:96-105).
 __device__ __forceinline__ void link_into(int cnt, int off, uint32_t code, int pfx, int32_t (&row)[8],
@@ -203,94 +199,104 @@ struct Window {
   int w0, w1;
 };
 
-__device__ __forceinline__ uint2 fetch_desc(const OctArgs& a, uint32_t base, const Window& w, int q) {
-  if (q >= w.w0 && q < w.w1) return w.desc[q - w.w0];
-  const uint32_t gq = base + static_cast<uint32_t>(q);
+// ---- rare paths ----------------------------------------------------------------
+
+// Descriptor of a node outside the window.
+__device__ __forceinline__ uint2 desc_global(const OctArgs& a, uint32_t gq) {
   return make_desc(a.edge_count[gq], a.edge_offset[gq], a.codes[gq], a.prefix_n[gq],
                    a.left_child[gq], a.has_leaf_left[gq] != 0, a.has_leaf_right[gq] != 0);
 }
 
-__device__ __forceinline__ void link_desc(uint2 d, int32_t (&row)[8], int32_t& mask) {
-  const uint32_t which = d.y & 7u;
-  const int32_t top = static_cast<int32_t>(d.x);
+// Merges a register row into this thread's LDS row; returns the mask.
+__device__ __forceinline__ int32_t merge_row(const int32_t (&r)[8], int32_t m, int32_t* row) {
 #pragma unroll
-  for (uint32_t w = 0; w < 8; ++w) row[w] = (w == which) ? max(row[w], top) : row[w];
-  mask |= static_cast<int32_t>(1u << which);
+  for (int w = 0; w < 8; ++w)
+    if ((m >> w) & 1) atomicMax(&row[w], r[w]);
+  return m;
 }
 
-// The pull of a bottom slot owned by radix node p >= 1 (children lc, lc + 1
-// unless leaves), breadth-first so that each level costs one round trip.
-// Below p (count > 0) at most two nodes of count 0 follow each other on a
-// path (prefix lengths increase strictly and a count-0 node has its parent's
-// level), so the internal nodes three levels down all have a non-zero count;
-// any that does not is still handled, by the depth-first walk.
-__device__ __forceinline__ void pull_below(const OctArgs& a, uint32_t base, const Window& w, int n,
-                                           int lc, bool leaf_l, bool leaf_r, int32_t (&row)[8],
-                                           int32_t& mask) {
-  int q1[2] = {lc, lc + 1};
-  bool v1[2] = {!leaf_l && lc < n, !leaf_r && lc + 1 < n};
-  uint2 d1[2];
+// Depth-first pull from global memory below node q (count 0), for a walk
+// deeper than the pull stack (not reached below a node of count > 0).
+__device__ __forceinline__ int32_t pull_overflow(const OctArgs& a, uint32_t base, int q, int32_t* row) {
+  int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t m = 0;
+  int stack[16];
+  stack[0] = q;
+  pull_links(a, base, stack, 1, r, m);
+  return merge_row(r, m, row);
+}
+
+// Node 0's links (slot 0 is also offset[0]): every chain reached from the
+// children of the complete-range root through nodes of count 0.
+__device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, int n, int32_t* row) {
+  int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t m = 0;
+  const int g = full_root_split(a.codes + base, n);
+  int stack[16];
+  int sp = 0;
+  if (g != 0) stack[sp++] = g;
+  if (g + 1 < n) stack[sp++] = g + 1;
+  pull_links(a, base, stack, sp, r, m);
+  return merge_row(r, m, row);
+}
+
+// ---- the pull ------------------------------------------------------------------
+constexpr int kPullStack = 4;
+
+// Pull of a chain bottom whose radix node has the children pair `pair` (left
+// child | has_leaf_left << 30 | has_leaf_right << 31): depth-first through
+// nodes of count 0, a pair of siblings per step.  Every link is an LDS max
+// into this thread's row; returns the mask of octants linked.  Below a node
+// of count > 0 at most two nodes of count 0 follow each other on a path
+// (prefix lengths increase strictly and a count-0 node has its parent's
+// level), so at most three pairs wait on the stack; a deeper walk is still
+// handled, from global memory.
+__device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, const Window& w,
+                                             int n, uint32_t pair, int32_t* row, uint32_t* stk) {
+  int32_t m = 0;
+  int sp = 0;
+  for (;;) {
+    const int c = static_cast<int>(pair & 0x3fffffffu);
 #pragma unroll
-  for (int k = 0; k < 2; ++k) d1[k] = v1[k] ? fetch_desc(a, base, w, q1[k]) : make_uint2(0u, 3u);
-  int q2[4];
-  bool v2[4];
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const bool link = v1[k] && (d1[k].y & kLinkBit);
-    if (link) link_desc(d1[k], row, mask);
-    const bool down = v1[k] && !link;
-    const int c = static_cast<int>(d1[k].x);
-    q2[2 * k] = c;
-    q2[2 * k + 1] = c + 1;
-    v2[2 * k] = down && !(d1[k].y & 1u) && c < n;
-    v2[2 * k + 1] = down && !(d1[k].y & 2u) && c + 1 < n;
-  }
-  uint2 d2[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) d2[k] = v2[k] ? fetch_desc(a, base, w, q2[k]) : make_uint2(0u, 3u);
-  int q3[8];
-  bool v3[8];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const bool link = v2[k] && (d2[k].y & kLinkBit);
-    if (link) link_desc(d2[k], row, mask);
-    const bool down = v2[k] && !link;
-    const int c = static_cast<int>(d2[k].x);
-    q3[2 * k] = c;
-    q3[2 * k + 1] = c + 1;
-    v3[2 * k] = down && !(d2[k].y & 1u) && c < n;
-    v3[2 * k + 1] = down && !(d2[k].y & 2u) && c + 1 < n;
-  }
-  uint2 d3[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) d3[k] = v3[k] ? fetch_desc(a, base, w, q3[k]) : make_uint2(0u, 3u);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (!v3[k]) continue;
-    if (d3[k].y & kLinkBit) {
-      link_desc(d3[k], row, mask);
-    } else {
-      int stack[16];
-      stack[0] = q3[k];
-      pull_links(a, base, stack, 1, row, mask);
+    for (int side = 0; side < 2; ++side) {
+      const int q = c + side;
+      if (((pair >> (30 + side)) & 1u) || q >= n) continue;  // a leaf (or no such node)
+      uint2 d;
+      if (q >= w.w0 && q < w.w1) d = w.desc[q - w.w0];
+      else d = desc_global(a, base + static_cast<uint32_t>(q));
+      if (d.y & kLinkBit) {
+        atomicMax(&row[d.y & 7u], static_cast<int32_t>(d.x));
+        m |= static_cast<int32_t>(1u << (d.y & 7u));
+      } else if (sp < kPullStack) {
+        stk[sp++] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
+      } else {
+        m |= pull_overflow(a, base, q, row);
+      }
     }
+    if (sp == 0) break;
+    pair = stk[--sp];
   }
+  return m;
 }
 
-// Corners, cell sizes and rows.  After the window, the workgroup walks the
-// contiguous slot range its nodes own ([offset of its first node, offset +
-// count of its last node)), one slot per thread, so every store is
-// coalesced: the owner of slot s is the last node of the tile whose offset is
-// <= s (offsets are an exclusive scan of the counts), found by a binary search
-// in LDS.  Slot offset + j is the node j levels above the deepest one; j = 0
-// is the chain's bottom, whose row is pulled.  Node 0 writes the root cell
-// when no node contributes (D8).
+// Corners, cell sizes and rows.  A workgroup owns kOctSpan consecutive radix
+// nodes (a "tile"; thread t owns the four starting at 4t).  It builds the
+// window's descriptors in LDS, then walks the contiguous slot range its nodes
+// own ([offset of its first node, offset + count of its last node)), one slot
+// per thread, so every store is coalesced: the owner of slot s is the last
+// node of the tile whose offset is <= s (offsets are an exclusive scan of the
+// counts), found by a binary search in LDS.  Slot offset + j is the node j
+// levels above the chain's deepest one; j = 0 is the bottom, whose row is
+// pulled.  Node 0 writes the root cell when no node contributes (D8).
 __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
   __shared__ uint2 s_desc[kOctWin];
   __shared__ __align__(16) int32_t s_off[kOctSpan];
   __shared__ __align__(16) uint32_t s_code[kOctSpan];
   __shared__ __align__(16) uint32_t s_lcf[kOctSpan];  // left_child | leaf_l << 30 | leaf_r << 31
   __shared__ uint32_t s_lvl[kOctSpan / 4];  // prefix_n / 3, one byte per node
+  __shared__ __align__(16) int32_t s_row[kOctBlock * 8];  // each thread's pulled row
+  __shared__ uint32_t s_stk[kOctBlock * kPullStack];      // each thread's pull stack
+  __shared__ float s_cell[32];  // range / 2^k
   __shared__ int64_t s_end;
 
   // neighbouring tiles share window margins: XCD-grouped order
@@ -319,6 +325,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   w.w1 = min(b + kOctSpan + kOctHalo, n);
 
   // 1. the tile's nodes (four per thread) and the window's descriptors
+  if (tid < 32) s_cell[tid] = __fdiv_rn(a.range, static_cast<float>(1u << tid));
   const int q0 = b + 4 * static_cast<int>(tid);
   const Nodes4 g = load_nodes4(a, sg.base, q0, n);
   constexpr int kHaloGroups = kOctHalo / 4;
@@ -360,6 +367,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   __syncthreads();
 
   // 2. the slots, one per thread
+  int32_t* row = s_row + tid * 8;
+  uint32_t* stk = s_stk + tid * kPullStack;
   const int64_t lo = max<int64_t>(s_off[0], 0);
   const int64_t hi = min<int64_t>(s_end, rows);
   for (int64_t slot = lo + tid; slot < hi; slot += kOctBlock) {
@@ -374,29 +383,25 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     const int level = lvl - j;
     const uint32_t sh = level_shift(level);
     corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
-    cell[slot] = cell_of(a.range, level - root_level);
-    int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int32_t m = 0;
+    cell[slot] = s_cell[static_cast<uint32_t>(level - root_level) & 31u];
+    int4 r0, r1;
+    int32_t m;
     if (j > 0) {  // the child below, in the same chain
       const uint32_t which = (ci >> level_shift(lvl - (j - 1))) & 7u;
-#pragma unroll
-      for (uint32_t c = 0; c < 8; ++c) r[c] = (c == which) ? static_cast<int32_t>(slot - 1) : 0;
+      const int32_t c = static_cast<int32_t>(slot - 1);
+      r0 = make_int4(which == 0 ? c : 0, which == 1 ? c : 0, which == 2 ? c : 0, which == 3 ? c : 0);
+      r1 = make_int4(which == 4 ? c : 0, which == 5 ? c : 0, which == 6 ? c : 0, which == 7 ? c : 0);
       m = static_cast<int32_t>(1u << which);
     } else {  // the bottom: pull the chains that link here
-      const uint32_t lcf = s_lcf[k];
-      pull_below(a, sg.base, w, n, static_cast<int>(lcf & 0x3fffffffu), (lcf >> 30) & 1u,
-                 (lcf >> 31) & 1u, r, m);
-      if (slot == 0) {  // slot 0 is also offset[0]: node 0's links land here
-        const int gsplit = full_root_split(codes, n);
-        int stack[16];
-        int sp = 0;
-        if (gsplit != 0) stack[sp++] = gsplit;
-        if (gsplit + 1 < n) stack[sp++] = gsplit + 1;
-        pull_links(a, sg.base, stack, sp, r, m);
-      }
+      reinterpret_cast<int4*>(row)[0] = make_int4(0, 0, 0, 0);
+      reinterpret_cast<int4*>(row)[1] = make_int4(0, 0, 0, 0);
+      m = pull_pair(a, sg.base, w, n, s_lcf[k], row, stk);
+      if (slot == 0) m |= pull_root(a, sg.base, n, row);
+      r0 = reinterpret_cast<const int4*>(row)[0];
+      r1 = reinterpret_cast<const int4*>(row)[1];
     }
-    rows8[slot * 2] = make_int4(r[0], r[1], r[2], r[3]);
-    rows8[slot * 2 + 1] = make_int4(r[4], r[5], r[6], r[7]);
+    rows8[slot * 2] = r0;
+    rows8[slot * 2 + 1] = r1;
     mask[slot] = m;
     leaf_mask[slot] = 0;
   }
@@ -405,9 +410,9 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
   const uint32_t n_tiles = (n_index + kOctSpan - 1) / kOctSpan;
   if (n_tiles == 0) return;
-  const uint32_t grid = xcd_grid(n_tiles);
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_oct_nodes, dim3(grid), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
+  hipLaunchKernelGGL(k_oct_nodes, dim3(xcd_grid(n_tiles)), dim3(kOctBlock), 0, stream, a, seg,
+                     n_tiles);
 }
 
 // Leaf linking (opt-in): each leaf child of brt node i goes into the bottom

@@ -211,12 +211,18 @@ __device__ __forceinline__ int log2_ceil_u32(uint32_t x) {
 // Codes of a segment, read through an LDS copy of the window around the
 // workgroup's nodes (the searches of most nodes stay inside it) and from
 // global memory outside it.
+// The LDS read is issued unconditionally (at a clamped index; the empty asm
+// statement pins its result) and the global one only outside the window, so
+// the common case stays a ds_read instead of both becoming one flat load.
 struct CodeWindow {
   const uint32_t* g;  // segment's codes
   const uint32_t* s;  // LDS copy of g[lo, hi)
   int lo, hi;
   __device__ __forceinline__ uint32_t operator[](int k) const {
-    return (k >= lo && k < hi) ? s[k - lo] : g[k];
+    uint32_t v = s[min(max(k - lo, 0), hi - lo - 1)];
+    asm volatile("" : "+v"(v));
+    if (k < lo || k >= hi) v = g[k];
+    return v;
   }
 };
 

@@ -242,11 +242,14 @@ __device__ __forceinline__ uint32_t morton_encode(float x, float y, float z, flo
 }
 
 // Corner of a cell: (float(d) / 1024) * range + min per axis, w = 1.
+// The reference divides by 1024.0f; the grid coordinate is an integer in
+// [0, 1023], so the quotient is exact and equals the product with 2^-10.
 __device__ __forceinline__ float4 morton_decode(uint32_t code, float min_coord, float range) {
+  constexpr float kInv1024 = 0x1p-10f;
   float4 r;
-  r.x = __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(compact3(code)), 1024.0f), range), min_coord);
-  r.y = __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(compact3(code >> 1)), 1024.0f), range), min_coord);
-  r.z = __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(compact3(code >> 2)), 1024.0f), range), min_coord);
+  r.x = __fadd_rn(__fmul_rn(static_cast<float>(compact3(code)) * kInv1024, range), min_coord);
+  r.y = __fadd_rn(__fmul_rn(static_cast<float>(compact3(code >> 1)) * kInv1024, range), min_coord);
+  r.z = __fadd_rn(__fmul_rn(static_cast<float>(compact3(code >> 2)) * kInv1024, range), min_coord);
   r.w = 1.0f;
   return r;
 }
