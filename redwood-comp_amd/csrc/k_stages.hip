@@ -187,17 +187,17 @@ void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, ui
 
 // ---------------------------------------------------------------------------
 // S4  Karras binary radix tree (reference: apps/tree/host/func_brt.hpp:45-138,
-// apps/tree/cuda/04_radix_tree.cu:12-144), one thread per node i in [0, n_brt).
+// apps/tree/cuda/04_radix_tree.cu:12-144) for nodes [0, n_brt).
 // Quirks kept: delta = uint8(clz(a^b) - 1) (Q1), the root's range is
 // [0, n_brt - 1] (Q2), clz(0) = 32 and log2_ceil(0) = 0 (D3).
 // Parent slots: every internal node other than the root of the complete tree
 // over [0, n_brt] has exactly one claimer among nodes i >= 1 (the Karras
 // construction); the Q2 root's own claims lose to any such writer under the
 // max-writer rule (D7) and otherwise write 0.  So nodes i >= 1 store plainly,
-// node 0 stores nothing for its Q2 claims, and node 0's thread writes 0 to
-// parent[0] and to the internal children of the complete-range root (the only
-// slots no node i >= 1 claims).  The result equals max-of-writers over zeroed
-// slots without atomics or a clearing pass.
+// node 0 stores nothing for its Q2 claims, and 0 goes to parent[0] and to the
+// internal children of the complete-range root (the only slots no node i >= 1
+// claims).  The result equals max-of-writers over zeroed slots without
+// atomics or a clearing pass.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int delta8(uint32_t a, uint32_t b) {
   return static_cast<uint8_t>(__clz(a ^ b) - 1);  // __clz(0) == 32
@@ -208,32 +208,12 @@ __device__ __forceinline__ int log2_ceil_u32(uint32_t x) {
   return fl + (((1u << fl) < x) ? 1 : 0);
 }
 
-// Codes of a segment, read through an LDS copy of the window around the
-// workgroup's nodes (the searches of most nodes stay inside it) and from
-// global memory outside it.
-// The LDS read is issued unconditionally (at a clamped index; the empty asm
-// statement pins its result) and the global one only outside the window, so
-// the common case stays a ds_read instead of both becoming one flat load.
-struct CodeWindow {
-  const uint32_t* g;  // segment's codes
-  const uint32_t* s;  // LDS copy of g[lo, hi)
-  int lo, hi;
-  __device__ __forceinline__ uint32_t operator[](int k) const {
-    uint32_t v = s[min(max(k - lo, 0), hi - lo - 1)];
-    asm volatile("" : "+v"(v));
-    if (k < lo || k >= hi) v = g[k];
-    return v;
-  }
-};
-
-constexpr int kTreeBlock = 1024;  // radix nodes per workgroup, one per thread
-constexpr int kTreeHalo = 768;    // codes cached on each side of the workgroup's nodes
-static_assert(kChunk % kTreeBlock == 0, "tiles must not straddle segments");
-
-// Split search of the reference (func_brt.hpp:90-104) for node i with range
-// length l, direction d; returns gamma.
-__device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, int d, int l,
-                                         int dnode, uint32_t ci) {
+// ---- reference search (any 32-bit keys) ------------------------------------
+// Range and split of node i exactly as func_brt.hpp:57-104 computes them,
+// reading codes from global memory; used when a key has bit 31 set (uint8
+// wrap of delta, so the searches are not plain boundary searches).
+__device__ __forceinline__ int ref_split(const uint32_t* codes, int i, int n, int d, int l, int dnode,
+                                         uint32_t ci) {
   int s = 0;
   const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
   const int scut = (d == -1) ? i - 1 : n - i - 1;
@@ -246,52 +226,132 @@ __device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, 
   return i + s * d + min(d, 0);
 }
 
-// Largest k in [0, K] with delta8(ci, codes[i + k*d]) > ref, for a predicate
-// that holds at k = 0 and is monotone in k (true, then false).  Sorted unique
-// keys below 2^31 make delta8(ci, codes[i + k*d]) non-increasing in k (no
-// uint8 wrap), and then the reference's doubling + binary search and its
-// split search both return exactly this boundary.  The whole wave searches
-// for one lane: 64 probes per round trip, so a range of 2^24 takes 4-5 round
-// trips instead of 24 dependent ones.  Wave-uniform arguments; every lane
-// must call it.
-__device__ int wave_boundary(const CodeWindow& codes, int i, int d, uint32_t ci, int ref, int K) {
-  const uint32_t lane = lane_id();
-  int lo = 0, hi = K;
-  while (lo < hi) {
-    const uint64_t span = static_cast<uint64_t>(hi - lo);
-    const int k = lo + static_cast<int>(((lane + 1) * span + 63) / 64);  // lane 63 probes hi
-    const bool p = delta8(ci, codes[i + k * d]) > ref;
-    const uint32_t c = static_cast<uint32_t>(__popcll(ballot(p)));  // probes that hold
-    if (c == 64) return hi;
-    const int kc = lo + static_cast<int>(((c + 1) * span + 63) / 64);  // first probe that fails
-    const int kp = (c == 0) ? lo : lo + static_cast<int>((c * span + 63) / 64);
-    lo = kp;
-    hi = kc - 1;
+__device__ void ref_node(const TreeArgs& a, const uint32_t* codes, uint32_t base, int i, int n) {
+  const uint32_t ci = codes[i];
+  int d = 1, l = 0;
+  if (i == 0) {
+    l = n - 1;  // Q2
+  } else {
+    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
+    d = (diff > 0) - (diff < 0);
+    const int dmin = delta8(ci, codes[i - d]);
+    int lmax = 2;
+    while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
+    const int cutoff = (d == -1) ? i : n - i;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
   }
-  return lo;
+  const int j = i + l * d;
+  const int dnode = delta8(ci, codes[j]);
+  const int gamma = ref_split(codes, i, n, d, l, dnode, ci);
+  const bool ll = min(i, j) == gamma;
+  const bool lr = max(i, j) == gamma + 1;
+  a.prefix_n[base + i] = static_cast<uint8_t>(dnode);
+  a.left_child[base + i] = gamma;
+  a.has_leaf_left[base + i] = ll ? 1 : 0;
+  a.has_leaf_right[base + i] = lr ? 1 : 0;
+  int32_t* parent = a.parent + base;
+  if (i > 0) {
+    if (!ll) parent[gamma] = i;
+    if (!lr) parent[gamma + 1] = i;
+  } else {
+    parent[0] = 0;
+    const int g = ref_split(codes, 0, n, 1, n, delta8(ci, codes[n]), ci);  // complete range
+    if (g != 0) parent[g] = 0;
+    if (g + 1 < n) parent[g + 1] = 0;
+  }
 }
 
-// Writes the tile's nv leading entries of an LDS array to global memory, 16
-// bytes per lane where the group is whole (dst 16-byte aligned).
+// ---- nearest smaller adjacent deltas (keys below 2^31) ---------------------
+// With a[k] = delta8(code[k], code[k+1]) and keys sorted, unique and below
+// 2^31, delta8(code[x], code[y]) = min(a[x..y-1]), and the reference's searches
+// reduce to nearest-smaller-value queries: the internal node split at k covers
+// keys [L, R] = [psv(k) + 1, nsv(k)] (psv/nsv: nearest position on the left /
+// right with a smaller a, -1 / n if none), its prefix is a[k], its children
+// are k and k + 1, and its index is L if a[psv] > a[nsv] (it is its parent's
+// right child) and R otherwise.  The split with no smaller value on either
+// side is the root of the complete range [0, n]; node 0 reports the range
+// [0, n-1] instead (Q2): the same split unless that is n - 1, else the split
+// whose range is exactly [0, n-1].  Pinned against the reference search on
+// the CPU by the tests.
+//
+// A workgroup owns kTreeBlock splits and keeps a[] for a window reaching
+// kTreeHalo further on each side in LDS, in blocks of 64 (one wave):
+//   * inside the split's own block: binary lifting over a per-wave sparse
+//     table of minima (6 steps);
+//   * across blocks: a bitmask per threshold t of the blocks whose minimum is
+//     below t picks the block, and a binary search over that block's prefix
+//     (or suffix) minima the position;
+//   * beyond the window: a wave-cooperative 64-ary search in global memory.
+constexpr int kTreeBlock = 1024;  // splits per workgroup, one per thread
+constexpr int kTreeHalo = 1024;   // window margin on each side (multiple of 64)
+constexpr int kTreeWin = kTreeBlock + 2 * kTreeHalo;
+constexpr int kTreeWinBlocks = kTreeWin / kWave;
+static_assert(kTreeWinBlocks <= 64, "one bit per window block");
+static_assert(kChunk % kTreeBlock == 0, "tiles must not straddle segments");
+
+// First t in [lo, hi] where pred holds, for pred false ... false true ... true
+// on [lo, hi]; hi + 1 if it never holds.  Wave-uniform arguments; every lane
+// calls it; 64 probes per round trip.
+template <class Pred>
+__device__ int wave_first_true(int lo, int hi, Pred pred) {
+  const uint32_t lane = lane_id();
+  for (;;) {
+    if (lo > hi) return lo;
+    const uint32_t span = static_cast<uint32_t>(hi - lo) + 1u;
+    if (span <= static_cast<uint32_t>(kWave)) {
+      const int t = lo + static_cast<int>(lane);
+      const uint64_t b = ballot(t <= hi && pred(t));
+      return b ? lo + static_cast<int>(__ffsll(static_cast<long long>(b)) - 1) : hi + 1;
+    }
+    // strictly increasing probes, the last one at hi
+    const int t = lo + static_cast<int>((static_cast<uint64_t>(lane + 1) * span) / kWave) - 1;
+    const uint64_t b = ballot(pred(t));
+    if (b == 0) return hi + 1;
+    const int c = __ffsll(static_cast<long long>(b)) - 1;
+    const int tc = __shfl(t, c, kWave);
+    const int tp = (c == 0) ? lo - 1 : __shfl(t, c - 1, kWave);
+    lo = tp + 1;
+    hi = tc;
+  }
+}
+
+// Writes the tile's entries [0, nv) that this workgroup produced (own[e] =
+// 1) to global memory: whole 16-byte groups with one store, the rest entry by
+// entry (dst 16-byte aligned).
 template <typename T>
-__device__ __forceinline__ void flush_tile(T* __restrict__ dst, const T* src, int nv, int first_group,
-                                           int lanes) {
+__device__ __forceinline__ void flush_owned(T* __restrict__ dst, const T* src, const uint8_t* own,
+                                            int nv, int first_group, int lanes) {
   constexpr int kPer = 16 / sizeof(T);
   for (int g = first_group; g * kPer < nv; g += lanes) {
-    if ((g + 1) * kPer <= nv) {
+    bool whole = (g + 1) * kPer <= nv;
+    if (whole) {
+      if constexpr (kPer == 16) {
+        const uint4 o = reinterpret_cast<const uint4*>(own)[g];
+        whole = (o.x & o.y & o.z & o.w) == 0x01010101u;
+      } else {
+        whole = reinterpret_cast<const uint32_t*>(own)[g] == 0x01010101u;
+      }
+    }
+    if (whole) {
       *reinterpret_cast<uint4*>(dst + g * kPer) = reinterpret_cast<const uint4*>(src)[g];
     } else {
-      for (int e = g * kPer; e < nv; ++e) dst[e] = src[e];
+      for (int e = g * kPer; e < min(nv, (g + 1) * kPer); ++e)
+        if (own[e]) dst[e] = src[e];
     }
   }
 }
 
 __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs seg, uint32_t n_tiles) {
-  __shared__ uint32_t s_codes[kTreeBlock + 2 * kTreeHalo];
+  __shared__ uint8_t s_a[kTreeWin];                   // a[w0 + p], 0xff past the window
+  __shared__ uint8_t s_pmin[kTreeWin], s_smin[kTreeWin];  // per-block prefix / suffix minima
+  __shared__ uint64_t s_below[32];                    // bit b: block b's minimum < t
   __shared__ __align__(16) int32_t s_lc[kTreeBlock];
   __shared__ __align__(16) uint8_t s_pfx[kTreeBlock];
   __shared__ __align__(16) uint8_t s_ll[kTreeBlock];
   __shared__ __align__(16) uint8_t s_lr[kTreeBlock];
+  __shared__ __align__(16) uint8_t s_own[kTreeBlock];
+
   // neighbouring tiles share window codes: XCD-grouped order
   const uint32_t tile = xcd_tile();
   if (tile >= n_tiles) return;
@@ -301,110 +361,203 @@ __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs s
   const int b0 = static_cast<int>(first - sg.base);
   if (first < sg.base || b0 >= n) return;  // whole workgroup past the segment's nodes
   const uint32_t* __restrict__ gcodes = a.codes + sg.base;
-  CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kTreeBlock + kTreeHalo, n + 1)};
-  for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kTreeBlock)
-    s_codes[k - codes.lo] = gcodes[k];
-  __syncthreads();
-  const int i = b0 + static_cast<int>(threadIdx.x);
-  const bool valid = i < n;
-  const uint32_t lane = threadIdx.x % kWave;
-  // keys below 2^31 (the largest is the last): searches are boundary searches
-  const bool mono = gcodes[n] < 0x80000000u;
-  const uint32_t ci = valid ? codes[i] : 0u;
+  const int tid = static_cast<int>(threadIdx.x);
 
-  // Range: direction d and length l (func_brt.hpp:57-88).  A lane is "big"
-  // when its doubling probe leaves the LDS window.
-  int d = 1, l = 0, dmin = 0, cutoff = 0, lmax = 2;
-  bool big = false;
-  if (valid && i == 0) {
-    l = n - 1;  // Q2
-  } else if (valid) {
-    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
-    d = (diff > 0) - (diff < 0);
-    dmin = delta8(ci, codes[i - d]);
-    cutoff = (d == -1) ? i : n - i;
-    for (;;) {
-      const int pos = i + lmax * d;
-      if (pos < 0 || pos > n) break;
-      if (pos < codes.lo || pos >= codes.hi) {
-        big = true;
-        break;
-      }
-      if (!(delta8(ci, s_codes[pos - codes.lo]) > dmin)) break;
-      lmax *= 2;
-    }
-    if (!big || !mono) {  // the reference's own search
-      while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
-      for (int t = lmax / 2; t >= 1; t /= 2)
-        if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
-    }
+  if (gcodes[n] >= 0x80000000u) {  // keys with bit 31 set: the reference's own search
+    if (b0 + tid < n) ref_node(a, gcodes, sg.base, b0 + tid, n);
+    return;
   }
-  for (uint64_t m = ballot(valid && big && mono); m; m &= m - 1) {
-    const int o = __ffsll(static_cast<long long>(m)) - 1;
-    const int ol = wave_boundary(codes, __shfl(i, o, kWave), __shfl(d, o, kWave),
-                                 __shfl(ci, o, kWave), __shfl(dmin, o, kWave),
-                                 __shfl(cutoff, o, kWave));
-    if (static_cast<int>(lane) == o) l = ol;
-  }
-
-  // Split (func_brt.hpp:90-104); big when the far end is outside the window.
-  const int j = i + l * d;
-  const int dnode = valid ? delta8(ci, codes[j]) : 0;
-  const bool sbig = valid && (j < codes.lo || j >= codes.hi);
-  int gamma = 0;
-  if (valid && (!sbig || !mono)) gamma = brt_split(codes, i, n, d, l, dnode, ci);
-  for (uint64_t m = ballot(sbig && mono); m; m &= m - 1) {
-    const int o = __ffsll(static_cast<long long>(m)) - 1;
-    const int oi = __shfl(i, o, kWave), od = __shfl(d, o, kWave);
-    const int os = wave_boundary(codes, oi, od, __shfl(ci, o, kWave), __shfl(dnode, o, kWave),
-                                 max(__shfl(l, o, kWave) - 1, 0));
-    if (static_cast<int>(lane) == o) gamma = oi + os * od + min(od, 0);
-  }
-
-  // Node 0 also parents the internal children of the root over the complete
-  // range [0, n] (split with l = n); its wave searches together.
-  int groot = 0;
-  if (b0 == 0 && threadIdx.x < kWave) {
-    const uint32_t c0 = __shfl(ci, 0, kWave);
-    const int dfull = delta8(c0, codes[n]);
-    if (mono) groot = wave_boundary(codes, 0, 1, c0, dfull, max(n - 1, 0));
-    else if (lane == 0) groot = brt_split(codes, 0, n, 1, n, dfull, c0);
-  }
-
-  if (valid) {
-    int32_t* __restrict__ parent = a.parent + sg.base;
-    const bool ll = min(i, j) == gamma;
-    const bool lr = max(i, j) == gamma + 1;
-    s_pfx[threadIdx.x] = static_cast<uint8_t>(dnode);
-    s_lc[threadIdx.x] = gamma;
-    s_ll[threadIdx.x] = ll ? 1 : 0;
-    s_lr[threadIdx.x] = lr ? 1 : 0;
-    if (i > 0) {
-      if (!ll) parent[gamma] = i;
-      if (!lr) parent[gamma + 1] = i;
-    } else {
+  int32_t* __restrict__ parent = a.parent + sg.base;
+  if (n == 1) {  // two keys: node 0 alone, over [0, 0] (Q2)
+    if (tid == 0) {
+      a.prefix_n[sg.base] = static_cast<uint8_t>(delta8(gcodes[0], gcodes[0]));
+      a.left_child[sg.base] = 0;
+      a.has_leaf_left[sg.base] = 1;
+      a.has_leaf_right[sg.base] = 0;
       parent[0] = 0;
-      if (groot != 0) parent[groot] = 0;
-      if (groot + 1 < n) parent[groot + 1] = 0;
     }
+    return;
+  }
+
+  const int lane = tid % kWave, wv = tid / kWave;
+  const int w0 = max(b0 - kTreeHalo, 0);
+  const int w1 = min(b0 + kTreeBlock + kTreeHalo, n);  // a[] window [w0, w1)
+  const int nblk = (w1 - w0 + kWave - 1) / kWave;
+
+  // 1. a[] over the window
+  for (int p = tid; p < kTreeWin; p += kTreeBlock) {
+    const int k = w0 + p;
+    s_a[p] = (k < w1) ? static_cast<uint8_t>(delta8(gcodes[k], gcodes[k + 1])) : 0xffu;
+  }
+  s_own[tid] = 0;
+  __syncthreads();
+
+  // 2. per-block prefix / suffix minima
+  for (int blk = wv; blk < nblk; blk += kTreeBlock / kWave) {
+    const int p = blk * kWave + lane;
+    uint32_t pm = s_a[p], sm = pm;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t u = __shfl_up(pm, o, kWave);
+      const uint32_t v = __shfl_down(sm, o, kWave);
+      if (lane >= o) pm = min(pm, u);
+      if (lane + o < kWave) sm = min(sm, v);
+    }
+    s_pmin[p] = static_cast<uint8_t>(pm);
+    s_smin[p] = static_cast<uint8_t>(sm);
   }
   __syncthreads();
-  // the tile's per-node outputs, 16 bytes per lane
+
+  // 3. blocks below each threshold (one wave)
+  if (wv == 0) {
+    const uint32_t bm = (lane < nblk) ? s_pmin[lane * kWave + kWave - 1] : 0xffu;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const uint64_t m = ballot(bm < static_cast<uint32_t>(t));
+      if (lane == t) mine = m;
+    }
+    if (lane < 32) s_below[lane] = mine;
+  }
+  __syncthreads();
+
+  // 4. nearest smaller values of this thread's split k
+  const int k = b0 + tid;
+  const bool valid = k < n;
+  const uint32_t v = s_a[k - w0];
+  const int wbase = b0 + wv * kWave;          // first split of this wave
+  const int wblk = (wbase - w0) / kWave;      // its window block
+  // sparse table of minima over the wave: lvl[j] at lane i = min(a[i .. i + 2^j - 1])
+  uint32_t lvl[6];
+  lvl[0] = v;
+#pragma unroll
+  for (int j = 1; j < 6; ++j) {
+    const uint32_t u = __shfl_down(lvl[j - 1], 1 << (j - 1), kWave);
+    lvl[j] = (lane + (1 << (j - 1)) < kWave) ? min(lvl[j - 1], u) : lvl[j - 1];
+  }
+  // right: skip maximal runs >= v (binary lifting); 64 = none in the wave
+  int nx = lane + 1;
+#pragma unroll
+  for (int j = 5; j >= 0; --j) {
+    const uint32_t m = __shfl(lvl[j], min(nx, kWave - 1), kWave);
+    if (nx + (1 << j) <= kWave && m >= v) nx += 1 << j;
+  }
+  // left: runs ending at px; -1 = none in the wave
+  int px = lane - 1;
+#pragma unroll
+  for (int j = 5; j >= 0; --j) {
+    const int st = px - (1 << j) + 1;
+    const uint32_t m = __shfl(lvl[j], max(st, 0), kWave);
+    if (st >= 0 && m >= v) px -= 1 << j;
+  }
+  int nsv = n, psv = -1;
+  bool far_n = false, far_p = false;
+  if (valid) {
+    if (nx < kWave) {
+      nsv = wbase + nx;
+    } else {
+      const uint64_t cand = s_below[v] & (wblk + 1 < 64 ? (~0ull << (wblk + 1)) : 0ull);
+      if (cand) {
+        const int b2 = __ffsll(static_cast<long long>(cand)) - 1;
+        const uint8_t* pm = s_pmin + b2 * kWave;
+        int q = 0;  // first position of block b2 below v
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1)
+          if (pm[q + st - 1] >= v) q += st;
+        nsv = w0 + b2 * kWave + q;
+      } else if (w1 < n) {
+        far_n = true;
+      }
+    }
+    if (px >= 0) {
+      psv = wbase + px;
+    } else {
+      const uint64_t cand = s_below[v] & ((1ull << wblk) - 1);
+      if (cand) {
+        const int b2 = 63 - __clzll(static_cast<long long>(cand));
+        const uint8_t* sm = s_smin + b2 * kWave;
+        int q = kWave - 1;  // last position of block b2 below v
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1)
+          if (sm[q - st + 1] >= v) q -= st;
+        psv = w0 + b2 * kWave + q;
+      } else if (w0 > 0) {
+        far_p = true;
+      }
+    }
+  }
+  // beyond the window: min(a[w1 .. t]) < v, i.e. delta8(code[w1], code[t + 1]) < v
+  for (uint64_t m = ballot(far_n); m; m &= m - 1) {
+    const int o = __ffsll(static_cast<long long>(m)) - 1;
+    const int ov = __shfl(static_cast<int>(v), o, kWave);
+    const uint32_t c1 = gcodes[w1];
+    const int r = wave_first_true(w1, n - 1, [&](int t) { return delta8(c1, gcodes[t + 1]) < ov; });
+    if (lane == o) nsv = r;
+  }
+  // min(a[t .. w0 - 1]) >= v, i.e. delta8(code[t], code[w0]) >= v; psv is one before
+  for (uint64_t m = ballot(far_p); m; m &= m - 1) {
+    const int o = __ffsll(static_cast<long long>(m)) - 1;
+    const int ov = __shfl(static_cast<int>(v), o, kWave);
+    const uint32_t c0 = gcodes[w0];
+    const int r = wave_first_true(0, w0 - 1, [&](int t) { return delta8(gcodes[t], c0) >= ov; });
+    if (lane == o) psv = r - 1;
+  }
+
+  // 5. the node split at k
+  const auto a_at = [&](int x) -> int {
+    int r = s_a[min(max(x - w0, 0), kTreeWin - 1)];
+    asm volatile("" : "+v"(r));  // keep the LDS read; global only outside the window
+    if (x < w0 || x >= w1) r = delta8(gcodes[x], gcodes[x + 1]);
+    return r;
+  };
+  const auto put = [&](int idx, int pfx, int lc, bool ll, bool lr) {
+    const int e = idx - b0;
+    if (e >= 0 && e < kTreeBlock) {
+      s_pfx[e] = static_cast<uint8_t>(pfx);
+      s_lc[e] = lc;
+      s_ll[e] = ll ? 1 : 0;
+      s_lr[e] = lr ? 1 : 0;
+      s_own[e] = 1;
+    } else {
+      a.prefix_n[sg.base + idx] = static_cast<uint8_t>(pfx);
+      a.left_child[sg.base + idx] = lc;
+      a.has_leaf_left[sg.base + idx] = ll ? 1 : 0;
+      a.has_leaf_right[sg.base + idx] = lr ? 1 : 0;
+    }
+  };
+  if (valid) {
+    const int L = psv + 1, R = nsv;
+    int idx = 0;  // the complete-range root has index 0 (its outputs are Q2's)
+    if (psv >= 0 || nsv < n) {
+      const int ap = (psv >= 0) ? a_at(psv) : -1;
+      const int an = (nsv < n) ? a_at(nsv) : -1;
+      idx = (ap > an) ? L : R;
+      put(idx, static_cast<int>(v), k, L == k, R == k + 1);
+    }
+    // node 0 (Q2): the split of [0, n-1]
+    if (psv == -1 && (nsv == n - 1 || (nsv == n && k < n - 1)))
+      put(0, static_cast<int>(v), k, k == 0, k + 1 == n - 1);
+    if (L != k) parent[k] = idx;
+    if (R != k + 1) parent[k + 1] = idx;
+    if (k == 0) parent[0] = 0;
+  }
+  __syncthreads();
+
+  // 6. the tile's outputs, 16 bytes per lane where whole
   const int nv = min(kTreeBlock, n - b0);
   const uint32_t at = sg.base + static_cast<uint32_t>(b0);
-  const int t = static_cast<int>(threadIdx.x);
-  if (t < 64) flush_tile(a.prefix_n + at, s_pfx, nv, t, 64);
-  else if (t < 128) flush_tile(a.has_leaf_left + at, s_ll, nv, t - 64, 64);
-  else if (t < 192) flush_tile(a.has_leaf_right + at, s_lr, nv, t - 128, 64);
-  else if (t >= 256 && t < 512) flush_tile(a.left_child + at, s_lc, nv, t - 256, 256);
+  if (tid < 64) flush_owned(a.prefix_n + at, s_pfx, s_own, nv, tid, 64);
+  else if (tid < 128) flush_owned(a.has_leaf_left + at, s_ll, s_own, nv, tid - 64, 64);
+  else if (tid < 192) flush_owned(a.has_leaf_right + at, s_lr, s_own, nv, tid - 128, 64);
+  else if (tid >= 256 && tid < 512) flush_owned(a.left_child + at, s_lc, s_own, nv, tid - 256, 256);
 }
 
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
   const uint32_t n_tiles = (n_index + kTreeBlock - 1) / kTreeBlock;
   if (n_tiles == 0) return;
-  const uint32_t grid = xcd_grid(n_tiles);
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_radix_tree, dim3(grid), dim3(kTreeBlock), 0, stream, a, seg, n_tiles);
+  hipLaunchKernelGGL(k_radix_tree, dim3(xcd_grid(n_tiles)), dim3(kTreeBlock), 0, stream, a, seg,
+                     n_tiles);
 }
 
 // ---------------------------------------------------------------------------

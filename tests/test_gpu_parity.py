@@ -118,6 +118,47 @@ def test_unique_u32(gpu, n):
         assert np.array_equal(d_out.numpy(want.size), want)
 
 
+def radix_keys(rng, kind, n):
+    if kind == "k30":  # Morton-range keys
+        k = rng.integers(0, 1 << 30, size=n, dtype=np.uint64)
+    elif kind == "k32":  # bit 31 set on some keys: the reference search path
+        k = rng.integers(0, 1 << 32, size=n, dtype=np.uint64)
+    elif kind == "dense":  # consecutive runs: long chains of equal deltas
+        k = np.arange(n, dtype=np.uint64) * 3 + (rng.integers(0, 3, size=n) == 0)
+    elif kind == "lastfar":
+        k = rng.integers(0, 1 << 20, size=n, dtype=np.uint64)
+        k[-1] = (1 << 30) - 1
+    else:  # firstfar
+        k = rng.integers(0, 1 << 20, size=n, dtype=np.uint64) | (1 << 29)
+        k[0] = 0
+    return np.unique(k).astype(np.uint32)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 64, 1000, 5000, 70000, 600000])
+@pytest.mark.parametrize("kind", ["k30", "k32", "dense", "lastfar", "firstfar"])
+def test_radix_tree_stage(gpu, orc, pool, kind, n):
+    """S4 alone on sorted unique keys (rwh_radix_tree), against the oracle."""
+    rw = gpu
+    L = rw.lib()
+    rng = np.random.default_rng(n * 7 + len(kind))
+    k = radix_keys(rng, kind, n)
+    if k.size < 2:
+        pytest.skip("fewer than two unique keys")
+    nu = k.size
+    d_k = rw.DeviceArray.from_numpy(k)
+    d_nu = rw.DeviceArray.from_numpy(np.array([nu], np.uint32))
+    out = {"prefix_n": rw.DeviceArray(nu, np.uint8), "has_leaf_left": rw.DeviceArray(nu, np.uint8),
+           "has_leaf_right": rw.DeviceArray(nu, np.uint8), "left_child": rw.DeviceArray(nu, np.int32),
+           "parents": rw.DeviceArray(nu, np.int32, zero=True)}
+    rw.check(L.rwh_radix_tree(d_k.ptr, d_nu.ptr, nu, out["prefix_n"].ptr, out["has_leaf_left"].ptr,
+                              out["has_leaf_right"].ptr, out["left_child"].ptr, out["parents"].ptr,
+                              None))
+    rw.check(L.rwh_stream_sync(None))
+    want = orc.radix_tree(pool, k)
+    for name, arr in out.items():
+        assert np.array_equal(arr.numpy(nu - 1), want[name]), name
+
+
 def test_repeated_builds_are_identical(gpu):
     rw = gpu
     pts = rw.gmm_points(300000, seed=9, n_centers=16, sigma=10.24)

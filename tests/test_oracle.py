@@ -189,6 +189,73 @@ def test_radix_tree_brute_force_tiny(orc, pool):
             assert np.array_equal(got[k], want[k]), (k, c.tolist())
 
 
+def tree_from_adjacent_deltas(c):
+    """Third construction of the radix tree, the one the GPU uses for keys
+    below 2^31: with a[k] = delta8(c[k], c[k+1]), the node split at k covers
+    keys [psv(k) + 1, nsv(k)] (nearest positions with a smaller a on the left /
+    right), has prefix a[k], and is numbered by its endpoint next to the larger
+    of a[psv], a[nsv]; node 0 reports the range [0, n-1] (Q2)."""
+    c = [int(x) for x in c]
+    n = len(c) - 1
+    P = np.zeros(n, np.uint8)
+    HL = np.zeros(n, np.uint8)
+    HR = np.zeros(n, np.uint8)
+    LC = np.zeros(n, np.int32)
+    par = np.zeros(n, np.int32)
+    if n == 1:
+        P[0], HL[0] = delta8(c[0], c[0]), 1
+        return {"prefix_n": P, "has_leaf_left": HL, "has_leaf_right": HR, "left_child": LC,
+                "parents": par}
+    a = [delta8(c[k], c[k + 1]) for k in range(n)]
+
+    def put(i, v, k, ll, lr):
+        P[i], LC[i], HL[i], HR[i] = v, k, int(ll), int(lr)
+
+    for k in range(n):
+        v = a[k]
+        psv = next((q for q in range(k - 1, -1, -1) if a[q] < v), -1)
+        nsv = next((q for q in range(k + 1, n) if a[q] < v), n)
+        lo, hi = psv + 1, nsv
+        idx = 0
+        if psv >= 0 or nsv < n:
+            ap = a[psv] if psv >= 0 else -1
+            an = a[nsv] if nsv < n else -1
+            idx = lo if ap > an else hi
+            put(idx, v, k, lo == k, hi == k + 1)
+        if psv == -1 and (nsv == n - 1 or (nsv == n and k < n - 1)):
+            put(0, v, k, k == 0, k + 1 == n - 1)
+        if lo != k:
+            par[k] = idx
+        if hi != k + 1:
+            par[k + 1] = idx
+    return {"prefix_n": P, "has_leaf_left": HL, "has_leaf_right": HR, "left_child": LC,
+            "parents": par}
+
+
+def test_radix_tree_adjacent_delta_formulation(orc, pool):
+    rng = np.random.default_rng(14)
+    cases = [np.array([0, 1], np.uint32), np.array([0, 1, 2], np.uint32),
+             np.array([1, 2, 3, 1 << 29], np.uint32), np.array([0, 1 << 29, (1 << 29) + 1], np.uint32)]
+    for s in range(1500):
+        n = int(rng.integers(2, 90))
+        bits = int(rng.choice([4, 8, 12, 30]))
+        if (1 << bits) < n:
+            bits = 12
+        c = np.sort(rng.choice(1 << bits, size=n, replace=False)).astype(np.uint32)
+        if s % 5 == 0:
+            c[-1] = (1 << 30) - 1  # an isolated last key
+        elif s % 5 == 1:
+            c = np.concatenate([[0], c[1:] | (1 << 29)]).astype(np.uint32)  # isolated first key
+        cases.append(np.unique(c))
+    for c in cases:
+        if c.size < 2:
+            continue
+        got = tree_from_adjacent_deltas(c)
+        want = orc.radix_tree(pool, c)
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (k, c.tolist())
+
+
 def test_radix_tree_matches_reference_source(orc, pool):
     if not orc.ref_available():
         pytest.skip("reference CPU kernels not built (no /root/reference here)")
