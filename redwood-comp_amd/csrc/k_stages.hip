@@ -226,7 +226,12 @@ __device__ __forceinline__ int ref_split(const uint32_t* codes, int i, int n, in
   return i + s * d + min(d, 0);
 }
 
-__device__ void ref_node(const TreeArgs& a, const uint32_t* codes, uint32_t base, int i, int n) {
+struct RefNode {
+  int dnode, gamma;
+  bool ll, lr;
+};
+
+__device__ RefNode ref_node(const uint32_t* codes, int i, int n) {
   const uint32_t ci = codes[i];
   int d = 1, l = 0;
   if (i == 0) {
@@ -242,23 +247,38 @@ __device__ void ref_node(const TreeArgs& a, const uint32_t* codes, uint32_t base
       if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
   }
   const int j = i + l * d;
-  const int dnode = delta8(ci, codes[j]);
-  const int gamma = ref_split(codes, i, n, d, l, dnode, ci);
-  const bool ll = min(i, j) == gamma;
-  const bool lr = max(i, j) == gamma + 1;
-  a.prefix_n[base + i] = static_cast<uint8_t>(dnode);
-  a.left_child[base + i] = gamma;
-  a.has_leaf_left[base + i] = ll ? 1 : 0;
-  a.has_leaf_right[base + i] = lr ? 1 : 0;
+  RefNode r;
+  r.dnode = delta8(ci, codes[j]);
+  r.gamma = ref_split(codes, i, n, d, l, r.dnode, ci);
+  r.ll = min(i, j) == r.gamma;
+  r.lr = max(i, j) == r.gamma + 1;
+  return r;
+}
+
+// The whole segment with the reference's search.  Such a tree need not be a
+// proper binary tree (a parent slot can have several claimers), so parent[]
+// is the maximum claimer over zeroed slots (D7) built with atomics by the
+// segment's first workgroup alone (zeroing, barrier, claims); the other
+// outputs are written by each node's own thread.
+__device__ void ref_tile(const TreeArgs& a, const uint32_t* codes, uint32_t base, int b0, int n) {
+  const int tid = static_cast<int>(threadIdx.x);
+  if (b0 + tid < n) {
+    const int i = b0 + tid;
+    const RefNode r = ref_node(codes, i, n);
+    a.prefix_n[base + i] = static_cast<uint8_t>(r.dnode);
+    a.left_child[base + i] = r.gamma;
+    a.has_leaf_left[base + i] = r.ll ? 1 : 0;
+    a.has_leaf_right[base + i] = r.lr ? 1 : 0;
+  }
+  if (b0 != 0) return;
   int32_t* parent = a.parent + base;
-  if (i > 0) {
-    if (!ll) parent[gamma] = i;
-    if (!lr) parent[gamma + 1] = i;
-  } else {
-    parent[0] = 0;
-    const int g = ref_split(codes, 0, n, 1, n, delta8(ci, codes[n]), ci);  // complete range
-    if (g != 0) parent[g] = 0;
-    if (g + 1 < n) parent[g + 1] = 0;
+  for (int i = tid; i < n; i += static_cast<int>(blockDim.x)) parent[i] = 0;
+  __syncthreads();
+  // node 0's claims (its Q2 children, the complete root's children) are all 0
+  for (int i = max(tid, 1); i < n; i += static_cast<int>(blockDim.x)) {
+    const RefNode r = ref_node(codes, i, n);
+    if (!r.ll) atomicMax(&parent[r.gamma], i);
+    if (!r.lr) atomicMax(&parent[r.gamma + 1], i);
   }
 }
 
@@ -364,7 +384,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs s
   const int tid = static_cast<int>(threadIdx.x);
 
   if (gcodes[n] >= 0x80000000u) {  // keys with bit 31 set: the reference's own search
-    if (b0 + tid < n) ref_node(a, gcodes, sg.base, b0 + tid, n);
+    ref_tile(a, gcodes, sg.base, b0, n);
     return;
   }
   int32_t* __restrict__ parent = a.parent + sg.base;
