@@ -275,7 +275,8 @@ __device__ void ref_tile(const TreeArgs& a, const uint32_t* codes, uint32_t base
   for (int i = tid; i < n; i += static_cast<int>(blockDim.x)) parent[i] = 0;
   __syncthreads();
   // node 0's claims (its Q2 children, the complete root's children) are all 0
-  for (int i = max(tid, 1); i < n; i += static_cast<int>(blockDim.x)) {
+  for (int i = tid; i < n; i += static_cast<int>(blockDim.x)) {
+    if (i == 0) continue;
     const RefNode r = ref_node(codes, i, n);
     if (!r.ll) atomicMax(&parent[r.gamma], i);
     if (!r.lr) atomicMax(&parent[r.gamma + 1], i);
