@@ -507,7 +507,62 @@ __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs s
       }
     }
   }
-  // beyond the window: min(a[w1 .. t]) < v, i.e. delta8(code[w1], code[t + 1]) < v
+  // 5. the node split at k (splits whose range leaves the window wait: step 7)
+  const auto a_at = [&](int x) -> int {
+    int r = s_a[min(max(x - w0, 0), kTreeWin - 1)];
+    asm volatile("" : "+v"(r));  // keep the LDS read; global only outside the window
+    if (x < w0 || x >= w1) r = delta8(gcodes[x], gcodes[x + 1]);
+    return r;
+  };
+  // staged: into the tile's LDS copy (when inside the tile), else straight out
+  const auto put = [&](int idx, int pfx, int lc, bool ll, bool lr, bool staged) {
+    const int e = idx - b0;
+    if (staged && e >= 0 && e < kTreeBlock) {
+      s_pfx[e] = static_cast<uint8_t>(pfx);
+      s_lc[e] = lc;
+      s_ll[e] = ll ? 1 : 0;
+      s_lr[e] = lr ? 1 : 0;
+      s_own[e] = 1;
+    } else {
+      a.prefix_n[sg.base + idx] = static_cast<uint8_t>(pfx);
+      a.left_child[sg.base + idx] = lc;
+      a.has_leaf_left[sg.base + idx] = ll ? 1 : 0;
+      a.has_leaf_right[sg.base + idx] = lr ? 1 : 0;
+    }
+  };
+  const auto emit = [&](bool staged) {
+    const int L = psv + 1, R = nsv;
+    int idx = 0;  // the complete-range root has index 0 (its outputs are Q2's)
+    if (psv >= 0 || nsv < n) {
+      const int ap = (psv >= 0) ? a_at(psv) : -1;
+      const int an = (nsv < n) ? a_at(nsv) : -1;
+      idx = (ap > an) ? L : R;
+      put(idx, static_cast<int>(v), k, L == k, R == k + 1, staged);
+    }
+    // node 0 (Q2): the split of [0, n-1]
+    if (psv == -1 && (nsv == n - 1 || (nsv == n && k < n - 1)))
+      put(0, static_cast<int>(v), k, k == 0, k + 1 == n - 1, staged);
+    if (L != k) parent[k] = idx;
+    if (R != k + 1) parent[k + 1] = idx;
+    if (k == 0) parent[0] = 0;
+  };
+  const bool far = far_n || far_p;
+  if (valid && !far) emit(true);
+  __syncthreads();
+
+  // 6. the tile's staged outputs, 16 bytes per lane where whole
+  const int nv = min(kTreeBlock, n - b0);
+  const uint32_t at = sg.base + static_cast<uint32_t>(b0);
+  if (tid < 64) flush_owned(a.prefix_n + at, s_pfx, s_own, nv, tid, 64);
+  else if (tid < 128) flush_owned(a.has_leaf_left + at, s_ll, s_own, nv, tid - 64, 64);
+  else if (tid < 192) flush_owned(a.has_leaf_right + at, s_lr, s_own, nv, tid - 128, 64);
+  else if (tid >= 256 && tid < 512) flush_owned(a.left_child + at, s_lc, s_own, nv, tid - 256, 256);
+
+  // 7. splits whose range leaves the window: wave-cooperative searches in
+  // global memory, after the barrier so that the other waves are not held,
+  // and outputs written straight out (the flush skipped their entries).
+  if (!ballot(far)) return;
+  // min(a[w1 .. t]) < v, i.e. delta8(code[w1], code[t + 1]) < v
   for (uint64_t m = ballot(far_n); m; m &= m - 1) {
     const int o = __ffsll(static_cast<long long>(m)) - 1;
     const int ov = __shfl(static_cast<int>(v), o, kWave);
@@ -523,54 +578,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs s
     const int r = wave_first_true(0, w0 - 1, [&](int t) { return delta8(gcodes[t], c0) >= ov; });
     if (lane == o) psv = r - 1;
   }
-
-  // 5. the node split at k
-  const auto a_at = [&](int x) -> int {
-    int r = s_a[min(max(x - w0, 0), kTreeWin - 1)];
-    asm volatile("" : "+v"(r));  // keep the LDS read; global only outside the window
-    if (x < w0 || x >= w1) r = delta8(gcodes[x], gcodes[x + 1]);
-    return r;
-  };
-  const auto put = [&](int idx, int pfx, int lc, bool ll, bool lr) {
-    const int e = idx - b0;
-    if (e >= 0 && e < kTreeBlock) {
-      s_pfx[e] = static_cast<uint8_t>(pfx);
-      s_lc[e] = lc;
-      s_ll[e] = ll ? 1 : 0;
-      s_lr[e] = lr ? 1 : 0;
-      s_own[e] = 1;
-    } else {
-      a.prefix_n[sg.base + idx] = static_cast<uint8_t>(pfx);
-      a.left_child[sg.base + idx] = lc;
-      a.has_leaf_left[sg.base + idx] = ll ? 1 : 0;
-      a.has_leaf_right[sg.base + idx] = lr ? 1 : 0;
-    }
-  };
-  if (valid) {
-    const int L = psv + 1, R = nsv;
-    int idx = 0;  // the complete-range root has index 0 (its outputs are Q2's)
-    if (psv >= 0 || nsv < n) {
-      const int ap = (psv >= 0) ? a_at(psv) : -1;
-      const int an = (nsv < n) ? a_at(nsv) : -1;
-      idx = (ap > an) ? L : R;
-      put(idx, static_cast<int>(v), k, L == k, R == k + 1);
-    }
-    // node 0 (Q2): the split of [0, n-1]
-    if (psv == -1 && (nsv == n - 1 || (nsv == n && k < n - 1)))
-      put(0, static_cast<int>(v), k, k == 0, k + 1 == n - 1);
-    if (L != k) parent[k] = idx;
-    if (R != k + 1) parent[k + 1] = idx;
-    if (k == 0) parent[0] = 0;
-  }
-  __syncthreads();
-
-  // 6. the tile's outputs, 16 bytes per lane where whole
-  const int nv = min(kTreeBlock, n - b0);
-  const uint32_t at = sg.base + static_cast<uint32_t>(b0);
-  if (tid < 64) flush_owned(a.prefix_n + at, s_pfx, s_own, nv, tid, 64);
-  else if (tid < 128) flush_owned(a.has_leaf_left + at, s_ll, s_own, nv, tid - 64, 64);
-  else if (tid < 192) flush_owned(a.has_leaf_right + at, s_lr, s_own, nv, tid - 128, 64);
-  else if (tid >= 256 && tid < 512) flush_owned(a.left_child + at, s_lc, s_own, nv, tid - 256, 256);
+  if (far) emit(false);
 }
 
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
