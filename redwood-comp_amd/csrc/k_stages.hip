@@ -283,310 +283,108 @@ __device__ void ref_tile(const TreeArgs& a, const uint32_t* codes, uint32_t base
   }
 }
 
-// ---- nearest smaller adjacent deltas (keys below 2^31) ---------------------
-// With a[k] = delta8(code[k], code[k+1]) and keys sorted, unique and below
-// 2^31, delta8(code[x], code[y]) = min(a[x..y-1]), and the reference's searches
-// reduce to nearest-smaller-value queries: the internal node split at k covers
-// keys [L, R] = [psv(k) + 1, nsv(k)] (psv/nsv: nearest position on the left /
-// right with a smaller a, -1 / n if none), its prefix is a[k], its children
-// are k and k + 1, and its index is L if a[psv] > a[nsv] (it is its parent's
-// right child) and R otherwise.  The split with no smaller value on either
-// side is the root of the complete range [0, n]; node 0 reports the range
-// [0, n-1] instead (Q2): the same split unless that is n - 1, else the split
-// whose range is exactly [0, n-1].  Pinned against the reference search on
-// the CPU by the tests.
-//
-// A workgroup owns kTreeBlock splits and keeps a[] for a window reaching
-// kTreeHalo further on each side in LDS, in blocks of 64 (one wave):
-//   * inside the split's own block: binary lifting over a per-wave sparse
-//     table of minima (6 steps);
-//   * across blocks: a bitmask per threshold t of the blocks whose minimum is
-//     below t picks the block, and a binary search over that block's prefix
-//     (or suffix) minima the position;
-//   * beyond the window: a wave-cooperative 64-ary search in global memory.
-constexpr int kTreeBlock = 1024;  // splits per workgroup, one per thread
-constexpr int kTreeHalo = 1024;   // window margin on each side (multiple of 64)
-constexpr int kTreeWin = kTreeBlock + 2 * kTreeHalo;
-constexpr int kTreeWinBlocks = kTreeWin / kWave;
-static_assert(kTreeWinBlocks <= 64, "one bit per window block");
-static_assert(kChunk % kTreeBlock == 0, "tiles must not straddle segments");
-
-// First t in [lo, hi] where pred holds, for pred false ... false true ... true
-// on [lo, hi]; hi + 1 if it never holds.  Wave-uniform arguments; every lane
-// calls it; 64 probes per round trip.
-template <class Pred>
-__device__ int wave_first_true(int lo, int hi, Pred pred) {
-  const uint32_t lane = lane_id();
-  for (;;) {
-    if (lo > hi) return lo;
-    const uint32_t span = static_cast<uint32_t>(hi - lo) + 1u;
-    if (span <= static_cast<uint32_t>(kWave)) {
-      const int t = lo + static_cast<int>(lane);
-      const uint64_t b = ballot(t <= hi && pred(t));
-      return b ? lo + static_cast<int>(__ffsll(static_cast<long long>(b)) - 1) : hi + 1;
-    }
-    // strictly increasing probes, the last one at hi
-    const int t = lo + static_cast<int>((static_cast<uint64_t>(lane + 1) * span) / kWave) - 1;
-    const uint64_t b = ballot(pred(t));
-    if (b == 0) return hi + 1;
-    const int c = __ffsll(static_cast<long long>(b)) - 1;
-    const int tc = __shfl(t, c, kWave);
-    const int tp = (c == 0) ? lo - 1 : __shfl(t, c - 1, kWave);
-    lo = tp + 1;
-    hi = tc;
+// ---- Karras search through an LDS window (keys below 2^31) ---------------
+// Codes of a segment, read through an LDS copy of the window around the
+// workgroup's nodes (the searches of most nodes stay inside it) and from
+// global memory outside it.  The LDS read is issued unconditionally (at a
+// clamped index; the empty asm statement pins its result) and the global one
+// only outside the window, so the common case stays a ds_read instead of both
+// becoming one flat load.
+struct CodeWindow {
+  const uint32_t* g;  // segment's codes
+  const uint32_t* s;  // LDS copy of g[lo, hi)
+  int lo, hi;
+  __device__ __forceinline__ uint32_t operator[](int k) const {
+    uint32_t v = s[min(max(k - lo, 0), hi - lo - 1)];
+    asm volatile("" : "+v"(v));
+    if (k < lo || k >= hi) v = g[k];
+    return v;
   }
+};
+
+constexpr int kTreeHalo = 384;  // codes cached on each side of the workgroup's nodes
+
+// Split search of the reference (func_brt.hpp:90-104) for node i with range
+// length l, direction d; returns gamma.
+__device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, int d, int l,
+                                         int dnode, uint32_t ci) {
+  int s = 0;
+  const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
+  const int scut = (d == -1) ? i - 1 : n - i - 1;
+  uint32_t t = (static_cast<uint32_t>(l) + 1) / 2;
+  for (uint32_t div = 2; div <= max_div; div <<= 1) {
+    if (s + static_cast<int>(t) <= scut && delta8(ci, codes[i + (s + static_cast<int>(t)) * d]) > dnode)
+      s += static_cast<int>(t);
+    t = (static_cast<uint32_t>(l) + (div << 1) - 1) / (div << 1);
+  }
+  return i + s * d + min(d, 0);
 }
 
-// Writes the tile's entries [0, nv) that this workgroup produced (own[e] =
-// 1) to global memory: whole 16-byte groups with one store, the rest entry by
-// entry (dst 16-byte aligned).
-template <typename T>
-__device__ __forceinline__ void flush_owned(T* __restrict__ dst, const T* src, const uint8_t* own,
-                                            int nv, int first_group, int lanes) {
-  constexpr int kPer = 16 / sizeof(T);
-  for (int g = first_group; g * kPer < nv; g += lanes) {
-    bool whole = (g + 1) * kPer <= nv;
-    if (whole) {
-      if constexpr (kPer == 16) {
-        const uint4 o = reinterpret_cast<const uint4*>(own)[g];
-        whole = (o.x & o.y & o.z & o.w) == 0x01010101u;
-      } else {
-        whole = reinterpret_cast<const uint32_t*>(own)[g] == 0x01010101u;
-      }
-    }
-    if (whole) {
-      *reinterpret_cast<uint4*>(dst + g * kPer) = reinterpret_cast<const uint4*>(src)[g];
-    } else {
-      for (int e = g * kPer; e < min(nv, (g + 1) * kPer); ++e)
-        if (own[e]) dst[e] = src[e];
-    }
-  }
-}
-
-__global__ __launch_bounds__(kTreeBlock) void k_radix_tree(TreeArgs a, SegArgs seg, uint32_t n_tiles) {
-  __shared__ uint8_t s_a[kTreeWin];                   // a[w0 + p], 0xff past the window
-  __shared__ uint8_t s_pmin[kTreeWin], s_smin[kTreeWin];  // per-block prefix / suffix minima
-  __shared__ uint64_t s_below[32];                    // bit b: block b's minimum < t
-  __shared__ __align__(16) int32_t s_lc[kTreeBlock];
-  __shared__ __align__(16) uint8_t s_pfx[kTreeBlock];
-  __shared__ __align__(16) uint8_t s_ll[kTreeBlock];
-  __shared__ __align__(16) uint8_t s_lr[kTreeBlock];
-  __shared__ __align__(16) uint8_t s_own[kTreeBlock];
-
-  // neighbouring tiles share window codes: XCD-grouped order
-  const uint32_t tile = xcd_tile();
-  if (tile >= n_tiles) return;
-  const uint32_t first = tile * kTreeBlock;
-  const Seg sg = seg_of(seg, first);
+// One thread per node i.  With keys below 2^31 (the largest is the last)
+// the tree is a proper binary tree and every parent slot has one claimer,
+// stored plainly; otherwise the segment takes the reference path above.
+__global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs seg) {
+  __shared__ uint32_t s_codes[kElemBlock + 2 * kTreeHalo];
+  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
+  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
   const int n = static_cast<int>(sg.count);  // n_brt; codes hold n + 1 keys
-  const int b0 = static_cast<int>(first - sg.base);
-  if (first < sg.base || b0 >= n) return;  // whole workgroup past the segment's nodes
+  const int b0 = static_cast<int>(blockIdx.x * kElemBlock - sg.base);
+  if (b0 >= n) return;  // whole workgroup past the segment's nodes
   const uint32_t* __restrict__ gcodes = a.codes + sg.base;
-  const int tid = static_cast<int>(threadIdx.x);
-
   if (gcodes[n] >= 0x80000000u) {  // keys with bit 31 set: the reference's own search
     ref_tile(a, gcodes, sg.base, b0, n);
     return;
   }
+  CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kElemBlock + kTreeHalo, n + 1)};
+  for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kElemBlock)
+    s_codes[k - codes.lo] = gcodes[k];
+  __syncthreads();
+  const int i = static_cast<int>(gi - sg.base);
+  if (i >= n) return;
   int32_t* __restrict__ parent = a.parent + sg.base;
-  if (n == 1) {  // two keys: node 0 alone, over [0, 0] (Q2)
-    if (tid == 0) {
-      a.prefix_n[sg.base] = static_cast<uint8_t>(delta8(gcodes[0], gcodes[0]));
-      a.left_child[sg.base] = 0;
-      a.has_leaf_left[sg.base] = 1;
-      a.has_leaf_right[sg.base] = 0;
-      parent[0] = 0;
-    }
-    return;
-  }
+  const uint32_t ci = codes[i];
 
-  const int lane = tid % kWave, wv = tid / kWave;
-  const int w0 = max(b0 - kTreeHalo, 0);
-  const int w1 = min(b0 + kTreeBlock + kTreeHalo, n);  // a[] window [w0, w1)
-  const int nblk = (w1 - w0 + kWave - 1) / kWave;
-
-  // 1. a[] over the window
-  for (int p = tid; p < kTreeWin; p += kTreeBlock) {
-    const int k = w0 + p;
-    s_a[p] = (k < w1) ? static_cast<uint8_t>(delta8(gcodes[k], gcodes[k + 1])) : 0xffu;
+  int d, l = 0;
+  if (i == 0) {
+    d = 1;
+    l = n - 1;  // Q2
+  } else {
+    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
+    d = (diff > 0) - (diff < 0);
+    const int dmin = delta8(ci, codes[i - d]);
+    int lmax = 2;
+    while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
+    const int cutoff = (d == -1) ? i : n - i;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
   }
-  s_own[tid] = 0;
-  __syncthreads();
-
-  // 2. per-block prefix / suffix minima
-  for (int blk = wv; blk < nblk; blk += kTreeBlock / kWave) {
-    const int p = blk * kWave + lane;
-    uint32_t pm = s_a[p], sm = pm;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const uint32_t u = __shfl_up(pm, o, kWave);
-      const uint32_t v = __shfl_down(sm, o, kWave);
-      if (lane >= o) pm = min(pm, u);
-      if (lane + o < kWave) sm = min(sm, v);
-    }
-    s_pmin[p] = static_cast<uint8_t>(pm);
-    s_smin[p] = static_cast<uint8_t>(sm);
+  const int j = i + l * d;
+  const int dnode = delta8(ci, codes[j]);
+  const int gamma = brt_split(codes, i, n, d, l, dnode, ci);
+  const bool ll = min(i, j) == gamma;
+  const bool lr = max(i, j) == gamma + 1;
+  a.prefix_n[sg.base + i] = static_cast<uint8_t>(dnode);
+  a.left_child[sg.base + i] = gamma;
+  a.has_leaf_left[sg.base + i] = ll ? 1 : 0;
+  a.has_leaf_right[sg.base + i] = lr ? 1 : 0;
+  if (i > 0) {
+    if (!ll) parent[gamma] = i;
+    if (!lr) parent[gamma + 1] = i;
+  } else {
+    parent[0] = 0;
+    // children of the root over the complete range [0, n]
+    const int dfull = delta8(ci, codes[n]);
+    const int g = brt_split(codes, 0, n, 1, n, dfull, ci);
+    if (g != 0) parent[g] = 0;
+    if (g + 1 < n) parent[g + 1] = 0;
   }
-  __syncthreads();
-
-  // 3. blocks below each threshold (one wave)
-  if (wv == 0) {
-    const uint32_t bm = (lane < nblk) ? s_pmin[lane * kWave + kWave - 1] : 0xffu;
-    uint64_t mine = 0;
-#pragma unroll
-    for (int t = 0; t < 32; ++t) {
-      const uint64_t m = ballot(bm < static_cast<uint32_t>(t));
-      if (lane == t) mine = m;
-    }
-    if (lane < 32) s_below[lane] = mine;
-  }
-  __syncthreads();
-
-  // 4. nearest smaller values of this thread's split k
-  const int k = b0 + tid;
-  const bool valid = k < n;
-  const uint32_t v = s_a[k - w0];
-  const int wbase = b0 + wv * kWave;          // first split of this wave
-  const int wblk = (wbase - w0) / kWave;      // its window block
-  // sparse table of minima over the wave: lvl[j] at lane i = min(a[i .. i + 2^j - 1])
-  uint32_t lvl[6];
-  lvl[0] = v;
-#pragma unroll
-  for (int j = 1; j < 6; ++j) {
-    const uint32_t u = __shfl_down(lvl[j - 1], 1 << (j - 1), kWave);
-    lvl[j] = (lane + (1 << (j - 1)) < kWave) ? min(lvl[j - 1], u) : lvl[j - 1];
-  }
-  // right: skip maximal runs >= v (binary lifting); 64 = none in the wave
-  int nx = lane + 1;
-#pragma unroll
-  for (int j = 5; j >= 0; --j) {
-    const uint32_t m = __shfl(lvl[j], min(nx, kWave - 1), kWave);
-    if (nx + (1 << j) <= kWave && m >= v) nx += 1 << j;
-  }
-  // left: runs ending at px; -1 = none in the wave
-  int px = lane - 1;
-#pragma unroll
-  for (int j = 5; j >= 0; --j) {
-    const int st = px - (1 << j) + 1;
-    const uint32_t m = __shfl(lvl[j], max(st, 0), kWave);
-    if (st >= 0 && m >= v) px -= 1 << j;
-  }
-  int nsv = n, psv = -1;
-  bool far_n = false, far_p = false;
-  if (valid) {
-    if (nx < kWave) {
-      nsv = wbase + nx;
-    } else {
-      const uint64_t cand = s_below[v] & (wblk + 1 < 64 ? (~0ull << (wblk + 1)) : 0ull);
-      if (cand) {
-        const int b2 = __ffsll(static_cast<long long>(cand)) - 1;
-        const uint8_t* pm = s_pmin + b2 * kWave;
-        int q = 0;  // first position of block b2 below v
-#pragma unroll
-        for (int st = 32; st >= 1; st >>= 1)
-          if (pm[q + st - 1] >= v) q += st;
-        nsv = w0 + b2 * kWave + q;
-      } else if (w1 < n) {
-        far_n = true;
-      }
-    }
-    if (px >= 0) {
-      psv = wbase + px;
-    } else {
-      const uint64_t cand = s_below[v] & ((1ull << wblk) - 1);
-      if (cand) {
-        const int b2 = 63 - __clzll(static_cast<long long>(cand));
-        const uint8_t* sm = s_smin + b2 * kWave;
-        int q = kWave - 1;  // last position of block b2 below v
-#pragma unroll
-        for (int st = 32; st >= 1; st >>= 1)
-          if (sm[q - st + 1] >= v) q -= st;
-        psv = w0 + b2 * kWave + q;
-      } else if (w0 > 0) {
-        far_p = true;
-      }
-    }
-  }
-  // 5. the node split at k (splits whose range leaves the window wait: step 7)
-  const auto a_at = [&](int x) -> int {
-    int r = s_a[min(max(x - w0, 0), kTreeWin - 1)];
-    asm volatile("" : "+v"(r));  // keep the LDS read; global only outside the window
-    if (x < w0 || x >= w1) r = delta8(gcodes[x], gcodes[x + 1]);
-    return r;
-  };
-  // staged: into the tile's LDS copy (when inside the tile), else straight out
-  const auto put = [&](int idx, int pfx, int lc, bool ll, bool lr, bool staged) {
-    const int e = idx - b0;
-    if (staged && e >= 0 && e < kTreeBlock) {
-      s_pfx[e] = static_cast<uint8_t>(pfx);
-      s_lc[e] = lc;
-      s_ll[e] = ll ? 1 : 0;
-      s_lr[e] = lr ? 1 : 0;
-      s_own[e] = 1;
-    } else {
-      a.prefix_n[sg.base + idx] = static_cast<uint8_t>(pfx);
-      a.left_child[sg.base + idx] = lc;
-      a.has_leaf_left[sg.base + idx] = ll ? 1 : 0;
-      a.has_leaf_right[sg.base + idx] = lr ? 1 : 0;
-    }
-  };
-  const auto emit = [&](bool staged) {
-    const int L = psv + 1, R = nsv;
-    int idx = 0;  // the complete-range root has index 0 (its outputs are Q2's)
-    if (psv >= 0 || nsv < n) {
-      const int ap = (psv >= 0) ? a_at(psv) : -1;
-      const int an = (nsv < n) ? a_at(nsv) : -1;
-      idx = (ap > an) ? L : R;
-      put(idx, static_cast<int>(v), k, L == k, R == k + 1, staged);
-    }
-    // node 0 (Q2): the split of [0, n-1]
-    if (psv == -1 && (nsv == n - 1 || (nsv == n && k < n - 1)))
-      put(0, static_cast<int>(v), k, k == 0, k + 1 == n - 1, staged);
-    if (L != k) parent[k] = idx;
-    if (R != k + 1) parent[k + 1] = idx;
-    if (k == 0) parent[0] = 0;
-  };
-  const bool far = far_n || far_p;
-  if (valid && !far) emit(true);
-  __syncthreads();
-
-  // 6. the tile's staged outputs, 16 bytes per lane where whole
-  const int nv = min(kTreeBlock, n - b0);
-  const uint32_t at = sg.base + static_cast<uint32_t>(b0);
-  if (tid < 64) flush_owned(a.prefix_n + at, s_pfx, s_own, nv, tid, 64);
-  else if (tid < 128) flush_owned(a.has_leaf_left + at, s_ll, s_own, nv, tid - 64, 64);
-  else if (tid < 192) flush_owned(a.has_leaf_right + at, s_lr, s_own, nv, tid - 128, 64);
-  else if (tid >= 256 && tid < 512) flush_owned(a.left_child + at, s_lc, s_own, nv, tid - 256, 256);
-
-  // 7. splits whose range leaves the window: wave-cooperative searches in
-  // global memory, after the barrier so that the other waves are not held,
-  // and outputs written straight out (the flush skipped their entries).
-  if (!ballot(far)) return;
-  // min(a[w1 .. t]) < v, i.e. delta8(code[w1], code[t + 1]) < v
-  for (uint64_t m = ballot(far_n); m; m &= m - 1) {
-    const int o = __ffsll(static_cast<long long>(m)) - 1;
-    const int ov = __shfl(static_cast<int>(v), o, kWave);
-    const uint32_t c1 = gcodes[w1];
-    const int r = wave_first_true(w1, n - 1, [&](int t) { return delta8(c1, gcodes[t + 1]) < ov; });
-    if (lane == o) nsv = r;
-  }
-  // min(a[t .. w0 - 1]) >= v, i.e. delta8(code[t], code[w0]) >= v; psv is one before
-  for (uint64_t m = ballot(far_p); m; m &= m - 1) {
-    const int o = __ffsll(static_cast<long long>(m)) - 1;
-    const int ov = __shfl(static_cast<int>(v), o, kWave);
-    const uint32_t c0 = gcodes[w0];
-    const int r = wave_first_true(0, w0 - 1, [&](int t) { return delta8(gcodes[t], c0) >= ov; });
-    if (lane == o) psv = r - 1;
-  }
-  if (far) emit(false);
 }
 
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
-  const uint32_t n_tiles = (n_index + kTreeBlock - 1) / kTreeBlock;
-  if (n_tiles == 0) return;
+  const uint32_t grid = (n_index + kElemBlock - 1) / kElemBlock;
+  if (grid == 0) return;
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_radix_tree, dim3(xcd_grid(n_tiles)), dim3(kTreeBlock), 0, stream, a, seg,
-                     n_tiles);
+  hipLaunchKernelGGL(k_radix_tree, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
 }
 
 // ---------------------------------------------------------------------------
