@@ -49,13 +49,23 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
     const uint32_t body = first + ((end - first) & ~3u);
     const uint4* v4 = reinterpret_cast<const uint4*>(keys + first);
     const uint32_t n4 = (body - first) >> 2;
-    for (uint32_t i = threadIdx.x; i < n4; i += kHistBlock) {
-      const uint4 q = v4[i];
-      const uint32_t k[4] = {q.x, q.y, q.z, q.w};
+    // four 16-byte loads in flight per thread before their counts
+    constexpr uint32_t kDepth = 4;
+    for (uint32_t i0 = threadIdx.x; i0 < n4; i0 += kDepth * kHistBlock) {
+      uint4 q[kDepth];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        for (uint32_t p = 0; p < n_passes; ++p)
-          atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & mask[p])], 1u);
+      for (uint32_t r = 0; r < kDepth; ++r)
+        if (i0 + r * kHistBlock < n4) q[r] = v4[i0 + r * kHistBlock];
+#pragma unroll
+      for (uint32_t r = 0; r < kDepth; ++r) {
+        if (i0 + r * kHistBlock >= n4) break;
+        const uint32_t k[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (uint32_t p = 0; p < kMaxPasses; ++p)
+            if (p < n_passes) atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & mask[p])], 1u);
+      }
     }
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
       const uint32_t k = keys[i];
@@ -157,8 +167,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
     uint32_t keys[KPT];
     if (n_valid == TILE) {
+      const uint32_t* src = in + wbase;  // one base, immediate offsets per key
 #pragma unroll
-      for (int k = 0; k < KPT; ++k) keys[k] = in[wbase + k * kWave];
+      for (int k = 0; k < KPT; ++k) keys[k] = src[k * kWave];
     } else {
 #pragma unroll
       for (int k = 0; k < KPT; ++k) {
@@ -177,15 +188,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       const uint32_t d = (keys[k] >> shift) & dmask;
-      uint32_t mis_lo = 0, mis_hi = 0;  // lanes whose digit differs in some bit
+      uint32_t pl = ~0u, ph = ~0u;  // lanes whose digit agrees in every bit so far
 #pragma unroll
       for (int b = 0; b < BITS; ++b) {
-        const uint64_t bal = ballot(((d >> b) & 1u) != 0);
         const uint32_t own = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-        mis_lo |= static_cast<uint32_t>(bal) ^ own;
-        mis_hi |= static_cast<uint32_t>(bal >> 32) ^ own;
+        const uint64_t bal = ballot(own != 0);
+        pl &= ~(own ^ static_cast<uint32_t>(bal));
+        ph &= ~(own ^ static_cast<uint32_t>(bal >> 32));
       }
-      const uint64_t m = ~((static_cast<uint64_t>(mis_hi) << 32) | mis_lo);
+      const uint64_t m = (static_cast<uint64_t>(ph) << 32) | pl;
       const uint32_t below = popc_below(m);
       const uint32_t old = wh[d];
       if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));

@@ -18,7 +18,7 @@ struct SortVariant {
 constexpr SortVariant kSortVariants[] = {{256, 16}, {512, 16}, {256, 32}, {1024, 8}, {512, 8}};
 constexpr int kNumSortVariants = sizeof(kSortVariants) / sizeof(kSortVariants[0]);
 constexpr uint32_t kSortMinTile = 4096;  // smallest tile: sizes the look-back table
-constexpr int kHistBlock = 512;
+constexpr int kHistBlock = 1024;
 constexpr uint32_t kHistCopies = 16;
 constexpr uint32_t kMaxPasses = 4;
 constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
