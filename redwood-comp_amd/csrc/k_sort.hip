@@ -142,6 +142,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   __shared__ uint32_t s_whist[WAVES][256];
   __shared__ uint32_t s_tstart[256];
   __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_count[256];  // the tile's digit counts (valid keys)
   __shared__ uint32_t s_tmp[4];
   __shared__ Grab s_grab;
 
@@ -165,6 +166,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t wbase = first + wave * (KPT * kWave) + lane;
 
     for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
+    if (tid < 256) s_count[tid] = 0;
     uint32_t keys[KPT];
     if (n_valid == TILE) {
       const uint32_t* src = in + wbase;  // one base, immediate offsets per key
@@ -178,6 +180,27 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       }
     }
     __syncthreads();
+
+    // The tile's digit counts first (LDS atomics), so that its aggregate is
+    // published, and its own look-back done, before the ranking: successors
+    // waiting on this tile are released early.
+#pragma unroll
+    for (int k = 0; k < KPT; ++k)
+      if (n_valid == TILE || wbase + k * kWave < lim) atomicAdd(&s_count[(keys[k] >> shift) & dmask], 1u);
+    __syncthreads();
+    uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles
+    if (tid < 256) {
+      const uint32_t d = tid;
+      const uint32_t cnt_valid = s_count[d];
+      uint64_t* slot = &lookback[static_cast<size_t>(tile) * 256 + d];
+      if (tile_in_seg == 0) {
+        lb_store(slot, lb_pack(epoch, kIncl, cnt_valid));
+      } else {
+        lb_store(slot, lb_pack(epoch, kAgg, cnt_valid));
+        excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
+        lb_store(slot, lb_pack(epoch, kIncl, excl + cnt_valid));
+      }
+    }
 
     // Wave-level multisplit: lanes holding the same digit find each other with
     // BITS ballots (a lane is a peer unless some ballot differs from this
@@ -204,8 +227,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     }
     __syncthreads();
 
-    // Per digit: exclusive offsets of the waves, tile count, publish aggregate.
-    uint32_t cnt = 0, cnt_valid = 0;
+    // Per digit: exclusive offsets of the waves and the tile's digit starts
+    // (padding keys included: they are staged last and never written out).
+    uint32_t cnt = 0;
     if (tid < 256) {
       const uint32_t d = tid;
       uint32_t run = 0;
@@ -216,13 +240,14 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
         run += t;
       }
       cnt = run;
-      cnt_valid = run - ((d == dmask) ? (TILE - n_valid) : 0u);
-      lb_store(&lookback[static_cast<size_t>(tile) * 256 + d],
-               lb_pack(epoch, tile_in_seg == 0 ? kIncl : kAgg, cnt_valid));
     }
     uint32_t total;
     const uint32_t tstart = scan256(cnt, s_tmp, &total);
-    if (tid < 256) s_tstart[tid] = tstart;
+    if (tid < 256) {
+      s_tstart[tid] = tstart;
+      s_gbase[tid] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + tid] +
+                     excl - tstart;
+    }
     __syncthreads();
 
     // Stage the tile in digit order.
@@ -230,18 +255,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     for (int k = 0; k < KPT; ++k) {
       const uint32_t d = (keys[k] >> shift) & dmask;
       s_stage[s_tstart[d] + s_whist[wave][d] + rank[k]] = keys[k];
-    }
-
-    // Look back for this digit's exclusive prefix over the segment's earlier tiles.
-    if (tid < 256) {
-      const uint32_t d = tid;
-      uint32_t excl = 0;
-      if (tile_in_seg > 0) {
-        excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
-        lb_store(&lookback[static_cast<size_t>(tile) * 256 + d], lb_pack(epoch, kIncl, excl + cnt_valid));
-      }
-      s_gbase[d] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + d] +
-                   excl - tstart;
     }
     __syncthreads();
 
