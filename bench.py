@@ -43,6 +43,19 @@ def stage_bytes(n, nu, nb, no):
     }
 
 
+def measured_sort_traffic(log2n):
+    """HBM bytes of one sort call from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated by a 256 MiB copy); None for a
+    workload that summary does not cover."""
+    path = os.path.join(ROOT, "profiles", f"build_2p{log2n}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("sort_hbm_bytes_per_call:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def enqueue_stages(rw, ws, tb, n, stream, events=None):
     """The stage entry points in build order; events[i] recorded before stage i."""
     L = rw.lib()
@@ -271,7 +284,7 @@ def main():
                    "n_points": n, "n_unique": nu, "n_octree_nodes": no},
         "roofline": {"bound": "hbm", "kernel": "S2 onesweep radix sort (histogram + 4 digit passes)",
                      "achieved": sort["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sort["frac"], "traffic": None},
+                     "frac": sort["frac"], "traffic": measured_sort_traffic(args.log2n)},
         "stages": stages,
         "sort_variants": {f"{b}x{k}": round(ms, 4) for (b, k), ms in zip(rw.sort_variants(), variant_ms)},
         "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
