@@ -113,7 +113,7 @@ __device__ int full_root_split(const uint32_t* codes, int n) {
 
 // ---------------------------------------------------------------------------
 // The node kernel.  A workgroup owns kOctSpan consecutive radix nodes (a
-// "tile"; thread t owns the four nodes starting at 4t) and first builds, in
+// "tile"; loader thread t reads the four nodes starting at 4t) and first builds, in
 // LDS, a pull descriptor for every node of a window reaching kOctHalo nodes
 // beyond the tile on either side:
 //   count > 0:  x = top slot (offset + count - 1), y = kLinkBit | octant of
@@ -125,8 +125,9 @@ __device__ int full_root_split(const uint32_t* codes, int n) {
 // descriptor from global memory.  The node arrays are read with 16-byte
 // (int) and 4-byte (byte arrays) loads, four nodes per thread.
 // ---------------------------------------------------------------------------
-constexpr int kOctBlock = 256;
-constexpr int kOctSpan = kOctBlock * 4;  // radix nodes per workgroup
+constexpr int kOctBlock = 512;                // threads per workgroup
+constexpr int kOctLoaders = 256;              // threads that read four nodes each
+constexpr int kOctSpan = kOctLoaders * 4;     // radix nodes per workgroup
 constexpr int kOctHalo = 128;            // window margin on each side (multiple of 4)
 constexpr int kOctWin = kOctSpan + 2 * kOctHalo;
 constexpr uint32_t kLinkBit = 0x80000000u;
@@ -207,11 +208,12 @@ __device__ __forceinline__ uint2 desc_global(const OctArgs& a, uint32_t gq) {
                    a.left_child[gq], a.has_leaf_left[gq] != 0, a.has_leaf_right[gq] != 0);
 }
 
-// Merges a register row into this thread's LDS row; returns the mask.
+// Merges a register row into this thread's LDS row (entry w at row[w *
+// kOctBlock]); returns the mask.
 __device__ __forceinline__ int32_t merge_row(const int32_t (&r)[8], int32_t m, int32_t* row) {
 #pragma unroll
   for (int w = 0; w < 8; ++w)
-    if ((m >> w) & 1) atomicMax(&row[w], r[w]);
+    if ((m >> w) & 1) atomicMax(&row[w * kOctBlock], r[w]);
   return m;
 }
 
@@ -265,22 +267,22 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
       if (q >= w.w0 && q < w.w1) d = w.desc[q - w.w0];
       else d = desc_global(a, base + static_cast<uint32_t>(q));
       if (d.y & kLinkBit) {
-        atomicMax(&row[d.y & 7u], static_cast<int32_t>(d.x));
+        atomicMax(&row[(d.y & 7u) * kOctBlock], static_cast<int32_t>(d.x));
         m |= static_cast<int32_t>(1u << (d.y & 7u));
       } else if (sp < kPullStack) {
-        stk[sp++] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
+        stk[(sp++) * kOctBlock] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
       } else {
         m |= pull_overflow(a, base, q, row);
       }
     }
     if (sp == 0) break;
-    pair = stk[--sp];
+    pair = stk[(--sp) * kOctBlock];
   }
   return m;
 }
 
 // Corners, cell sizes and rows.  A workgroup owns kOctSpan consecutive radix
-// nodes (a "tile"; thread t owns the four starting at 4t).  It builds the
+// nodes (a "tile"; loader thread t reads the four starting at 4t).  It builds the
 // window's descriptors in LDS, then walks the contiguous slot range its nodes
 // own ([offset of its first node, offset + count of its last node)), one slot
 // per thread, so every store is coalesced: the owner of slot s is the last
@@ -294,8 +296,10 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   __shared__ __align__(16) uint32_t s_code[kOctSpan];
   __shared__ __align__(16) uint32_t s_lcf[kOctSpan];  // left_child | leaf_l << 30 | leaf_r << 31
   __shared__ uint32_t s_lvl[kOctSpan / 4];  // prefix_n / 3, one byte per node
-  __shared__ __align__(16) int32_t s_row[kOctBlock * 8];  // each thread's pulled row
-  __shared__ uint32_t s_stk[kOctBlock * kPullStack];      // each thread's pull stack
+  // each thread's pulled row and pull stack, entry-major (entry e of thread t
+  // at [e * kOctBlock + t]) so that a wave's accesses fall in distinct banks
+  __shared__ int32_t s_row[8 * kOctBlock];
+  __shared__ uint32_t s_stk[kPullStack * kOctBlock];
   __shared__ float s_cell[32];  // range / 2^k
   __shared__ int64_t s_end;
 
@@ -324,14 +328,14 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   w.w0 = max(b - kOctHalo, 0);
   w.w1 = min(b + kOctSpan + kOctHalo, n);
 
-  // 1. the tile's nodes (four per thread) and the window's descriptors
+  // 1. the tile's nodes (four per loader thread) and the window's
+  // descriptors (the margins read by the other threads meanwhile)
   if (tid < 32) s_cell[tid] = __fdiv_rn(a.range, static_cast<float>(1u << tid));
-  const int q0 = b + 4 * static_cast<int>(tid);
-  const Nodes4 g = load_nodes4(a, sg.base, q0, n);
   constexpr int kHaloGroups = kOctHalo / 4;
-  if (tid < 2 * kHaloGroups) {
-    const int h = (tid < kHaloGroups) ? b - kOctHalo + 4 * static_cast<int>(tid)
-                                      : b + kOctSpan + 4 * static_cast<int>(tid - kHaloGroups);
+  static_assert(kOctBlock - kOctLoaders >= 2 * kHaloGroups, "one thread per margin group");
+  const int ht = static_cast<int>(tid) - kOctLoaders;
+  if (ht >= 0 && ht < 2 * kHaloGroups) {
+    const int h = (ht < kHaloGroups) ? b - kOctHalo + 4 * ht : b + kOctSpan + 4 * (ht - kHaloGroups);
     if (h >= w.w0 && h < w.w1) {
       const Nodes4 hg = load_nodes4(a, sg.base, h, n);
 #pragma unroll
@@ -339,7 +343,9 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
         if (h + e < w.w1) s_desc[h + e - w.w0] = hg.desc(e);
     }
   }
-  if (q0 < n) {
+  const int q0 = b + 4 * static_cast<int>(tid);
+  if (tid < kOctLoaders && q0 < n) {
+    const Nodes4 g = load_nodes4(a, sg.base, q0, n);
     uint32_t lcf[4], lv = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -367,8 +373,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   __syncthreads();
 
   // 2. the slots, one per thread
-  int32_t* row = s_row + tid * 8;
-  uint32_t* stk = s_stk + tid * kPullStack;
+  int32_t* row = s_row + tid;
+  uint32_t* stk = s_stk + tid;
   const int64_t lo = max<int64_t>(s_off[0], 0);
   const int64_t hi = min<int64_t>(s_end, rows);
   for (int64_t slot = lo + tid; slot < hi; slot += kOctBlock) {
@@ -393,12 +399,12 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
       r1 = make_int4(which == 4 ? c : 0, which == 5 ? c : 0, which == 6 ? c : 0, which == 7 ? c : 0);
       m = static_cast<int32_t>(1u << which);
     } else {  // the bottom: pull the chains that link here
-      reinterpret_cast<int4*>(row)[0] = make_int4(0, 0, 0, 0);
-      reinterpret_cast<int4*>(row)[1] = make_int4(0, 0, 0, 0);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) row[e * kOctBlock] = 0;
       m = pull_pair(a, sg.base, w, n, s_lcf[k], row, stk);
       if (slot == 0) m |= pull_root(a, sg.base, n, row);
-      r0 = reinterpret_cast<const int4*>(row)[0];
-      r1 = reinterpret_cast<const int4*>(row)[1];
+      r0 = make_int4(row[0], row[kOctBlock], row[2 * kOctBlock], row[3 * kOctBlock]);
+      r1 = make_int4(row[4 * kOctBlock], row[5 * kOctBlock], row[6 * kOctBlock], row[7 * kOctBlock]);
     }
     rows8[slot * 2] = r0;
     rows8[slot * 2 + 1] = r1;
