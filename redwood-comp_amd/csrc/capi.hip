@@ -549,8 +549,15 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
   RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, stream));
   RWH_TRY(rwh_radix_tree(bufs->morton_keys, d_nu, n, bufs->prefix_n, bufs->has_leaf_left,
                          bufs->has_leaf_right, bufs->left_child, bufs->parents, stream));
-  RWH_TRY(rwh_edge_count(bufs->prefix_n, bufs->parents, d_nu, n, bufs->edge_count, stream));
-  RWH_TRY(rwh_prefix_sum(ws, bufs->edge_count, d_nu, n, bufs->edge_offset, d_no, stream));
+  // S5 + S6 fused: the counts are computed inside the scan and written beside it
+  if (n == 0) {
+    RWH_HIP(hipMemsetAsync(d_no, 0, sizeof(uint32_t), static_cast<hipStream_t>(stream)));
+  } else {
+    launch_edge_scan(bufs->prefix_n, bufs->parents, bufs->edge_count, bufs->edge_offset,
+                     single_seg_dev(d_nu), n, d_no, ws->lb_scan, ws->ctrl + 2 * kCtrlWords,
+                     static_cast<hipStream_t>(stream));
+    RWH_TRY(launched());
+  }
   RWH_TRY(rwh_octree(bufs, d_nu, d_no, n, min_coord, range, stream));
   return RWH_OK;
 }
@@ -617,9 +624,8 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
   TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
               bufs->left_child, bufs->parents};
   launch_radix_tree(ta, uni, n_index, st);
-  launch_edge_count(bufs->prefix_n, bufs->parents, bufs->edge_count, uni, n_index, st);
-  launch_prefix_sum(bufs->edge_count, bufs->edge_offset, uni, n_index, ws->n_oct(), ws->lb_scan,
-                    ws->ctrl + 2 * kCtrlWords, st);
+  launch_edge_scan(bufs->prefix_n, bufs->parents, bufs->edge_count, bufs->edge_offset, uni, n_index,
+                   ws->n_oct(), ws->lb_scan, ws->ctrl + 2 * kCtrlWords, st);
   OctArgs oa = oct_args(bufs, min_coord, range);
   oa.n_oct = ws->n_oct();
   oa.oct_base = ws->oct_base();

@@ -419,11 +419,29 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // a one-workgroup scan there; D4).  Single pass with decoupled look-back;
 // the segment's last tile writes the total (n_octree_nodes).
 // ---------------------------------------------------------------------------
+// kFromTree: the fused build's S5 + S6.  The counts are not read but computed
+// from the tree (the S5 formula above) and written beside the offsets, which
+// saves the count array's round trip through HBM and one launch.
+struct EdgeSource {
+  const uint8_t* prefix_n;
+  const int32_t* parent;
+  int32_t* edge_count;  // written
+};
+
+__device__ __forceinline__ uint32_t edge_count_at(const EdgeSource& es, const Seg& sg, uint32_t gi,
+                                                  uint32_t pfx, int32_t par) {
+  if (gi == sg.base) return 0u;  // count[0] = 0
+  return static_cast<uint32_t>(static_cast<int>(pfx / 3) -
+                               static_cast<int>(es.prefix_n[sg.base + par] / 3));
+}
+
+template <bool kFromTree>
 __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __restrict__ in,
                                                            int32_t* __restrict__ out, SegArgs seg,
                                                            uint32_t* __restrict__ total,
                                                            uint64_t* __restrict__ lookback,
-                                                           uint32_t* __restrict__ ctrl) {
+                                                           uint32_t* __restrict__ ctrl,
+                                                           EdgeSource es) {
   constexpr int G = kScanKpt / 4;
   __shared__ uint32_t s_w[kScanBlock / kWave];
   __shared__ uint32_t s_bcast;
@@ -450,7 +468,36 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
     uint32_t v[G][4];
     uint32_t gsum[G];
     const bool full = first + kScanTile <= lim;
-    if (full) {  // plain 16-byte loads
+    if constexpr (kFromTree) {
+      if (full) {  // 16-byte parent loads, 4-byte prefix loads, gathered parent prefixes
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+          const int4 par = *reinterpret_cast<const int4*>(es.parent + i0);
+          const uint32_t pf = *reinterpret_cast<const uint32_t*>(es.prefix_n + i0);
+          v[g][0] = edge_count_at(es, sg, i0 + 0, pf & 0xffu, par.x);
+          v[g][1] = edge_count_at(es, sg, i0 + 1, (pf >> 8) & 0xffu, par.y);
+          v[g][2] = edge_count_at(es, sg, i0 + 2, (pf >> 16) & 0xffu, par.z);
+          v[g][3] = edge_count_at(es, sg, i0 + 3, pf >> 24, par.w);
+          *reinterpret_cast<int4*>(es.edge_count + i0) =
+              make_int4(static_cast<int>(v[g][0]), static_cast<int>(v[g][1]),
+                        static_cast<int>(v[g][2]), static_cast<int>(v[g][3]));
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const uint32_t i0 = wbase + 4 * (g * kWave + lane);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            v[g][c] = 0u;
+            if (i0 + c < lim) {
+              v[g][c] = edge_count_at(es, sg, i0 + c, es.prefix_n[i0 + c], es.parent[i0 + c]);
+              es.edge_count[i0 + c] = static_cast<int>(v[g][c]);
+            }
+          }
+        }
+      }
+    } else if (full) {  // plain 16-byte loads
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int4 q = *reinterpret_cast<const int4*>(in + wbase + 4 * (g * kWave + lane));
@@ -507,8 +554,18 @@ void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
   if (n_tiles == 0) return;
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_prefix_sum, dim3(n_tiles), dim3(kScanBlock), 0, stream, in, out, seg,
-                     total, lookback, ctrl);
+  hipLaunchKernelGGL(k_prefix_sum<false>, dim3(n_tiles), dim3(kScanBlock), 0, stream, in, out, seg,
+                     total, lookback, ctrl, EdgeSource{});
+}
+
+void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
+                      int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
+                      uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
+  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
+  if (n_tiles == 0) return;
+  seg.count_sub = 1;
+  hipLaunchKernelGGL(k_prefix_sum<true>, dim3(n_tiles), dim3(kScanBlock), 0, stream, nullptr, out,
+                     seg, total, lookback, ctrl, EdgeSource{prefix_n, parent, edge_count});
 }
 
 }  // namespace rw

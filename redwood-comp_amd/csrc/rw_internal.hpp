@@ -75,6 +75,11 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // Exclusive scan of the live count (n_brt) per segment; total per segment.
 void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
                        uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
+// S5 + S6 in one pass: edge counts computed from prefix_n / parent, written,
+// and scanned into out.
+void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
+                      int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
+                      uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
 
 struct OctArgs {
   const int32_t* edge_offset;
