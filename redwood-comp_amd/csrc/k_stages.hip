@@ -304,16 +304,23 @@ struct CodeWindow {
 
 constexpr int kTreeHalo = 384;  // codes cached on each side of the workgroup's nodes
 
+// With every key below 2^31, clz(a ^ b) >= 1, so delta8 = clz - 1 has no uint8
+// wrap and  delta8(ci, x) > t  <=>  (ci ^ x) < agree_below(t):  one xor and one
+// compare per probe instead of the clz arithmetic.
+__device__ __forceinline__ uint32_t agree_below(int t) {
+  return t >= 31 ? 0u : (1u << (30 - t));
+}
+
 // Split search of the reference (func_brt.hpp:90-104) for node i with range
-// length l, direction d; returns gamma.
+// length l, direction d, thr = agree_below(delta of the node); returns gamma.
 __device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, int d, int l,
-                                         int dnode, uint32_t ci) {
+                                         uint32_t thr, uint32_t ci) {
   int s = 0;
   const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
   const int scut = (d == -1) ? i - 1 : n - i - 1;
   uint32_t t = (static_cast<uint32_t>(l) + 1) / 2;
   for (uint32_t div = 2; div <= max_div; div <<= 1) {
-    if (s + static_cast<int>(t) <= scut && delta8(ci, codes[i + (s + static_cast<int>(t)) * d]) > dnode)
+    if (s + static_cast<int>(t) <= scut && (ci ^ codes[i + (s + static_cast<int>(t)) * d]) < thr)
       s += static_cast<int>(t);
     t = (static_cast<uint32_t>(l) + (div << 1) - 1) / (div << 1);
   }
@@ -349,18 +356,19 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     d = 1;
     l = n - 1;  // Q2
   } else {
-    const int diff = delta8(ci, codes[i + 1]) - delta8(ci, codes[i - 1]);
-    d = (diff > 0) - (diff < 0);
-    const int dmin = delta8(ci, codes[i - d]);
+    // sign(delta(i, i+1) - delta(i, i-1)) = sign of the clz difference
+    const int cp = __clz(ci ^ codes[i + 1]), cm = __clz(ci ^ codes[i - 1]);
+    d = (cp > cm) - (cp < cm);
+    const uint32_t thr = agree_below(delta8(ci, codes[i - d]));
     int lmax = 2;
-    while (i + lmax * d >= 0 && i + lmax * d <= n && delta8(ci, codes[i + lmax * d]) > dmin) lmax *= 2;
+    while (i + lmax * d >= 0 && i + lmax * d <= n && (ci ^ codes[i + lmax * d]) < thr) lmax *= 2;
     const int cutoff = (d == -1) ? i : n - i;
     for (int t = lmax / 2; t >= 1; t /= 2)
-      if (l + t <= cutoff && delta8(ci, codes[i + (l + t) * d]) > dmin) l += t;
+      if (l + t <= cutoff && (ci ^ codes[i + (l + t) * d]) < thr) l += t;
   }
   const int j = i + l * d;
   const int dnode = delta8(ci, codes[j]);
-  const int gamma = brt_split(codes, i, n, d, l, dnode, ci);
+  const int gamma = brt_split(codes, i, n, d, l, agree_below(dnode), ci);
   const bool ll = min(i, j) == gamma;
   const bool lr = max(i, j) == gamma + 1;
   a.prefix_n[sg.base + i] = static_cast<uint8_t>(dnode);
@@ -374,7 +382,7 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     parent[0] = 0;
     // children of the root over the complete range [0, n]
     const int dfull = delta8(ci, codes[n]);
-    const int g = brt_split(codes, 0, n, 1, n, dfull, ci);
+    const int g = brt_split(codes, 0, n, 1, n, agree_below(dfull), ci);
     if (g != 0) parent[g] = 0;
     if (g + 1 < n) parent[g + 1] = 0;
   }
