@@ -82,7 +82,9 @@ uint32_t rwh_workspace_max_segments(const rwh_workspace* ws);
 
 /* Tile shapes of the sort's digit passes: variant v runs workgroups of
  * block threads with keys_per_thread keys each.  A workspace uses variant 0
- * until told otherwise; the host layer's autotuner picks one per size. */
+ * until told otherwise; the host layer's autotuner picks one per size.
+ * Shapes whose tile exceeds 8192 keys serve single-cloud sorts; a batched
+ * sort with one of them selected uses the 1024 x 8 shape. */
 int rwh_sort_variant_count(void);
 int rwh_sort_variant_shape(int variant, uint32_t* block, uint32_t* keys_per_thread);
 int rwh_workspace_set_sort_variant(rwh_workspace* ws, int variant);

@@ -135,7 +135,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
-  static_assert(kChunk % TILE == 0, "tiles must not straddle segments");
+  // a tile never straddles two clouds: it divides a chunk, or (a multiple of
+  // the chunk) it is only launched for a single cloud
+  static_assert(kChunk % TILE == 0 || TILE % kChunk == 0, "tile and chunk must nest");
   constexpr uint32_t dmask = (1u << BITS) - 1u;
 
   __shared__ uint32_t s_stage[TILE];
@@ -302,12 +304,14 @@ void launch_sort(const SortLaunch& L, hipStream_t stream) {
                      L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist);
   hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                      L.digit_start, n_passes);
-  static_assert(kNumSortVariants == 5, "one case per variant");
-  switch (L.variant) {
+  static_assert(kNumSortVariants == 6, "one case per variant");
+  const int variant = (L.n_segs > 1 && !sort_tile_fits_chunk(L.variant)) ? kSortChunkVariant : L.variant;
+  switch (variant) {
     case 1: launch_passes<kSortVariants[1].block, kSortVariants[1].kpt>(L, n_passes, stream); break;
     case 2: launch_passes<kSortVariants[2].block, kSortVariants[2].kpt>(L, n_passes, stream); break;
     case 3: launch_passes<kSortVariants[3].block, kSortVariants[3].kpt>(L, n_passes, stream); break;
     case 4: launch_passes<kSortVariants[4].block, kSortVariants[4].kpt>(L, n_passes, stream); break;
+    case 5: launch_passes<kSortVariants[5].block, kSortVariants[5].kpt>(L, n_passes, stream); break;
     default: launch_passes<kSortVariants[0].block, kSortVariants[0].kpt>(L, n_passes, stream); break;
   }
 }

@@ -9,14 +9,21 @@
 
 namespace rw {
 
-// Sort shapes.  A digit-pass tile is BLOCK threads x KPT keys and divides
-// kChunk; the variants below are compiled, one is chosen per call (the
-// autotuner of the host layer picks per size bucket).
+// Sort shapes.  A digit-pass tile is BLOCK threads x KPT keys; the variants
+// below are compiled, one is chosen per call (the autotuner of the host layer
+// picks per size bucket).  A tile larger than kChunk could straddle two
+// clouds, so such shapes serve single-cloud sorts only; a batched sort with
+// one of them selected runs kSortChunkVariant instead.
 struct SortVariant {
   int block, kpt;
 };
-constexpr SortVariant kSortVariants[] = {{256, 16}, {512, 16}, {256, 32}, {1024, 8}, {512, 8}};
+constexpr SortVariant kSortVariants[] = {{256, 16}, {512, 16}, {256, 32}, {1024, 8}, {512, 8}, {1024, 16}};
 constexpr int kNumSortVariants = sizeof(kSortVariants) / sizeof(kSortVariants[0]);
+constexpr int kSortChunkVariant = 3;  // 1024 x 8: the chunk-sized tile
+constexpr bool sort_tile_fits_chunk(int v) {
+  return static_cast<uint32_t>(kSortVariants[v].block * kSortVariants[v].kpt) <= kChunk;
+}
+static_assert(sort_tile_fits_chunk(kSortChunkVariant), "fallback must fit a chunk");
 constexpr uint32_t kSortMinTile = 4096;  // smallest tile: sizes the look-back table
 constexpr int kHistBlock = 1024;
 constexpr uint32_t kHistCopies = 16;

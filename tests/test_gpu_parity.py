@@ -101,6 +101,23 @@ def test_sort_u32(gpu, n, key_bits):
     assert np.array_equal(d_out.numpy(), np.sort(k))
 
 
+@pytest.mark.parametrize("n", [1, 4097, 100000, 1 << 20])
+def test_sort_every_shape(gpu, n):
+    """Every compiled tile shape sorts exactly (single cloud, 30-bit keys)."""
+    rw = gpu
+    L = rw.lib()
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 1 << 30, size=n, dtype=np.uint64).astype(np.uint32)
+    d_in = rw.DeviceArray.from_numpy(k)
+    d_out = rw.DeviceArray(n, np.uint32)
+    ws = rw.Workspace(n)
+    for v in range(L.rwh_sort_variant_count()):
+        print(f"case sort shape {v} n={n}", flush=True)
+        ws.set_sort_variant(v)
+        rw.check(L.rwh_sort_u32(ws.ptr, d_in.ptr, d_out.ptr, n, 30, None))
+        assert np.array_equal(d_out.numpy(), np.sort(k)), v
+
+
 @pytest.mark.parametrize("n", [0, 1, 2, 4096, 4097, 100000])
 def test_unique_u32(gpu, n):
     rw = gpu
@@ -191,15 +208,20 @@ def test_dropin_failure_surfaces_in_future(gpu, stage):
     assert "injected" in str(e.value)
 
 
-def test_batch_equals_oracle_per_cloud(gpu, orc, pool):
+@pytest.mark.parametrize("shape", [None, 5])
+def test_batch_equals_oracle_per_cloud(gpu, orc, pool, shape):
+    """shape 5 (1024 x 16, a tile of two chunks) is chosen for the workspace:
+    the batched sort must fall back to a chunk-sized tile."""
     rw = gpu
     sizes = [0, 1, 2, 3, 1000, 8191, 8192, 8193, 30000, 70000, 5, 150000]
     clouds = []
     for s, n in enumerate(sizes):
         clouds.append(rw.uniform_points(n, 100 + s) if s % 2 == 0 else
                       rw.gmm_points(n, 100 + s, 16, 10.24))
-    print("case batch", flush=True)
+    print(f"case batch shape={shape}", flush=True)
     b = rw.Batch(clouds)
+    if shape is not None:
+        b.ws.set_sort_variant(shape)
     b.run()
     res = b.read()
     for s, c in enumerate(clouds):
