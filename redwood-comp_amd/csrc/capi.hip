@@ -118,7 +118,8 @@ namespace {
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs seg, uint32_t n_segs,
-                 uint32_t n_index, uint32_t key_bits, bool plain, hipStream_t st) {
+                 uint32_t n_index, uint32_t key_bits, bool plain, hipStream_t st,
+                 bool hist_ready = false) {
   SortLaunch L{};
   L.keys_in = in;
   L.keys_out = out;
@@ -138,6 +139,7 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
   L.digit_start = ws->digit_start;
   L.lookback = ws->lb_sort;
   L.ctrl = ws->ctrl + 0 * kCtrlWords;
+  L.hist_ready = hist_ready;
   if (n_index == 0) return RWH_OK;
   launch_sort(L, st);
   return launched();
@@ -544,8 +546,17 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
   RWH_TRY(check_bufs(bufs));
   uint32_t* d_nu = ws->n_unique();
   uint32_t* d_no = ws->n_oct();
-  RWH_TRY(rwh_morton(bufs->points, n, min_coord, range, bufs->morton_keys, stream));
-  RWH_TRY(rwh_sort_u32(ws, bufs->morton_keys, bufs->morton_keys_alt, n, 30, stream));
+  // S1 fused with the sort's histogram sweep, then the digit passes
+  if (injected(1) || injected(2)) return RWH_ERR_INJECTED;
+  RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
+  if (n > 0) {
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n,
+                       min_coord, range, 30, ws->hist, st);
+    RWH_TRY(launched());
+    RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
+                         st, true));
+  }
   RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, stream));
   RWH_TRY(rwh_radix_tree(bufs->morton_keys, d_nu, n, bufs->prefix_n, bufs->has_leaf_left,
                          bufs->has_leaf_right, bufs->left_child, bufs->parents, stream));

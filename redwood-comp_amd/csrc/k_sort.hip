@@ -81,6 +81,62 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Histogram sweep fused with stage 1, for a whole single-cloud build: each
+// workgroup encodes a unit of points (16-byte float4 loads, four in flight per
+// thread), writes the codes and counts their digits in LDS as k_digit_hist
+// does, so the sort does not read the codes an extra time.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __restrict__ pts,
+                                                            uint32_t* __restrict__ codes, uint32_t n,
+                                                            float min_coord, float range,
+                                                            uint32_t unit_keys, uint32_t n_passes,
+                                                            uint32_t last_bits,
+                                                            uint32_t* __restrict__ hist) {
+  uint32_t mask[kMaxPasses];
+#pragma unroll
+  for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
+  __shared__ uint32_t s_h[kMaxPasses * 256];
+  for (uint32_t i = threadIdx.x; i < kMaxPasses * 256; i += kHistBlock) s_h[i] = 0;
+  __syncthreads();
+  const uint32_t u = blockIdx.x;
+  const uint32_t first = u * unit_keys;
+  const uint32_t end = min(first + unit_keys, n);
+  constexpr uint32_t kDepth = 4;
+  for (uint32_t i0 = first + threadIdx.x; i0 < end; i0 += kDepth * kHistBlock) {
+    float4 q[kDepth];
+#pragma unroll
+    for (uint32_t r = 0; r < kDepth; ++r)
+      if (i0 + r * kHistBlock < end) q[r] = pts[i0 + r * kHistBlock];
+#pragma unroll
+    for (uint32_t r = 0; r < kDepth; ++r) {
+      const uint32_t i = i0 + r * kHistBlock;
+      if (i >= end) break;
+      const uint32_t k = morton_encode(q[r].x, q[r].y, q[r].z, min_coord, range);
+      codes[i] = k;
+#pragma unroll
+      for (uint32_t p = 0; p < kMaxPasses; ++p)
+        if (p < n_passes) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t* dst = hist + (u % kHistCopies) * (kMaxPasses * 256);
+  for (uint32_t i = threadIdx.x; i < n_passes * 256; i += kHistBlock) {
+    const uint32_t c = s_h[i];
+    if (c) atomicAdd(&dst[i], c);
+  }
+}
+
+void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
+                        uint32_t key_bits, uint32_t* hist, hipStream_t stream) {
+  if (n == 0) return;
+  const uint32_t n_passes = (key_bits + 7) / 8;
+  const uint32_t last_bits = key_bits - 8 * (n_passes - 1);
+  const uint32_t units = (n + kPlainHistUnit - 1) / kPlainHistUnit;
+  hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, n,
+                     min_coord, range, kPlainHistUnit, n_passes, last_bits, hist);
+}
+
 // Block-wide exclusive scan of one value per thread for the first 256 threads
 // (4 waves); s_tmp holds 4 words.  Returns the exclusive prefix; *total gets
 // the sum.  All threads of the block must call it.
@@ -300,8 +356,9 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
 void launch_sort(const SortLaunch& L, hipStream_t stream) {
   const uint32_t n_passes = (L.key_bits + 7) / 8;
   const uint32_t last_bits = L.key_bits - 8 * (n_passes - 1);
-  hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
-                     L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist);
+  if (!L.hist_ready)
+    hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
+                       L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist);
   hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                      L.digit_start, n_passes);
   static_assert(kNumSortVariants == 6, "one case per variant");

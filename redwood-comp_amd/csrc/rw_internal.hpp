@@ -57,8 +57,13 @@ struct SortLaunch {
   uint32_t* digit_start;   // [segs][passes][256]
   uint64_t* lookback;      // [n_tiles][256]
   uint32_t* ctrl;          // kCtrlWords
+  bool hist_ready;         // hist already counted (launch_morton_hist)
 };
 void launch_sort(const SortLaunch& L, hipStream_t stream);
+// Stage 1 for one cloud of n points fused with the sort's histogram sweep
+// (the digit histograms of key_bits-bit keys go to hist's copies).
+void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
+                        uint32_t key_bits, uint32_t* hist, hipStream_t stream);
 
 void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index,
                    float min_coord, float range, hipStream_t stream);
