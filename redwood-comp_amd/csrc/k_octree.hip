@@ -129,7 +129,6 @@ constexpr int kOctBlock = 512;                // threads per workgroup
 constexpr int kOctLoaders = 256;              // threads that read four nodes each
 constexpr int kOctSpan = kOctLoaders * 4;     // radix nodes per workgroup
 constexpr int kOctHalo = 128;            // window margin on each side (multiple of 4)
-constexpr int kOwnerSlots = 2048;        // slots of a tile with an owner mark in LDS
 constexpr int kOctWin = kOctSpan + 2 * kOctHalo;
 constexpr uint32_t kLinkBit = 0x80000000u;
 static_assert(kChunk % kOctSpan == 0, "tiles must not straddle segments");
@@ -303,9 +302,6 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   __shared__ uint32_t s_stk[kPullStack * kOctBlock];
   __shared__ float s_cell[32];  // range / 2^k
   __shared__ int64_t s_end;
-  // owner (tile-local node) of the tile's first kOwnerSlots slots; 0xffff:
-  // not marked (the binary search over the offsets decides)
-  __shared__ uint16_t s_owner[kOwnerSlots];
 
   // neighbouring tiles share window margins: XCD-grouped order
   const uint32_t tile = xcd_tile();
@@ -335,7 +331,6 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   // 1. the tile's nodes (four per loader thread) and the window's
   // descriptors (the margins read by the other threads meanwhile)
   if (tid < 32) s_cell[tid] = __fdiv_rn(a.range, static_cast<float>(1u << tid));
-  for (uint32_t i = tid; i < kOwnerSlots; i += kOctBlock) s_owner[i] = 0xffffu;
   constexpr int kHaloGroups = kOctHalo / 4;
   static_assert(kOctBlock - kOctLoaders >= 2 * kHaloGroups, "one thread per margin group");
   const int ht = static_cast<int>(tid) - kOctLoaders;
@@ -349,14 +344,11 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     }
   }
   const int q0 = b + 4 * static_cast<int>(tid);
-  int my_off[4] = {0, 0, 0, 0}, my_cnt[4] = {0, 0, 0, 0};  // kept for the owner marks
   if (tid < kOctLoaders && q0 < n) {
     const Nodes4 g = load_nodes4(a, sg.base, q0, n);
     uint32_t lcf[4], lv = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      my_off[e] = g.off[e];
-      my_cnt[e] = (q0 + e > 0 && q0 + e < n) ? g.cnt[e] : 0;
       if (q0 + e < n) s_desc[q0 + e - w.w0] = g.desc(e);
       lcf[e] = (static_cast<uint32_t>(g.lc[e]) & 0x3fffffffu) | (g.ll(e) ? 1u << 30 : 0u) |
                (g.lr(e) ? 1u << 31 : 0u);
@@ -380,30 +372,16 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   }
   __syncthreads();
 
-  // owner marks: each node writes its index over its chain's slots
-  const int64_t s0 = s_off[0];
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    for (int jj = 0; jj < my_cnt[e]; ++jj) {
-      const int64_t rel = static_cast<int64_t>(my_off[e]) + jj - s0;
-      if (rel >= 0 && rel < kOwnerSlots) s_owner[rel] = static_cast<uint16_t>(4 * tid + e);
-    }
-  __syncthreads();
-
   // 2. the slots, one per thread
   int32_t* row = s_row + tid;
   uint32_t* stk = s_stk + tid;
-  const int64_t lo = max<int64_t>(s0, 0);
+  const int64_t lo = max<int64_t>(s_off[0], 0);
   const int64_t hi = min<int64_t>(s_end, rows);
   for (int64_t slot = lo + tid; slot < hi; slot += kOctBlock) {
-    const int64_t rel = slot - s0;
-    int k = (rel < kOwnerSlots) ? static_cast<int>(s_owner[rel]) : 0xffff;
-    if (k == 0xffff) {  // past the marked slots: last node with offset <= slot
-      k = 0;
+    int k = 0;  // last node with offset <= slot
 #pragma unroll
-      for (int step = kOctSpan / 2; step >= 1; step >>= 1)
-        if (k + step < nv && s_off[k + step] <= slot) k += step;
-    }
+    for (int step = kOctSpan / 2; step >= 1; step >>= 1)
+      if (k + step < nv && s_off[k + step] <= slot) k += step;
     if (b + k == 0) continue;  // node 0 owns no chain (the reference starts at 1)
     const int j = static_cast<int>(slot - s_off[k]);
     const uint32_t ci = s_code[k];
