@@ -327,60 +327,11 @@ __device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, 
   return i + s * d + min(d, 0);
 }
 
-// Range end j, prefix, split and the node's outputs, for node i with direction
-// d and range length l (the reference's split search, func_brt.hpp:90-104);
-// node 0 also parents the internal children of the complete-range root.
-__device__ __forceinline__ void tree_node_out(const TreeArgs& a, const CodeWindow& codes,
-                                              uint32_t base, int i, int n, int d, int l,
-                                              uint32_t ci) {
-  int32_t* __restrict__ parent = a.parent + base;
-  const int j = i + l * d;
-  const int dnode = delta8(ci, codes[j]);
-  const int gamma = brt_split(codes, i, n, d, l, agree_below(dnode), ci);
-  const bool ll = min(i, j) == gamma;
-  const bool lr = max(i, j) == gamma + 1;
-  a.prefix_n[base + i] = static_cast<uint8_t>(dnode);
-  a.left_child[base + i] = gamma;
-  a.has_leaf_left[base + i] = ll ? 1 : 0;
-  a.has_leaf_right[base + i] = lr ? 1 : 0;
-  if (i > 0) {
-    if (!ll) parent[gamma] = i;
-    if (!lr) parent[gamma + 1] = i;
-  } else {
-    parent[0] = 0;
-    // children of the root over the complete range [0, n]
-    const int dfull = delta8(ci, codes[n]);
-    const int g = brt_split(codes, 0, n, 1, n, agree_below(dfull), ci);
-    if (g != 0) parent[g] = 0;
-    if (g + 1 < n) parent[g + 1] = 0;
-  }
-}
-
-// Direction and the threshold form of delta(i, i - d) (node i >= 1).
-__device__ __forceinline__ void tree_direction(const CodeWindow& codes, int i, uint32_t ci, int& d,
-                                               uint32_t& thr) {
-  // sign(delta(i, i+1) - delta(i, i-1)) = sign of the clz difference
-  const int cp = __clz(ci ^ codes[i + 1]), cm = __clz(ci ^ codes[i - 1]);
-  d = (cp > cm) - (cp < cm);
-  thr = agree_below(delta8(ci, codes[i - d]));
-}
-
 // One thread per node i.  With keys below 2^31 (the largest is the last)
 // the tree is a proper binary tree and every parent slot has one claimer,
 // stored plainly; otherwise the segment takes the reference path above.
-//
-// A wave runs each search loop as many times as its longest lane needs, and
-// most ranges are short while a few per wave are long.  So every lane first
-// probes the doubling steps up to kShortRange; a node whose range is still
-// growing there (and node 0) is queued in LDS, and the short ones finish at
-// once (their binary and split searches take at most 5 steps).  The queued
-// nodes then continue the same doubling sequence, packed densely into waves.
-constexpr int kShortRange = 32;
-
 __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs seg) {
   __shared__ uint32_t s_codes[kElemBlock + 2 * kTreeHalo];
-  __shared__ int s_long[kElemBlock];
-  __shared__ uint32_t s_nlong;
   const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
   const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
   const int n = static_cast<int>(sg.count);  // n_brt; codes hold n + 1 keys
@@ -394,62 +345,46 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
   CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kElemBlock + kTreeHalo, n + 1)};
   for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kElemBlock)
     s_codes[k - codes.lo] = gcodes[k];
-  if (threadIdx.x == 0) s_nlong = 0;
   __syncthreads();
   const int i = static_cast<int>(gi - sg.base);
-  const bool valid = i < n;
+  if (i >= n) return;
+  int32_t* __restrict__ parent = a.parent + sg.base;
+  const uint32_t ci = codes[i];
 
-  // 1. every node: the short part of the doubling search
-  bool queued = valid && i == 0;  // node 0's range is [0, n - 1] (Q2)
-  if (valid && i > 0) {
-    const uint32_t ci = codes[i];
-    int d;
-    uint32_t thr;
-    tree_direction(codes, i, ci, d, thr);
+  int d, l = 0;
+  if (i == 0) {
+    d = 1;
+    l = n - 1;  // Q2
+  } else {
+    // sign(delta(i, i+1) - delta(i, i-1)) = sign of the clz difference
+    const int cp = __clz(ci ^ codes[i + 1]), cm = __clz(ci ^ codes[i - 1]);
+    d = (cp > cm) - (cp < cm);
+    const uint32_t thr = agree_below(delta8(ci, codes[i - d]));
     int lmax = 2;
-    while (lmax <= kShortRange && i + lmax * d >= 0 && i + lmax * d <= n &&
-           (ci ^ codes[i + lmax * d]) < thr)
-      lmax *= 2;
-    if (lmax > kShortRange) {
-      queued = true;
-    } else {
-      const int cutoff = (d == -1) ? i : n - i;
-      int l = 0;
-      for (int t = lmax / 2; t >= 1; t /= 2)
-        if (l + t <= cutoff && (ci ^ codes[i + (l + t) * d]) < thr) l += t;
-      tree_node_out(a, codes, sg.base, i, n, d, l, ci);
-    }
-  }
-  const uint64_t bal = ballot(queued);
-  if (bal) {
-    const uint32_t lane = threadIdx.x % kWave;
-    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(bal)) - 1);
-    uint32_t at = 0;
-    if (lane == leader) at = atomicAdd(&s_nlong, static_cast<uint32_t>(__popcll(bal)));
-    at = __shfl(at, leader, kWave);
-    if (queued) s_long[at + popc_below(bal)] = i;
-  }
-  __syncthreads();
-
-  // 2. the long ranges, one per thread
-  const uint32_t n_long = s_nlong;
-  for (uint32_t e = threadIdx.x; e < n_long; e += kElemBlock) {
-    const int q = s_long[e];
-    const uint32_t cq = codes[q];
-    if (q == 0) {
-      tree_node_out(a, codes, sg.base, 0, n, 1, n - 1, cq);
-      continue;
-    }
-    int d;
-    uint32_t thr;
-    tree_direction(codes, q, cq, d, thr);
-    int lmax = 2 * kShortRange;  // every shorter doubling probe held
-    while (q + lmax * d >= 0 && q + lmax * d <= n && (cq ^ codes[q + lmax * d]) < thr) lmax *= 2;
-    const int cutoff = (d == -1) ? q : n - q;
-    int l = 0;
+    while (i + lmax * d >= 0 && i + lmax * d <= n && (ci ^ codes[i + lmax * d]) < thr) lmax *= 2;
+    const int cutoff = (d == -1) ? i : n - i;
     for (int t = lmax / 2; t >= 1; t /= 2)
-      if (l + t <= cutoff && (cq ^ codes[q + (l + t) * d]) < thr) l += t;
-    tree_node_out(a, codes, sg.base, q, n, d, l, cq);
+      if (l + t <= cutoff && (ci ^ codes[i + (l + t) * d]) < thr) l += t;
+  }
+  const int j = i + l * d;
+  const int dnode = delta8(ci, codes[j]);
+  const int gamma = brt_split(codes, i, n, d, l, agree_below(dnode), ci);
+  const bool ll = min(i, j) == gamma;
+  const bool lr = max(i, j) == gamma + 1;
+  a.prefix_n[sg.base + i] = static_cast<uint8_t>(dnode);
+  a.left_child[sg.base + i] = gamma;
+  a.has_leaf_left[sg.base + i] = ll ? 1 : 0;
+  a.has_leaf_right[sg.base + i] = lr ? 1 : 0;
+  if (i > 0) {
+    if (!ll) parent[gamma] = i;
+    if (!lr) parent[gamma + 1] = i;
+  } else {
+    parent[0] = 0;
+    // children of the root over the complete range [0, n]
+    const int dfull = delta8(ci, codes[n]);
+    const int g = brt_split(codes, 0, n, 1, n, agree_below(dfull), ci);
+    if (g != 0) parent[g] = 0;
+    if (g + 1 < n) parent[g + 1] = 0;
   }
 }
 
