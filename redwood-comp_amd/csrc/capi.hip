@@ -90,6 +90,7 @@ struct rwh_workspace {
   uint32_t* hist = nullptr;
   uint32_t* digit_start = nullptr;
   uint64_t* lb_sort = nullptr;
+  uint32_t* tile_counts = nullptr;  // [max_sort_tiles][256]: pass 0 tile prefixes
   uint64_t* lb_unique = nullptr;
   uint64_t* lb_scan = nullptr;
   uint32_t* ctrl = nullptr;    // 3 users x kCtrlWords
@@ -140,6 +141,7 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
   L.lookback = ws->lb_sort;
   L.ctrl = ws->ctrl + 0 * kCtrlWords;
   L.hist_ready = hist_ready;
+  L.tile_counts = ws->tile_counts;  // used when the sort is one cloud
   if (n_index == 0) return RWH_OK;
   launch_sort(L, st);
   return launched();
@@ -356,6 +358,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   const size_t o_hist = take(sizeof(uint32_t) * static_cast<size_t>(max_segments) * kHistCopies * kMaxPasses * 256);
   const size_t o_ds = take(sizeof(uint32_t) * static_cast<size_t>(max_segments) * kMaxPasses * 256);
   const size_t o_lbs = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_sort_tiles) * 256);
+  const size_t o_tc = take(sizeof(uint32_t) * static_cast<size_t>(ws->max_sort_tiles) * 256);
   const size_t o_lbu = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_scan_tiles));
   const size_t o_lbc = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_scan_tiles));
   const size_t o_ctrl = take(sizeof(uint32_t) * 3 * kCtrlWords);
@@ -374,6 +377,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   ws->hist = reinterpret_cast<uint32_t*>(base + o_hist);
   ws->digit_start = reinterpret_cast<uint32_t*>(base + o_ds);
   ws->lb_sort = reinterpret_cast<uint64_t*>(base + o_lbs);
+  ws->tile_counts = reinterpret_cast<uint32_t*>(base + o_tc);
   ws->lb_unique = reinterpret_cast<uint64_t*>(base + o_lbu);
   ws->lb_scan = reinterpret_cast<uint64_t*>(base + o_lbc);
   ws->ctrl = reinterpret_cast<uint32_t*>(base + o_ctrl);
@@ -552,7 +556,7 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
   if (n > 0) {
     const hipStream_t st = static_cast<hipStream_t>(stream);
     launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n,
-                       min_coord, range, 30, ws->hist, st);
+                       min_coord, range, 30, ws->hist, ws->sort_variant, ws->tile_counts, st);
     RWH_TRY(launched());
     RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
                          st, true));

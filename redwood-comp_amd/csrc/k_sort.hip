@@ -26,18 +26,48 @@ namespace rw {
 // the unit's counts are added to copy (u % kHistCopies) of its segment's
 // histogram, which spreads the global atomics over many cache lines.
 // ---------------------------------------------------------------------------
+// Pass 0's digit counts of the sort tiles a histogram unit covers (the unit
+// starts at a multiple of the tile size), from LDS to tile_counts[tile][256].
+__device__ __forceinline__ void store_tile_counts(const uint32_t* s_tc, uint32_t first, uint32_t end,
+                                                  uint32_t tile_shift, uint32_t* tile_counts) {
+  const uint32_t t0 = first >> tile_shift;
+  const uint32_t nt = ((end - first - 1) >> tile_shift) + 1;
+  for (uint32_t i = threadIdx.x; i < nt * 256; i += kHistBlock)
+    tile_counts[static_cast<size_t>(t0) * 256 + i] = s_tc[i];
+}
+
+// Pass 0's unit histogram as the sum of its tiles' counts (tiles past the
+// unit's keys hold zeros).  Ends with a barrier.
+__device__ __forceinline__ void finish_pass0(const uint32_t* s_tc, uint32_t* s_h) {
+  if (threadIdx.x < 256) {
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < kTilesPerHistUnit; ++t) c += s_tc[t * 256 + threadIdx.x];
+    s_h[threadIdx.x] = c;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __restrict__ keys,
                                                            SegArgs seg, uint32_t unit_keys,
                                                            uint32_t n_units, uint32_t n_passes,
                                                            uint32_t last_bits,
-                                                           uint32_t* __restrict__ hist) {
+                                                           uint32_t* __restrict__ hist,
+                                                           uint32_t tile_shift,
+                                                           uint32_t* __restrict__ tile_counts) {
   // the digit of pass p is masked exactly as the binning pass masks it, so
   // the counts match the ranks whatever the key values
   uint32_t mask[kMaxPasses];
 #pragma unroll
   for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
   __shared__ uint32_t s_h[kMaxPasses * 256];
+  __shared__ uint32_t s_tc[kTilesPerHistUnit * 256];  // pass 0 digit counts per sort tile
+  // With tile counts requested, pass 0 is counted per tile only and its unit
+  // histogram is summed from the tiles afterwards.
+  const bool per_tile = tile_counts != nullptr;
   for (uint32_t i = threadIdx.x; i < kMaxPasses * 256; i += kHistBlock) s_h[i] = 0;
+  if (per_tile)
+    for (uint32_t i = threadIdx.x; i < kTilesPerHistUnit * 256; i += kHistBlock) s_tc[i] = 0;
   __syncthreads();
   const uint32_t u = blockIdx.x;
   const uint32_t first = u * unit_keys;
@@ -60,19 +90,30 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
       for (uint32_t r = 0; r < kDepth; ++r) {
         if (i0 + r * kHistBlock >= n4) break;
         const uint32_t k[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
+        // the four keys share a tile (tiles hold a multiple of four keys)
+        uint32_t* tc = s_tc + (((4 * (i0 + r * kHistBlock)) >> tile_shift) * 256);
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < 4; ++c) {
 #pragma unroll
           for (uint32_t p = 0; p < kMaxPasses; ++p)
-            if (p < n_passes) atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & mask[p])], 1u);
+            if (p < n_passes && (p > 0 || !per_tile))
+              atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & mask[p])], 1u);
+          if (per_tile) atomicAdd(&tc[k[c] & mask[0]], 1u);
+        }
       }
     }
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
       const uint32_t k = keys[i];
-      for (uint32_t p = 0; p < n_passes; ++p) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+      for (uint32_t p = per_tile ? 1 : 0; p < n_passes; ++p)
+        atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+      if (per_tile) atomicAdd(&s_tc[((i - first) >> tile_shift) * 256 + (k & mask[0])], 1u);
     }
   }
   __syncthreads();
+  if (per_tile) {
+    if (first < lim) store_tile_counts(s_tc, first, min(first + unit_keys, lim), tile_shift, tile_counts);
+    finish_pass0(s_tc, s_h);
+  }
   uint32_t* dst = hist + (static_cast<size_t>(sg.id) * kHistCopies + (u % kHistCopies)) *
                              (kMaxPasses * 256);
   for (uint32_t i = threadIdx.x; i < n_passes * 256; i += kHistBlock) {
@@ -92,12 +133,20 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
                                                             float min_coord, float range,
                                                             uint32_t unit_keys, uint32_t n_passes,
                                                             uint32_t last_bits,
-                                                            uint32_t* __restrict__ hist) {
+                                                            uint32_t* __restrict__ hist,
+                                                            uint32_t tile_shift,
+                                                            uint32_t* __restrict__ tile_counts) {
   uint32_t mask[kMaxPasses];
 #pragma unroll
   for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
   __shared__ uint32_t s_h[kMaxPasses * 256];
+  __shared__ uint32_t s_tc[kTilesPerHistUnit * 256];  // pass 0 digit counts per sort tile
+  // With tile counts requested, pass 0 is counted per tile only and its unit
+  // histogram is summed from the tiles afterwards.
+  const bool per_tile = tile_counts != nullptr;
   for (uint32_t i = threadIdx.x; i < kMaxPasses * 256; i += kHistBlock) s_h[i] = 0;
+  if (per_tile)
+    for (uint32_t i = threadIdx.x; i < kTilesPerHistUnit * 256; i += kHistBlock) s_tc[i] = 0;
   __syncthreads();
   const uint32_t u = blockIdx.x;
   const uint32_t first = u * unit_keys;
@@ -116,10 +165,16 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
       codes[i] = k;
 #pragma unroll
       for (uint32_t p = 0; p < kMaxPasses; ++p)
-        if (p < n_passes) atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+        if (p < n_passes && (p > 0 || !per_tile))
+          atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+      if (per_tile) atomicAdd(&s_tc[((i - first) >> tile_shift) * 256 + (k & mask[0])], 1u);
     }
   }
   __syncthreads();
+  if (per_tile) {
+    if (first < end) store_tile_counts(s_tc, first, end, tile_shift, tile_counts);
+    finish_pass0(s_tc, s_h);
+  }
   uint32_t* dst = hist + (u % kHistCopies) * (kMaxPasses * 256);
   for (uint32_t i = threadIdx.x; i < n_passes * 256; i += kHistBlock) {
     const uint32_t c = s_h[i];
@@ -128,13 +183,42 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
 }
 
 void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
-                        uint32_t key_bits, uint32_t* hist, hipStream_t stream) {
+                        uint32_t key_bits, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
+                        hipStream_t stream) {
   if (n == 0) return;
   const uint32_t n_passes = (key_bits + 7) / 8;
   const uint32_t last_bits = key_bits - 8 * (n_passes - 1);
   const uint32_t units = (n + kPlainHistUnit - 1) / kPlainHistUnit;
+  if (n_passes < 2) tile_counts = nullptr;
   hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, n,
-                     min_coord, range, kPlainHistUnit, n_passes, last_bits, hist);
+                     min_coord, range, kPlainHistUnit, n_passes, last_bits, hist,
+                     sort_tile_shift(sort_variant), tile_counts);
+}
+
+// Pass 0's exclusive digit prefix of every tile of a single-cloud sort, in
+// place of the tile counts: one workgroup per digit scans that digit's column
+// (thread t sums a run of tiles, a block scan over the runs, then each thread
+// rewrites its run).
+__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ tile_counts,
+                                                    uint32_t n_tiles) {
+  __shared__ uint32_t s_w[1024 / kWave];
+  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  const uint32_t run = (n_tiles + 1023) / 1024;
+  const uint32_t t0 = t * run, t1 = min(t0 + run, n_tiles);
+  uint32_t sum = 0;
+  for (uint32_t k = t0; k < t1; ++k) sum += tile_counts[static_cast<size_t>(k) * 256 + d];
+  const uint32_t incl = wave_incl_scan(sum);
+  if (t % kWave == kWave - 1) s_w[t / kWave] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t w = 0; w < t / kWave; ++w) off += s_w[w];
+  uint32_t acc = off + incl - sum;
+  for (uint32_t k = t0; k < t1; ++k) {
+    const size_t at = static_cast<size_t>(k) * 256 + d;
+    const uint32_t c = tile_counts[at];
+    tile_counts[at] = acc;
+    acc += c;
+  }
 }
 
 // Block-wide exclusive scan of one value per thread for the first 256 threads
@@ -180,25 +264,32 @@ __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
 // ---------------------------------------------------------------------------
 // One onesweep digit pass.
 // Keys are loaded wave-striped (key k of lane l of wave w sits at
-// tile + w*KPT*64 + k*64 + l), which is the order the ranking walks, so the
-// sort is stable.
+// tile + w*KPT*64 + k*64 + l), which is the order the ranking walks, so every
+// pass after the first is stable.
 // ---------------------------------------------------------------------------
-template <int BLOCK, int KPT, int BITS>
+// kFirst (pass 0): the keys carry no order from earlier passes, and equal
+// digits of a keys-only sort need none, so ranks within a wave come from LDS
+// atomics instead of the stable multisplit.
+// kTable (pass 0 of a single-cloud sort): the tile is blockIdx.x and its
+// digit prefixes come from tile_excl, scanned from the histogram sweep's tile
+// counts, so the pass has no look-back chain.
+template <int BLOCK, int KPT, int BITS, bool kTable = false, bool kFirst = kTable>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg,
     const uint32_t* __restrict__ digit_start, uint32_t pass,
-    uint32_t n_passes, uint32_t shift, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl) {
+    uint32_t n_passes, uint32_t shift, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl,
+    const uint32_t* __restrict__ tile_excl) {
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
   // a tile never straddles two clouds: it divides a chunk, or (a multiple of
   // the chunk) it is only launched for a single cloud
   static_assert(kChunk % TILE == 0 || TILE % kChunk == 0, "tile and chunk must nest");
+  static_assert(!kTable || kFirst, "the tile table serves pass 0");
   constexpr uint32_t dmask = (1u << BITS) - 1u;
 
   __shared__ uint32_t s_stage[TILE];
   __shared__ uint32_t s_whist[WAVES][256];
-  __shared__ uint32_t s_tstart[256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_count[256];  // the tile's digit counts (valid keys)
   __shared__ uint32_t s_tmp[4];
@@ -207,14 +298,16 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave;
   const uint32_t lane = tid % kWave;
-  if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
-  __syncthreads();
-  const uint32_t epoch = s_grab.epoch;
+  if (!kTable) {
+    if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
+    __syncthreads();
+  }
+  const uint32_t epoch = kTable ? 0u : s_grab.epoch;
   // One tile per workgroup, in grab order: a tile only ever waits on tiles
   // whose workgroups took their grab earlier and are therefore running, so
   // the look-back spin ends whatever the residency of the grid.
   {
-    const uint32_t tile = s_grab.index;
+    const uint32_t tile = kTable ? blockIdx.x : s_grab.index;
     const uint32_t first = tile * TILE;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
@@ -242,12 +335,16 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     // The tile's digit counts first (LDS atomics), so that its aggregate is
     // published, and its own look-back done, before the ranking: successors
     // waiting on this tile are released early.
-#pragma unroll
-    for (int k = 0; k < KPT; ++k)
-      if (n_valid == TILE || wbase + k * kWave < lim) atomicAdd(&s_count[(keys[k] >> shift) & dmask], 1u);
-    __syncthreads();
     uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles
-    if (tid < 256) {
+    if (kTable) {
+      if (tid < 256) excl = tile_excl[static_cast<size_t>(tile) * 256 + tid];
+    } else {
+#pragma unroll
+      for (int k = 0; k < KPT; ++k)
+        if (n_valid == TILE || wbase + k * kWave < lim) atomicAdd(&s_count[(keys[k] >> shift) & dmask], 1u);
+      __syncthreads();
+    }
+    if (!kTable && tid < 256) {
       const uint32_t d = tid;
       const uint32_t cnt_valid = s_count[d];
       uint64_t* slot = &lookback[static_cast<size_t>(tile) * 256 + d];
@@ -260,24 +357,45 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       }
     }
 
-    // Wave-level multisplit: lanes holding the same digit find each other with
-    // BITS ballots (a lane is a peer unless some ballot differs from this
-    // lane's own bit); the lowest lane of each group advances the wave's
-    // counter for that digit.
+    // Ranks within the wave, per digit counter of the wave.
     uint32_t* wh = s_whist[wave];
     uint32_t rank[KPT];
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       const uint32_t d = (keys[k] >> shift) & dmask;
-      uint32_t pl = ~0u, ph = ~0u;  // lanes whose digit agrees in every bit so far
-#pragma unroll
-      for (int b = 0; b < BITS; ++b) {
-        const uint32_t own = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-        const uint64_t bal = ballot(own != 0);
-        pl &= ~(own ^ static_cast<uint32_t>(bal));
-        ph &= ~(own ^ static_cast<uint32_t>(bal >> 32));
+      if (kFirst) {
+        // padding keys take no rank: they are not staged
+        rank[k] = (n_valid == TILE || wbase + k * kWave < lim) ? atomicAdd(&wh[d], 1u) : 0u;
+        continue;
       }
-      const uint64_t m = (static_cast<uint64_t>(ph) << 32) | pl;
+      // Stable multisplit: lanes holding the same digit find each other with
+      // BITS ballots (a lane differs from this one where some ballot differs
+      // from this lane's own bit); the lowest lane of each group advances the
+      // wave's counter for that digit.
+      uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, 0, 1));
+      uint64_t bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
+      uint32_t xl = own ^ static_cast<uint32_t>(bal);
+      uint32_t xh = own ^ static_cast<uint32_t>(bal >> 32);
+#pragma unroll
+      for (int b = 1; b < BITS; ++b) {
+        own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, b, 1));
+        bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
+        // truth tables indexed by (x << 2 | own << 1 | bal): 0xF6 is
+        // x | (own ^ bal); the last bit takes its complement 0x09, so that
+        // x ends as the peer mask
+        if (b + 1 < BITS) {
+          xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0xF6);
+          xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0xF6);
+        } else {
+          xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0x09);
+          xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0x09);
+        }
+      }
+      if (BITS == 1) {
+        xl = ~xl;
+        xh = ~xh;
+      }
+      const uint64_t m = (static_cast<uint64_t>(xh) << 32) | xl;  // peers, this lane included
       const uint32_t below = popc_below(m);
       const uint32_t old = wh[d];
       if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
@@ -285,24 +403,24 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     }
     __syncthreads();
 
-    // Per digit: exclusive offsets of the waves and the tile's digit starts
-    // (padding keys included: they are staged last and never written out).
+    // Per digit: the tile's digit starts, then each wave's start of the digit
+    // in the staged tile (stable ranks count padding keys, staged last and
+    // never written out; pass 0 does not count them).
     uint32_t cnt = 0;
     if (tid < 256) {
-      const uint32_t d = tid;
-      uint32_t run = 0;
 #pragma unroll
-      for (int w = 0; w < WAVES; ++w) {
-        const uint32_t t = s_whist[w][d];
-        s_whist[w][d] = run;
-        run += t;
-      }
-      cnt = run;
+      for (int w = 0; w < WAVES; ++w) cnt += s_whist[w][tid];
     }
     uint32_t total;
     const uint32_t tstart = scan256(cnt, s_tmp, &total);
     if (tid < 256) {
-      s_tstart[tid] = tstart;
+      uint32_t run = tstart;
+#pragma unroll
+      for (int w = 0; w < WAVES; ++w) {
+        const uint32_t t = s_whist[w][tid];
+        s_whist[w][tid] = run;
+        run += t;
+      }
       s_gbase[tid] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + tid] +
                      excl - tstart;
     }
@@ -312,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
 #pragma unroll
     for (int k = 0; k < KPT; ++k) {
       const uint32_t d = (keys[k] >> shift) & dmask;
-      s_stage[s_tstart[d] + s_whist[wave][d] + rank[k]] = keys[k];
+      if (!kFirst || n_valid == TILE || wbase + k * kWave < lim) s_stage[wh[d] + rank[k]] = keys[k];
     }
     __syncthreads();
 
@@ -339,10 +457,26 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
     const bool to_out = ((n_passes - p) % 2) == 1;
     uint32_t* dst = to_out ? L.keys_out : L.keys_tmp;
     const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
+    if (p == 0 && L.tile_counts) {
+      // pass 0 (8 bits: n_passes >= 2) from the tile prefixes, no look-back
+      hipLaunchKernelGGL(k_tile_scan, dim3(256), dim3(1024), 0, stream, L.tile_counts, n_tiles);
+      hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, true>), dim3(n_tiles), dim3(BLOCK), 0, stream, src,
+                         dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl,
+                         L.tile_counts);
+      src = dst;
+      continue;
+    }
+    if (p == 0 && bits == 8) {
+      hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, false, true>), dim3(n_tiles), dim3(BLOCK), 0, stream,
+                         src, dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, nullptr);
+      src = dst;
+      continue;
+    }
 #define RW_ONESWEEP(B)                                                                       \
   case B:                                                                                    \
     hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_tiles), dim3(BLOCK), 0, stream, src, \
-                       dst, L.seg, L.digit_start, p, n_passes, 8 * p, L.lookback, L.ctrl);  \
+                       dst, L.seg, L.digit_start, p, n_passes, 8 * p, L.lookback, L.ctrl,   \
+                       nullptr);                                                             \
     break;
     switch (bits) {
       RW_ONESWEEP(1) RW_ONESWEEP(2) RW_ONESWEEP(3) RW_ONESWEEP(4)
@@ -356,20 +490,24 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
 void launch_sort(const SortLaunch& L, hipStream_t stream) {
   const uint32_t n_passes = (L.key_bits + 7) / 8;
   const uint32_t last_bits = L.key_bits - 8 * (n_passes - 1);
+  const int variant = (L.n_segs > 1 && !sort_tile_fits_chunk(L.variant)) ? kSortChunkVariant : L.variant;
+  SortLaunch M = L;
+  // pass 0 from tile prefixes: one cloud, whole histogram units, 8-bit pass 0
+  if (!(L.n_segs == 1 && L.hist_unit_keys == kPlainHistUnit && n_passes >= 2)) M.tile_counts = nullptr;
   if (!L.hist_ready)
     hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
-                       L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist);
+                       L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist,
+                       sort_tile_shift(variant), M.tile_counts);
   hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                      L.digit_start, n_passes);
   static_assert(kNumSortVariants == 6, "one case per variant");
-  const int variant = (L.n_segs > 1 && !sort_tile_fits_chunk(L.variant)) ? kSortChunkVariant : L.variant;
   switch (variant) {
-    case 1: launch_passes<kSortVariants[1].block, kSortVariants[1].kpt>(L, n_passes, stream); break;
-    case 2: launch_passes<kSortVariants[2].block, kSortVariants[2].kpt>(L, n_passes, stream); break;
-    case 3: launch_passes<kSortVariants[3].block, kSortVariants[3].kpt>(L, n_passes, stream); break;
-    case 4: launch_passes<kSortVariants[4].block, kSortVariants[4].kpt>(L, n_passes, stream); break;
-    case 5: launch_passes<kSortVariants[5].block, kSortVariants[5].kpt>(L, n_passes, stream); break;
-    default: launch_passes<kSortVariants[0].block, kSortVariants[0].kpt>(L, n_passes, stream); break;
+    case 1: launch_passes<kSortVariants[1].block, kSortVariants[1].kpt>(M, n_passes, stream); break;
+    case 2: launch_passes<kSortVariants[2].block, kSortVariants[2].kpt>(M, n_passes, stream); break;
+    case 3: launch_passes<kSortVariants[3].block, kSortVariants[3].kpt>(M, n_passes, stream); break;
+    case 4: launch_passes<kSortVariants[4].block, kSortVariants[4].kpt>(M, n_passes, stream); break;
+    case 5: launch_passes<kSortVariants[5].block, kSortVariants[5].kpt>(M, n_passes, stream); break;
+    default: launch_passes<kSortVariants[0].block, kSortVariants[0].kpt>(M, n_passes, stream); break;
   }
 }
 

@@ -29,6 +29,21 @@ constexpr int kHistBlock = 1024;
 constexpr uint32_t kHistCopies = 16;
 constexpr uint32_t kMaxPasses = 4;
 constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
+constexpr uint32_t kTilesPerHistUnit = kPlainHistUnit / kSortMinTile;
+// log2 of a variant's tile size; a single-cloud histogram unit holds whole tiles
+constexpr uint32_t sort_tile_shift(int v) {
+  uint32_t s = 0;
+  while ((1u << s) < static_cast<uint32_t>(kSortVariants[v].block * kSortVariants[v].kpt)) ++s;
+  return s;
+}
+constexpr bool sort_tiles_nest() {
+  for (int v = 0; v < kNumSortVariants; ++v) {
+    const uint32_t t = kSortVariants[v].block * kSortVariants[v].kpt;
+    if ((1u << sort_tile_shift(v)) != t || kPlainHistUnit % t != 0 || t < kSortMinTile) return false;
+  }
+  return true;
+}
+static_assert(sort_tiles_nest(), "tiles are powers of two dividing the histogram unit");
 constexpr int kLookbackWindow = 4;           // look-back words loaded per round trip
 
 // Compaction / scan tiles.
@@ -58,12 +73,17 @@ struct SortLaunch {
   uint64_t* lookback;      // [n_tiles][256]
   uint32_t* ctrl;          // kCtrlWords
   bool hist_ready;         // hist already counted (launch_morton_hist)
+  // [n_tiles][256]: pass 0 tile counts, for a single-cloud sort of 9+ bits
+  // (nullptr, or any other sort: pass 0 runs the look-back chain)
+  uint32_t* tile_counts;
 };
 void launch_sort(const SortLaunch& L, hipStream_t stream);
 // Stage 1 for one cloud of n points fused with the sort's histogram sweep
-// (the digit histograms of key_bits-bit keys go to hist's copies).
+// (the digit histograms of key_bits-bit keys go to hist's copies, and pass
+// 0's counts per tile of sort_variant to tile_counts unless it is nullptr).
 void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
-                        uint32_t key_bits, uint32_t* hist, hipStream_t stream);
+                        uint32_t key_bits, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
+                        hipStream_t stream);
 
 void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index,
                    float min_coord, float range, hipStream_t stream);
