@@ -90,7 +90,7 @@ struct rwh_workspace {
   uint32_t* hist = nullptr;
   uint32_t* digit_start = nullptr;
   uint64_t* lb_sort = nullptr;
-  uint32_t* tile_counts = nullptr;  // [max_sort_tiles][256]: pass 0 tile prefixes
+  uint32_t* tile_counts = nullptr;  // [256][tiles]: pass 0 tile prefixes (max_sort_tiles*256 words)
   uint64_t* lb_unique = nullptr;
   uint64_t* lb_scan = nullptr;
   uint32_t* ctrl = nullptr;    // 3 users x kCtrlWords

@@ -72,12 +72,18 @@ __device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c,
             a.prefix_n[base + c], row, mask);
 }
 
+// Steps a pull walk may take.  On a tree built from sorted unique codes a
+// walk visits at most two levels of count-0 nodes (a handful of steps); the
+// budget only ends walks over malformed trees handed in through the C ABI
+// (a cycle of left_child links), which would otherwise never end.
+constexpr int kPullBudget = 1024;
+
 // Pulls every chain reached from the given roots through nodes of count 0
 // (depth-first, one node per step).
 __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int* stack, int sp,
                                            int32_t (&row)[8], int32_t& mask) {
   constexpr int kStack = 16;
-  while (sp > 0) {
+  for (int steps = 0; sp > 0 && steps < kPullBudget; ++steps) {
     const int q = stack[--sp];
     if (a.edge_count[base + q] > 0) {
       add_link(a, base, q, row, mask);
@@ -257,7 +263,7 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
                                              int n, uint32_t pair, int32_t* row, uint32_t* stk) {
   int32_t m = 0;
   int sp = 0;
-  for (;;) {
+  for (int steps = 0; steps < kPullBudget; ++steps) {
     const int c = static_cast<int>(pair & 0x3fffffffu);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {

@@ -6,15 +6,21 @@
 // Onesweep scheme (Adinets & Merrill 2022) with decoupled look-back
 // (Merrill & Garland 2016), laid out for gfx950:
 //   * one sweep over the keys builds the digit histograms of every pass
-//     (16-byte loads, LDS histograms, spread global atomics);
+//     (16-byte loads, LDS histograms, spread global atomics); for a
+//     single-cloud sort it also counts pass 0's digits per tile;
 //   * a small kernel turns them into per-segment exclusive digit starts;
 //   * one launch per digit pass: a workgroup takes the next tile from an atomic
-//     counter, ranks its keys with 64-lane ballots (wave-level multisplit),
-//     publishes its digit counts, stages the tile in LDS in digit order while
-//     it looks back over its predecessors, then writes each digit run to its
-//     final place (coalesced along the run).
+//     counter, counts its digits and publishes them for its successors, looks
+//     back over its predecessors, ranks its keys (64-lane ballots: a stable
+//     wave-level multisplit), stages the tile in LDS in digit order, then
+//     writes each digit run to its final place (coalesced along the run);
+//   * pass 0 needs no stability (the keys carry no order yet), so it ranks
+//     with LDS atomics, and for a single cloud it takes its tile prefixes from
+//     a scan of the sweep's tile counts instead of the look-back.
 // Segments (clouds) are sorted independently in one launch: tiles never cross
 // a segment boundary and the look-back stops at the segment's first tile.
+#include <type_traits>
+
 #include "rw_device.hpp"
 #include "rw_internal.hpp"
 
@@ -27,13 +33,17 @@ namespace rw {
 // histogram, which spreads the global atomics over many cache lines.
 // ---------------------------------------------------------------------------
 // Pass 0's digit counts of the sort tiles a histogram unit covers (the unit
-// starts at a multiple of the tile size), from LDS to tile_counts[tile][256].
+// starts at a multiple of the tile size), from LDS to the digit-major table
+// tile_counts[digit][tile] of a sort of n keys.
 __device__ __forceinline__ void store_tile_counts(const uint32_t* s_tc, uint32_t first, uint32_t end,
-                                                  uint32_t tile_shift, uint32_t* tile_counts) {
+                                                  uint32_t n, uint32_t tile_shift, uint32_t* tile_counts) {
+  const uint32_t n_tiles = ((n - 1) >> tile_shift) + 1;
   const uint32_t t0 = first >> tile_shift;
   const uint32_t nt = ((end - first - 1) >> tile_shift) + 1;
-  for (uint32_t i = threadIdx.x; i < nt * 256; i += kHistBlock)
-    tile_counts[static_cast<size_t>(t0) * 256 + i] = s_tc[i];
+  for (uint32_t i = threadIdx.x; i < kTilesPerHistUnit * 256; i += kHistBlock) {
+    const uint32_t d = i / kTilesPerHistUnit, j = i % kTilesPerHistUnit;
+    if (j < nt) tile_counts[static_cast<size_t>(d) * n_tiles + t0 + j] = s_tc[j * 256 + d];
+  }
 }
 
 // Pass 0's unit histogram as the sum of its tiles' counts (tiles past the
@@ -111,7 +121,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
   }
   __syncthreads();
   if (per_tile) {
-    if (first < lim) store_tile_counts(s_tc, first, min(first + unit_keys, lim), tile_shift, tile_counts);
+    if (first < lim) store_tile_counts(s_tc, first, min(first + unit_keys, lim), lim, tile_shift, tile_counts);
     finish_pass0(s_tc, s_h);
   }
   uint32_t* dst = hist + (static_cast<size_t>(sg.id) * kHistCopies + (u % kHistCopies)) *
@@ -172,7 +182,7 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
   }
   __syncthreads();
   if (per_tile) {
-    if (first < end) store_tile_counts(s_tc, first, end, tile_shift, tile_counts);
+    if (first < end) store_tile_counts(s_tc, first, end, n, tile_shift, tile_counts);
     finish_pass0(s_tc, s_h);
   }
   uint32_t* dst = hist + (u % kHistCopies) * (kMaxPasses * 256);
@@ -196,28 +206,30 @@ void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float mi
 }
 
 // Pass 0's exclusive digit prefix of every tile of a single-cloud sort, in
-// place of the tile counts: one workgroup per digit scans that digit's column
-// (thread t sums a run of tiles, a block scan over the runs, then each thread
-// rewrites its run).
+// place of the tile counts: one workgroup per digit scans that digit's row of
+// the digit-major table, 1024 tiles per step (coalesced), carrying the total.
 __global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ tile_counts,
                                                     uint32_t n_tiles) {
   __shared__ uint32_t s_w[1024 / kWave];
-  const uint32_t d = blockIdx.x, t = threadIdx.x;
-  const uint32_t run = (n_tiles + 1023) / 1024;
-  const uint32_t t0 = t * run, t1 = min(t0 + run, n_tiles);
-  uint32_t sum = 0;
-  for (uint32_t k = t0; k < t1; ++k) sum += tile_counts[static_cast<size_t>(k) * 256 + d];
-  const uint32_t incl = wave_incl_scan(sum);
-  if (t % kWave == kWave - 1) s_w[t / kWave] = incl;
-  __syncthreads();
-  uint32_t off = 0;
-  for (uint32_t w = 0; w < t / kWave; ++w) off += s_w[w];
-  uint32_t acc = off + incl - sum;
-  for (uint32_t k = t0; k < t1; ++k) {
-    const size_t at = static_cast<size_t>(k) * 256 + d;
-    const uint32_t c = tile_counts[at];
-    tile_counts[at] = acc;
-    acc += c;
+  const uint32_t t = threadIdx.x;
+  uint32_t* row = tile_counts + static_cast<size_t>(blockIdx.x) * n_tiles;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n_tiles; base += 1024) {
+    const uint32_t k = base + t;
+    const uint32_t c = k < n_tiles ? row[k] : 0u;
+    const uint32_t incl = wave_incl_scan(c);
+    if (t % kWave == kWave - 1) s_w[t / kWave] = incl;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 1024 / kWave; ++w) {
+      const uint32_t x = s_w[w];
+      off += (w < t / kWave) ? x : 0u;
+      total += x;
+    }
+    if (k < n_tiles) row[k] = carry + off + incl - c;
+    carry += total;
+    __syncthreads();  // s_w is rewritten by the next step
   }
 }
 
@@ -337,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     // waiting on this tile are released early.
     uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles
     if (kTable) {
-      if (tid < 256) excl = tile_excl[static_cast<size_t>(tile) * 256 + tid];
+      if (tid < 256) excl = tile_excl[static_cast<size_t>(tid) * gridDim.x + tile];  // grid = tiles
     } else {
 #pragma unroll
       for (int k = 0; k < KPT; ++k)
@@ -445,27 +457,44 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
+// Calls f with the variant index as a compile-time constant.
+template <class F>
+void with_variant(int v, F&& f) {
+  static_assert(kNumSortVariants == 6, "one case per variant");
+  switch (v) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    default: f(std::integral_constant<int, 0>{}); break;
+  }
+}
+
+// Pass 0 of a single-cloud sort from the tile prefixes (no look-back): the
+// scan of the histogram sweep's tile counts, then the pass.
 template <int BLOCK, int KPT>
-void launch_passes(const SortLaunch& L, uint32_t n_passes, hipStream_t stream) {
+void launch_table_pass(const SortLaunch& L, uint32_t n_passes, uint32_t* dst, hipStream_t stream) {
+  constexpr uint32_t TILE = BLOCK * KPT;
+  const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
+  hipLaunchKernelGGL(k_tile_scan, dim3(256), dim3(1024), 0, stream, L.tile_counts, n_tiles);
+  hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, true>), dim3(n_tiles), dim3(BLOCK), 0, stream, L.keys_in,
+                     dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, L.tile_counts);
+}
+
+// Digit passes p_begin .. n_passes-1 on tiles of BLOCK x KPT.  The buffers
+// alternate in -> tmp -> out -> tmp -> out: an even number of passes ends in
+// out, an odd count starts with in -> out.
+template <int BLOCK, int KPT>
+void launch_passes(const SortLaunch& L, uint32_t n_passes, uint32_t p_begin, hipStream_t stream) {
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(TILE >= kSortMinTile, "look-back table sized for kSortMinTile");
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
-  // in -> tmp -> out -> tmp -> out (an even number of passes ends in out; an
-  // odd count starts with in -> out)
-  const uint32_t* src = L.keys_in;
-  for (uint32_t p = 0; p < n_passes; ++p) {
-    const bool to_out = ((n_passes - p) % 2) == 1;
-    uint32_t* dst = to_out ? L.keys_out : L.keys_tmp;
+  const auto dst_of = [&](uint32_t p) { return ((n_passes - p) % 2) == 1 ? L.keys_out : L.keys_tmp; };
+  const uint32_t* src = p_begin == 0 ? L.keys_in : dst_of(p_begin - 1);
+  for (uint32_t p = p_begin; p < n_passes; ++p) {
+    uint32_t* dst = dst_of(p);
     const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
-    if (p == 0 && L.tile_counts) {
-      // pass 0 (8 bits: n_passes >= 2) from the tile prefixes, no look-back
-      hipLaunchKernelGGL(k_tile_scan, dim3(256), dim3(1024), 0, stream, L.tile_counts, n_tiles);
-      hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, true>), dim3(n_tiles), dim3(BLOCK), 0, stream, src,
-                         dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl,
-                         L.tile_counts);
-      src = dst;
-      continue;
-    }
     if (p == 0 && bits == 8) {
       hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, false, true>), dim3(n_tiles), dim3(BLOCK), 0, stream,
                          src, dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, nullptr);
@@ -491,24 +520,23 @@ void launch_sort(const SortLaunch& L, hipStream_t stream) {
   const uint32_t n_passes = (L.key_bits + 7) / 8;
   const uint32_t last_bits = L.key_bits - 8 * (n_passes - 1);
   const int variant = (L.n_segs > 1 && !sort_tile_fits_chunk(L.variant)) ? kSortChunkVariant : L.variant;
-  SortLaunch M = L;
   // pass 0 from tile prefixes: one cloud, whole histogram units, 8-bit pass 0
-  if (!(L.n_segs == 1 && L.hist_unit_keys == kPlainHistUnit && n_passes >= 2)) M.tile_counts = nullptr;
+  const bool table = L.tile_counts && L.n_segs == 1 && L.hist_unit_keys == kPlainHistUnit && n_passes >= 2;
   if (!L.hist_ready)
     hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
                        L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist,
-                       sort_tile_shift(variant), M.tile_counts);
+                       sort_tile_shift(variant), table ? L.tile_counts : nullptr);
   hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                      L.digit_start, n_passes);
-  static_assert(kNumSortVariants == 6, "one case per variant");
-  switch (variant) {
-    case 1: launch_passes<kSortVariants[1].block, kSortVariants[1].kpt>(M, n_passes, stream); break;
-    case 2: launch_passes<kSortVariants[2].block, kSortVariants[2].kpt>(M, n_passes, stream); break;
-    case 3: launch_passes<kSortVariants[3].block, kSortVariants[3].kpt>(M, n_passes, stream); break;
-    case 4: launch_passes<kSortVariants[4].block, kSortVariants[4].kpt>(M, n_passes, stream); break;
-    case 5: launch_passes<kSortVariants[5].block, kSortVariants[5].kpt>(M, n_passes, stream); break;
-    default: launch_passes<kSortVariants[0].block, kSortVariants[0].kpt>(M, n_passes, stream); break;
+  if (table) {
+    with_variant(variant, [&](auto V) {
+      launch_table_pass<kSortVariants[V].block, kSortVariants[V].kpt>(
+          L, n_passes, ((n_passes % 2) == 1) ? L.keys_out : L.keys_tmp, stream);
+    });
   }
+  with_variant(variant, [&](auto V) {
+    launch_passes<kSortVariants[V].block, kSortVariants[V].kpt>(L, n_passes, table ? 1u : 0u, stream);
+  });
 }
 
 }  // namespace rw

@@ -39,10 +39,11 @@ void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_i
 }
 
 // ---------------------------------------------------------------------------
-// Tile-wide exclusive scan for kScanBlock threads.
+// Tile-wide exclusive scan for BLOCK threads.
 // ---------------------------------------------------------------------------
+template <int BLOCK>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
-  constexpr int W = kScanBlock / kWave;
+  constexpr int W = BLOCK / kWave;
   const uint32_t t = threadIdx.x, w = t / kWave;
   const uint32_t incl = wave_incl_scan(v);
   if ((t % kWave) == kWave - 1) s_w[w] = incl;
@@ -87,13 +88,15 @@ __device__ __forceinline__ uint32_t tile_prefix(uint32_t tile_total, uint32_t ti
 // Layout: wave w of a tile owns kScanKpt*64 consecutive keys, read as uint4
 // groups g (lane l holds keys 4*(g*64+l) .. +3 of the wave's range).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restrict__ in,
+template <int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_unique(const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, SegArgs seg,
                                                        uint32_t* __restrict__ n_unique,
                                                        uint64_t* __restrict__ lookback,
                                                        uint32_t* __restrict__ ctrl) {
-  constexpr int G = kScanKpt / 4;
-  __shared__ uint32_t s_w[kScanBlock / kWave];
+  constexpr int G = KPT / 4;
+  constexpr uint32_t TILE = BLOCK * KPT;
+  __shared__ uint32_t s_w[BLOCK / kWave];
   __shared__ uint32_t s_bcast;
   __shared__ Grab s_grab;
   const uint32_t tid = threadIdx.x;
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
   // the look-back spin ends whatever the residency of the grid.
   {
     const uint32_t tile = s_grab.index;
-    const uint32_t first = tile * kScanTile;
+    const uint32_t first = tile * TILE;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
     if (first >= lim) {
@@ -114,13 +117,13 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
       if (tid == 0 && first == sg.base) n_unique[sg.id] = 0;
       return;
     }
-    const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
-    const uint32_t wbase = first + wave * (kScanKpt * kWave);
+    const uint32_t tile_in_seg = (first - sg.base) / TILE;
+    const uint32_t wbase = first + wave * (KPT * kWave);
 
     uint32_t k[G][4];
     uint32_t flags[G];  // 4 head bits per group
     uint32_t cnt[G];
-    if (first + kScanTile <= lim) {  // full tile: plain 16-byte loads
+    if (first + TILE <= lim) {  // full tile: plain 16-byte loads
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const uint4 q = *reinterpret_cast<const uint4*>(in + wbase + 4 * (g * kWave + lane));
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
       wave_run += __shfl(incl, kWave - 1, kWave);
     }
     uint32_t tile_total;
-    const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
+    const uint32_t wexcl = block_excl_scan<BLOCK>(lane == 0 ? wave_run : 0u, s_w, &tile_total);
     const uint32_t wave_off = __shfl(wexcl, 0, kWave);
     const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
     const uint32_t base = sg.base + excl + wave_off;
@@ -173,16 +176,22 @@ __global__ __launch_bounds__(kScanBlock) void k_unique(const uint32_t* __restric
         if ((flags[g] >> c) & 1u) out[pos++] = k[g][c];
     }
     // the tile holding the segment's last key reports the count
-    if (tid == 0 && lim - first <= kScanTile) n_unique[sg.id] = excl + tile_total;
+    if (tid == 0 && lim - first <= TILE) n_unique[sg.id] = excl + tile_total;
   }
 }
 
 void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, uint32_t n_index,
                    uint32_t* n_unique, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
+  if (n_index == 0) return;
+  if (seg.n_segs == 1) {  // one cloud: the larger tile (fewer grabs and look-back steps)
+    const uint32_t n_tiles = (n_index + kScanBigTile - 1) / kScanBigTile;
+    hipLaunchKernelGGL((k_unique<kScanBigBlock, kScanBigKpt>), dim3(n_tiles), dim3(kScanBigBlock), 0, stream,
+                       sorted, unique_out, seg, n_unique, lookback, ctrl);
+    return;
+  }
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
-  if (n_tiles == 0) return;
-  hipLaunchKernelGGL(k_unique, dim3(n_tiles), dim3(kScanBlock), 0, stream, sorted, unique_out, seg,
-                     n_unique, lookback, ctrl);
+  hipLaunchKernelGGL((k_unique<kScanBlock, kScanKpt>), dim3(n_tiles), dim3(kScanBlock), 0, stream, sorted,
+                     unique_out, seg, n_unique, lookback, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,15 +452,16 @@ __device__ __forceinline__ uint32_t edge_count_at(const EdgeSource& es, const Se
                                static_cast<int>(es.prefix_n[sg.base + par] / 3));
 }
 
-template <bool kFromTree>
-__global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __restrict__ in,
+template <bool kFromTree, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict__ in,
                                                            int32_t* __restrict__ out, SegArgs seg,
                                                            uint32_t* __restrict__ total,
                                                            uint64_t* __restrict__ lookback,
                                                            uint32_t* __restrict__ ctrl,
                                                            EdgeSource es) {
-  constexpr int G = kScanKpt / 4;
-  __shared__ uint32_t s_w[kScanBlock / kWave];
+  constexpr int G = KPT / 4;
+  constexpr uint32_t TILE = BLOCK * KPT;
+  __shared__ uint32_t s_w[BLOCK / kWave];
   __shared__ uint32_t s_bcast;
   __shared__ Grab s_grab;
   const uint32_t tid = threadIdx.x;
@@ -464,18 +474,18 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
   // the look-back spin ends whatever the residency of the grid.
   {
     const uint32_t tile = s_grab.index;
-    const uint32_t first = tile * kScanTile;
+    const uint32_t first = tile * TILE;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
     if (first >= lim) {
       if (tid == 0 && first == sg.base) total[sg.id] = 0;
       return;
     }
-    const uint32_t tile_in_seg = (first - sg.base) / kScanTile;
-    const uint32_t wbase = first + wave * (kScanKpt * kWave);
+    const uint32_t tile_in_seg = (first - sg.base) / TILE;
+    const uint32_t wbase = first + wave * (KPT * kWave);
     uint32_t v[G][4];
     uint32_t gsum[G];
-    const bool full = first + kScanTile <= lim;
+    const bool full = first + TILE <= lim;
     if constexpr (kFromTree) {
       if (full) {  // 16-byte parent loads, 4-byte prefix loads, gathered parent prefixes
 #pragma unroll
@@ -530,7 +540,7 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
       wave_run += __shfl(incl, kWave - 1, kWave);
     }
     uint32_t tile_total;
-    const uint32_t wexcl = block_excl_scan(lane == 0 ? wave_run : 0u, s_w, &tile_total);
+    const uint32_t wexcl = block_excl_scan<BLOCK>(lane == 0 ? wave_run : 0u, s_w, &tile_total);
     const uint32_t wave_off = __shfl(wexcl, 0, kWave);
     const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
     const uint32_t base = excl + wave_off;
@@ -553,27 +563,38 @@ __global__ __launch_bounds__(kScanBlock) void k_prefix_sum(const int32_t* __rest
           if (i0 + c < lim) out[i0 + c] = static_cast<int>(o[c]);
       }
     }
-    if (tid == 0 && lim - first <= kScanTile) total[sg.id] = excl + tile_total;
+    if (tid == 0 && lim - first <= TILE) total[sg.id] = excl + tile_total;
   }
+}
+
+// Launches the scan of the live count of each segment on tiles of BLOCK x KPT
+// (one cloud: the larger tile).
+template <bool kFromTree>
+void launch_scan(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
+                 uint64_t* lookback, uint32_t* ctrl, EdgeSource es, hipStream_t stream) {
+  if (n_index == 0) return;
+  seg.count_sub = 1;
+  if (seg.n_segs == 1) {
+    const uint32_t n_tiles = (n_index + kScanBigTile - 1) / kScanBigTile;
+    hipLaunchKernelGGL((k_prefix_sum<kFromTree, kScanBigBlock, kScanBigKpt>), dim3(n_tiles), dim3(kScanBigBlock),
+                       0, stream, in, out, seg, total, lookback, ctrl, es);
+    return;
+  }
+  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((k_prefix_sum<kFromTree, kScanBlock, kScanKpt>), dim3(n_tiles), dim3(kScanBlock), 0,
+                     stream, in, out, seg, total, lookback, ctrl, es);
 }
 
 void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
                        uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
-  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
-  if (n_tiles == 0) return;
-  seg.count_sub = 1;
-  hipLaunchKernelGGL(k_prefix_sum<false>, dim3(n_tiles), dim3(kScanBlock), 0, stream, in, out, seg,
-                     total, lookback, ctrl, EdgeSource{});
+  launch_scan<false>(in, out, seg, n_index, total, lookback, ctrl, EdgeSource{}, stream);
 }
 
 void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
                       int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
                       uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
-  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
-  if (n_tiles == 0) return;
-  seg.count_sub = 1;
-  hipLaunchKernelGGL(k_prefix_sum<true>, dim3(n_tiles), dim3(kScanBlock), 0, stream, nullptr, out,
-                     seg, total, lookback, ctrl, EdgeSource{prefix_n, parent, edge_count});
+  launch_scan<true>(nullptr, out, seg, n_index, total, lookback, ctrl,
+                    EdgeSource{prefix_n, parent, edge_count}, stream);
 }
 
 }  // namespace rw

@@ -49,7 +49,13 @@ constexpr int kLookbackWindow = 4;           // look-back words loaded per round
 // Compaction / scan tiles.
 constexpr int kScanBlock = 512;
 constexpr int kScanKpt = 16;
-constexpr uint32_t kScanTile = kScanBlock * kScanKpt;  // 8192
+constexpr uint32_t kScanTile = kScanBlock * kScanKpt;  // 8192: a chunk, any cloud layout
+// A single cloud's compaction and scans run on a larger tile: fewer tiles mean
+// fewer serialised grabs and shorter look-back chains.
+constexpr int kScanBigBlock = 1024;
+constexpr int kScanBigKpt = 16;
+constexpr uint32_t kScanBigTile = kScanBigBlock * kScanBigKpt;  // 16384
+static_assert(kScanTile == kChunk, "segmented scans tile by chunk");
 
 // Elementwise tiles.
 constexpr int kElemBlock = 256;
@@ -73,7 +79,7 @@ struct SortLaunch {
   uint64_t* lookback;      // [n_tiles][256]
   uint32_t* ctrl;          // kCtrlWords
   bool hist_ready;         // hist already counted (launch_morton_hist)
-  // [n_tiles][256]: pass 0 tile counts, for a single-cloud sort of 9+ bits
+  // [256][n_tiles]: pass 0 tile counts, for a single-cloud sort of 9+ bits
   // (nullptr, or any other sort: pass 0 runs the look-back chain)
   uint32_t* tile_counts;
 };
