@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __restrict__ pts,
                                                             uint32_t* __restrict__ codes, uint32_t n,
-                                                            float min_coord, float range,
+                                                            Quant quant,
                                                             uint32_t unit_keys, uint32_t n_passes,
                                                             uint32_t last_bits,
                                                             uint32_t* __restrict__ hist,
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
     for (uint32_t r = 0; r < kDepth; ++r) {
       const uint32_t i = i0 + r * kHistBlock;
       if (i >= end) break;
-      const uint32_t k = morton_encode(q[r].x, q[r].y, q[r].z, min_coord, range);
+      const uint32_t k = morton_encode(q[r].x, q[r].y, q[r].z, quant);
       codes[i] = k;
 #pragma unroll
       for (uint32_t p = 0; p < kMaxPasses; ++p)
@@ -201,7 +201,7 @@ void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float mi
   const uint32_t units = (n + kPlainHistUnit - 1) / kPlainHistUnit;
   if (n_passes < 2) tile_counts = nullptr;
   hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, n,
-                     min_coord, range, kPlainHistUnit, n_passes, last_bits, hist,
+                     make_quant(min_coord, range), kPlainHistUnit, n_passes, last_bits, hist,
                      sort_tile_shift(sort_variant), tile_counts);
 }
 

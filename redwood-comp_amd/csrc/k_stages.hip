@@ -16,7 +16,7 @@ constexpr uint32_t kMortonTile = kElemBlock * kMortonPerThread;  // 1024
 
 __global__ __launch_bounds__(kElemBlock) void k_morton(const float4* __restrict__ pts,
                                                        uint32_t* __restrict__ codes, SegArgs seg,
-                                                       float min_coord, float range) {
+                                                       Quant q) {
   const uint32_t first = blockIdx.x * kMortonTile;
   const Seg sg = seg_of(seg, first);
   const uint32_t lim = sg.base + sg.count;
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kElemBlock) void k_morton(const float4* __restrict_
     const uint32_t i = first + j * kElemBlock + threadIdx.x;
     if (i < lim) {
       const float4 p = pts[i];
-      codes[i] = morton_encode(p.x, p.y, p.z, min_coord, range);
+      codes[i] = morton_encode(p.x, p.y, p.z, q);
     }
   }
 }
@@ -34,8 +34,8 @@ void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_i
                    float min_coord, float range, hipStream_t stream) {
   const uint32_t grid = (n_index + kMortonTile - 1) / kMortonTile;
   if (grid == 0) return;
-  hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kElemBlock), 0, stream, pts, codes, seg, min_coord,
-                     range);
+  hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kElemBlock), 0, stream, pts, codes, seg,
+                     make_quant(min_coord, range));
 }
 
 // ---------------------------------------------------------------------------

@@ -3,6 +3,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 #include <stdint.h>
 
 namespace rw {
@@ -225,20 +227,36 @@ __device__ __forceinline__ uint32_t compact3(uint32_t v) {
   return v;
 }
 
+// The box of one encode: (v - min) / range.  When range is a power of two
+// whose reciprocal is a normal float, inv_range holds that reciprocal and the
+// quotient is taken as the product with it: the same real number, so the same
+// rounded float as the IEEE division (a result in the subnormal range
+// quantises to 0 either way).  Otherwise inv_range is 0 and the encode divides.
+struct Quant {
+  float min_coord, range, inv_range;
+};
+
+inline Quant make_quant(float min_coord, float range) {
+  Quant q{min_coord, range, 0.0f};
+  int e = 0;
+  if (range > 0.0f && range <= 0x1p126f && range >= 0x1p-126f && std::frexp(range, &e) == 0.5f)
+    q.inv_range = std::ldexp(1.0f, 1 - e);  // range = 2^(e-1)
+  return q;
+}
+
 // u32((v - min) / range * 1024): IEEE sub, div, mul (no contraction), then a
 // saturating truncation (NaN and negatives -> 0, >= 2^32 -> 2^32-1).
-__device__ __forceinline__ uint32_t quantize(float v, float min_coord, float range) {
-  const float t = __fdiv_rn(__fsub_rn(v, min_coord), range);
+__device__ __forceinline__ uint32_t quantize(float v, const Quant& q) {
+  const float d = __fsub_rn(v, q.min_coord);
+  const float t = q.inv_range != 0.0f ? __fmul_rn(d, q.inv_range) : __fdiv_rn(d, q.range);
   const float s = __fmul_rn(t, 1024.0f);
   if (!(s > 0.0f)) return 0u;
   if (s >= 4294967296.0f) return 0xffffffffu;
   return static_cast<uint32_t>(s);
 }
 
-__device__ __forceinline__ uint32_t morton_encode(float x, float y, float z, float min_coord,
-                                                  float range) {
-  return spread3(quantize(x, min_coord, range)) | (spread3(quantize(y, min_coord, range)) << 1) |
-         (spread3(quantize(z, min_coord, range)) << 2);
+__device__ __forceinline__ uint32_t morton_encode(float x, float y, float z, const Quant& q) {
+  return spread3(quantize(x, q)) | (spread3(quantize(y, q)) << 1) | (spread3(quantize(z, q)) << 2);
 }
 
 // Corner of a cell: (float(d) / 1024) * range + min per axis, w = 1.

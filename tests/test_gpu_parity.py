@@ -70,6 +70,23 @@ def test_build_matches_oracle(gpu, orc, pool, n, kind):
     compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}")
 
 
+@pytest.mark.parametrize("mn,rg", [(0.0, 1024.0), (0.25, 0.5), (-64.0, 2.0 ** 20), (-3.7, 1000.0),
+                                   (1e-3, 3.0), (5.0, 2.0 ** -100)])
+def test_build_box_matches_oracle(gpu, orc, pool, mn, rg):
+    # power-of-two ranges take the product with the exact reciprocal, other
+    # ranges divide; both must give the reference's quotient bit for bit
+    rw = gpu
+    n = 100003
+    p = rw.uniform_points(n, 29)
+    p[:, :3] = np.float32(mn) + p[:, :3] / np.float32(1024.0) * np.float32(rg)
+    p[:64, :3] = np.float32(mn) - np.float32(abs(rg))   # below the box
+    p[64:128, :3] = np.float32(mn) + np.float32(2.0) * np.float32(rg)   # beyond it
+    print(f"case box mn={mn} rg={rg}", flush=True)
+    g_counts, g = rw.build_octree(p, min_coord=mn, range_=rg, strict=False)
+    (nu, nb, no, rc), o, _ = orc.build(pool, p, mn, rg)
+    compare(rw, n, g_counts, g, (nu, nb, no), o, f"box {mn} {rg}")
+
+
 @pytest.mark.parametrize("kind,n", [("uniform", 1 << 24), ("gmm", 1 << 24), ("gmm", 1 << 22)])
 def test_full_size_matches_oracle(gpu, orc, kind, n):
     rw = gpu

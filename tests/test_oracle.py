@@ -134,6 +134,26 @@ def test_morton_matches_magic_numbers(orc, pool):
     assert np.array_equal(orc.morton(pool, pts2, float(mn), float(rg)), morton_numpy(pts2, mn, rg))
 
 
+def test_pow2_range_product_equals_quotient():
+    # the encode multiplies by 1/range when range is a power of two with a
+    # normal reciprocal: the same real number, so the same IEEE float as the
+    # quotient, for every dividend (subnormal, huge, infinite, NaN included)
+    rng = np.random.default_rng(3)
+    bits_ = rng.integers(0, 1 << 32, size=400000, dtype=np.uint64).astype(np.uint32)
+    d = bits_.view(np.float32)
+    d = np.concatenate([d, np.float32([0.0, -0.0, np.inf, -np.inf, 1e-45, 3e-39, 1e38, 1023.999])])
+    with np.errstate(all="ignore"):
+        for e in (-126, -20, -3, -1, 0, 1, 3, 10, 20, 126):
+            r = np.float32(2.0 ** e)
+            inv = np.float32(2.0 ** -e)
+            q, m = d / r, d * inv
+            same = (q.view(np.uint32) == m.view(np.uint32)) | (np.isnan(q) & np.isnan(m))
+            assert same.all(), e
+        # and a non power of two does not have the property (so the encode divides)
+        r = np.float32(777.7)
+        assert ((d / r) != (d * (np.float32(1) / r)))[np.isfinite(d)].any()
+
+
 def test_morton_saturation_out_of_box(orc, pool):
     # below the box and NaN quantise to 0; far above saturates then keeps 10 bits (D10)
     pts = np.array([[-5, 0, 0, 0], [np.nan, 0, 0, 0], [1e12, 0, 0, 0]], np.float32)
