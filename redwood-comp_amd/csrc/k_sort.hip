@@ -19,6 +19,8 @@
 //     a scan of the sweep's tile counts instead of the look-back.
 // Segments (clouds) are sorted independently in one launch: tiles never cross
 // a segment boundary and the look-back stops at the segment's first tile.
+#include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "rw_device.hpp"
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg,
     const uint32_t* __restrict__ digit_start, uint32_t pass,
     uint32_t n_passes, uint32_t shift, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl,
-    const uint32_t* __restrict__ tile_excl) {
+    const uint32_t* __restrict__ tile_excl, uint32_t n_tiles) {
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
@@ -305,158 +307,202 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_count[256];  // the tile's digit counts (valid keys)
   __shared__ uint32_t s_tmp[4];
-  __shared__ Grab s_grab;
+  __shared__ Grab s_grab;  // the tile taken next
 
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave;
   const uint32_t lane = tid % kWave;
-  if (!kTable) {
-    if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
-    __syncthreads();
-  }
-  const uint32_t epoch = kTable ? 0u : s_grab.epoch;
-  // One tile per workgroup, in grab order: a tile only ever waits on tiles
-  // whose workgroups took their grab earlier and are therefore running, so
-  // the look-back spin ends whatever the residency of the grid.
-  {
-    const uint32_t tile = kTable ? blockIdx.x : s_grab.index;
-    const uint32_t first = tile * TILE;
-    const Seg sg = seg_of(seg, first);
-    const uint32_t lim = sg.base + sg.count;
-    if (first >= lim) return;  // a segment's padding (uniform across the workgroup)
-    const uint32_t n_valid = min(TILE, lim - first);
-    const uint32_t tile_in_seg = (first - sg.base) / TILE;
-    const uint32_t wbase = first + wave * (KPT * kWave) + lane;
 
-    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
-    if (tid < 256) s_count[tid] = 0;
-    uint32_t keys[KPT];
-    if (n_valid == TILE) {
-      const uint32_t* src = in + wbase;  // one base, immediate offsets per key
+  // A workgroup sorts tiles one after the other.  With kTable it walks
+  // blockIdx.x, + gridDim.x, ... (no tile waits on another).  Otherwise it
+  // takes each tile from the counter, the next one while the current one is
+  // staged, and its last grab finds none: a tile only ever waits on tiles
+  // taken earlier, by running workgroups that finish their earlier tiles
+  // first, so the look-back spin ends whatever the residency of the grid.
+  // The next tile's keys load while the current one is scattered.
+  const uint32_t n_grabs = n_tiles + gridDim.x;
+  if (!kTable && tid == 0) s_grab = grab_tiles(ctrl, n_grabs);
+  for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
+  if (tid < 256) s_count[tid] = 0;
+  __syncthreads();
+  const uint32_t epoch = kTable ? 0u : s_grab.epoch;
+  uint32_t tile = kTable ? blockIdx.x : s_grab.index;
+
+  uint32_t keys[KPT];
+  const auto load_keys = [&](uint32_t t) {
+    const uint32_t f = t * TILE;
+    const Seg g = seg_of(seg, f);
+    const uint32_t l = g.base + g.count;
+    if (f >= l) return;  // a segment's padding
+    const uint32_t wb = f + wave * (KPT * kWave) + lane;
+    if (f + TILE <= l) {
+      const uint32_t* src = in + wb;  // one base, immediate offsets per key
 #pragma unroll
       for (int k = 0; k < KPT; ++k) keys[k] = src[k * kWave];
     } else {
 #pragma unroll
       for (int k = 0; k < KPT; ++k) {
-        const uint32_t idx = wbase + k * kWave;
-        keys[k] = (idx < lim) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
+        const uint32_t idx = wb + k * kWave;
+        keys[k] = (idx < l) ? in[idx] : 0xffffffffu;  // padding ranks last in the top digit
       }
     }
-    __syncthreads();
+  };
+  if (tile < n_tiles) load_keys(tile);
 
-    // The tile's digit counts first (LDS atomics), so that its aggregate is
-    // published, and its own look-back done, before the ranking: successors
-    // waiting on this tile are released early.
-    uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles
-    if (kTable) {
-      if (tid < 256) excl = tile_excl[static_cast<size_t>(tid) * gridDim.x + tile];  // grid = tiles
-    } else {
-#pragma unroll
-      for (int k = 0; k < KPT; ++k)
-        if (n_valid == TILE || wbase + k * kWave < lim) atomicAdd(&s_count[(keys[k] >> shift) & dmask], 1u);
-      __syncthreads();
-    }
-    if (!kTable && tid < 256) {
-      const uint32_t d = tid;
-      const uint32_t cnt_valid = s_count[d];
-      uint64_t* slot = &lookback[static_cast<size_t>(tile) * 256 + d];
-      if (tile_in_seg == 0) {
-        lb_store(slot, lb_pack(epoch, kIncl, cnt_valid));
+  while (tile < n_tiles) {
+    const uint32_t first = tile * TILE;
+    const Seg sg = seg_of(seg, first);
+    const uint32_t lim = sg.base + sg.count;
+    const bool live = first < lim;  // else a segment's padding (uniform across the workgroup)
+    const uint32_t n_valid = live ? min(TILE, lim - first) : 0u;
+    const uint32_t wbase = first + wave * (KPT * kWave) + lane;
+    if (live) {
+      const uint32_t tile_in_seg = (first - sg.base) / TILE;
+      // The tile's digit counts first (LDS atomics), so that its aggregate
+      // is published, and its own look-back done, before the ranking:
+      // successors waiting on this tile are released early.
+      uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles
+      if (kTable) {
+        if (tid < 256) excl = tile_excl[static_cast<size_t>(tid) * n_tiles + tile];
       } else {
-        lb_store(slot, lb_pack(epoch, kAgg, cnt_valid));
-        excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
-        lb_store(slot, lb_pack(epoch, kIncl, excl + cnt_valid));
-      }
-    }
-
-    // Ranks within the wave, per digit counter of the wave.
-    uint32_t* wh = s_whist[wave];
-    uint32_t rank[KPT];
 #pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const uint32_t d = (keys[k] >> shift) & dmask;
-      if (kFirst) {
-        // padding keys take no rank: they are not staged
-        rank[k] = (n_valid == TILE || wbase + k * kWave < lim) ? atomicAdd(&wh[d], 1u) : 0u;
-        continue;
+        for (int k = 0; k < KPT; ++k)
+          if (n_valid == TILE || wbase + k * kWave < lim) atomicAdd(&s_count[(keys[k] >> shift) & dmask], 1u);
+        __syncthreads();
       }
-      // Stable multisplit: lanes holding the same digit find each other with
-      // BITS ballots (a lane differs from this one where some ballot differs
-      // from this lane's own bit); the lowest lane of each group advances the
-      // wave's counter for that digit.
-      uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, 0, 1));
-      uint64_t bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
-      uint32_t xl = own ^ static_cast<uint32_t>(bal);
-      uint32_t xh = own ^ static_cast<uint32_t>(bal >> 32);
-#pragma unroll
-      for (int b = 1; b < BITS; ++b) {
-        own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, b, 1));
-        bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
-        // truth tables indexed by (x << 2 | own << 1 | bal): 0xF6 is
-        // x | (own ^ bal); the last bit takes its complement 0x09, so that
-        // x ends as the peer mask
-        if (b + 1 < BITS) {
-          xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0xF6);
-          xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0xF6);
+      if (!kTable && tid < 256) {
+        const uint32_t d = tid;
+        const uint32_t cnt_valid = s_count[d];
+        uint64_t* slot = &lookback[static_cast<size_t>(tile) * 256 + d];
+        if (tile_in_seg == 0) {
+          lb_store(slot, lb_pack(epoch, kIncl, cnt_valid));
         } else {
-          xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0x09);
-          xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0x09);
+          lb_store(slot, lb_pack(epoch, kAgg, cnt_valid));
+          excl = lb_exclusive<kLookbackWindow>(lookback + d, 256, tile, tile_in_seg, epoch);
+          lb_store(slot, lb_pack(epoch, kIncl, excl + cnt_valid));
         }
       }
-      if (BITS == 1) {
-        xl = ~xl;
-        xh = ~xh;
-      }
-      const uint64_t m = (static_cast<uint64_t>(xh) << 32) | xl;  // peers, this lane included
-      const uint32_t below = popc_below(m);
-      const uint32_t old = wh[d];
-      if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
-      rank[k] = old + below;
-    }
-    __syncthreads();
 
-    // Per digit: the tile's digit starts, then each wave's start of the digit
-    // in the staged tile (stable ranks count padding keys, staged last and
-    // never written out; pass 0 does not count them).
-    uint32_t cnt = 0;
-    if (tid < 256) {
+      // Ranks within the wave, per digit counter of the wave.
+      uint32_t* wh = s_whist[wave];
+      uint32_t rank[KPT];
 #pragma unroll
-      for (int w = 0; w < WAVES; ++w) cnt += s_whist[w][tid];
-    }
-    uint32_t total;
-    const uint32_t tstart = scan256(cnt, s_tmp, &total);
-    if (tid < 256) {
-      uint32_t run = tstart;
+      for (int k = 0; k < KPT; ++k) {
+        const uint32_t d = (keys[k] >> shift) & dmask;
+        if (kFirst) {
+          // padding keys take no rank: they are not staged
+          rank[k] = (n_valid == TILE || wbase + k * kWave < lim) ? atomicAdd(&wh[d], 1u) : 0u;
+          continue;
+        }
+        // Stable multisplit: lanes holding the same digit find each other
+        // with BITS ballots (a lane differs from this one where some ballot
+        // differs from this lane's own bit); the lowest lane of each group
+        // advances the wave's counter for that digit.
+        uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, 0, 1));
+        uint64_t bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
+        uint32_t xl = own ^ static_cast<uint32_t>(bal);
+        uint32_t xh = own ^ static_cast<uint32_t>(bal >> 32);
 #pragma unroll
-      for (int w = 0; w < WAVES; ++w) {
-        const uint32_t t = s_whist[w][tid];
-        s_whist[w][tid] = run;
-        run += t;
+        for (int b = 1; b < BITS; ++b) {
+          own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, b, 1));
+          bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
+          // truth tables indexed by (x << 2 | own << 1 | bal): 0xF6 is
+          // x | (own ^ bal); the last bit takes its complement 0x09, so that
+          // x ends as the peer mask
+          if (b + 1 < BITS) {
+            xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0xF6);
+            xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0xF6);
+          } else {
+            xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0x09);
+            xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0x09);
+          }
+        }
+        if (BITS == 1) {
+          xl = ~xl;
+          xh = ~xh;
+        }
+        const uint64_t m = (static_cast<uint64_t>(xh) << 32) | xl;  // peers, this lane included
+        const uint32_t below = popc_below(m);
+        const uint32_t old = wh[d];
+        if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
+        rank[k] = old + below;
       }
-      s_gbase[tid] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + tid] +
-                     excl - tstart;
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // Stage the tile in digit order.
+      // Per digit: the tile's digit starts, then each wave's start of the
+      // digit in the staged tile (stable ranks count padding keys, staged
+      // last and never written out; pass 0 does not count them).
+      uint32_t cnt = 0;
+      if (tid < 256) {
 #pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const uint32_t d = (keys[k] >> shift) & dmask;
-      if (!kFirst || n_valid == TILE || wbase + k * kWave < lim) s_stage[wh[d] + rank[k]] = keys[k];
+        for (int w = 0; w < WAVES; ++w) cnt += s_whist[w][tid];
+      }
+      uint32_t total;
+      const uint32_t tstart = scan256(cnt, s_tmp, &total);
+      if (tid < 256) {
+        uint32_t run = tstart;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+          const uint32_t t = s_whist[w][tid];
+          s_whist[w][tid] = run;
+          run += t;
+        }
+        s_gbase[tid] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + tid] +
+                       excl - tstart;
+      }
+      __syncthreads();
+
+      // Stage the tile in digit order.
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) {
+        const uint32_t d = (keys[k] >> shift) & dmask;
+        if (!kFirst || n_valid == TILE || wbase + k * kWave < lim) s_stage[wh[d] + rank[k]] = keys[k];
+      }
     }
+    // The next tile: taken now, its keys loading during the scatter.
+    if (!kTable && tid == 0) s_grab = grab_tiles(ctrl, n_grabs);
     __syncthreads();
+    const uint32_t next = kTable ? tile + gridDim.x : s_grab.index;
+    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
+    if (tid < 256) s_count[tid] = 0;
+    if (next < n_tiles) load_keys(next);
 
     // Scatter: consecutive staged keys of one digit go to consecutive addresses.
     for (uint32_t p = tid; p < n_valid; p += BLOCK) {
       const uint32_t key = s_stage[p];
       out[s_gbase[(key >> shift) & dmask] + p] = key;
     }
+    __syncthreads();  // the staged tile and the counters are reused
+    tile = next;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
+// Workgroups of a kernel that fit on the device at once (at most n_tiles):
+// the sort passes loop over their tiles, so a larger grid would only queue.
+// Cached per kernel and device (a benign race computes the same value
+// twice); any grid is correct, this one is the fastest.
+template <auto Kernel>
+uint32_t resident_grid(int block, uint32_t n_tiles) {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<uint32_t> cache[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  uint32_t g = cache[dev].load(std::memory_order_relaxed);
+  if (g == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, block, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 1;
+    g = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(cus);
+    cache[dev].store(g, std::memory_order_relaxed);
+  }
+  return std::max<uint32_t>(1, std::min(g, n_tiles));
+}
+
 // Calls f with the variant index as a compile-time constant.
 template <class F>
 void with_variant(int v, F&& f) {
@@ -477,9 +523,10 @@ template <int BLOCK, int KPT>
 void launch_table_pass(const SortLaunch& L, uint32_t n_passes, uint32_t* dst, hipStream_t stream) {
   constexpr uint32_t TILE = BLOCK * KPT;
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
+  constexpr auto kern = k_onesweep<BLOCK, KPT, 8, true>;
   hipLaunchKernelGGL(k_tile_scan, dim3(256), dim3(1024), 0, stream, L.tile_counts, n_tiles);
-  hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, true>), dim3(n_tiles), dim3(BLOCK), 0, stream, L.keys_in,
-                     dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, L.tile_counts);
+  hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, L.keys_in,
+                     dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, L.tile_counts, n_tiles);
 }
 
 // Digit passes p_begin .. n_passes-1 on tiles of BLOCK x KPT.  The buffers
@@ -496,17 +543,19 @@ void launch_passes(const SortLaunch& L, uint32_t n_passes, uint32_t p_begin, hip
     uint32_t* dst = dst_of(p);
     const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
     if (p == 0 && bits == 8) {
-      hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, 8, false, true>), dim3(n_tiles), dim3(BLOCK), 0, stream,
-                         src, dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, nullptr);
+      constexpr auto kern = k_onesweep<BLOCK, KPT, 8, false, true>;
+      hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, src, dst,
+                         L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, nullptr, n_tiles);
       src = dst;
       continue;
     }
-#define RW_ONESWEEP(B)                                                                       \
-  case B:                                                                                    \
-    hipLaunchKernelGGL((k_onesweep<BLOCK, KPT, B>), dim3(n_tiles), dim3(BLOCK), 0, stream, src, \
-                       dst, L.seg, L.digit_start, p, n_passes, 8 * p, L.lookback, L.ctrl,   \
-                       nullptr);                                                             \
-    break;
+#define RW_ONESWEEP(B)                                                                          \
+  case B: {                                                                                     \
+    constexpr auto kern = k_onesweep<BLOCK, KPT, B>;                                            \
+    hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, \
+                       src, dst, L.seg, L.digit_start, p, n_passes, 8 * p, L.lookback, L.ctrl,  \
+                       nullptr, n_tiles);                                                       \
+  } break;
     switch (bits) {
       RW_ONESWEEP(1) RW_ONESWEEP(2) RW_ONESWEEP(3) RW_ONESWEEP(4)
       RW_ONESWEEP(5) RW_ONESWEEP(6) RW_ONESWEEP(7) default: RW_ONESWEEP(8)
