@@ -301,15 +301,23 @@ __device__ void ref_tile(const TreeArgs& a, const uint32_t* codes, uint32_t base
 // becoming one flat load.
 struct CodeWindow {
   const uint32_t* g;  // segment's codes
-  const uint32_t* s;  // LDS copy of g[lo, hi)
-  int lo, hi;
-  __device__ __forceinline__ uint32_t operator[](int k) const {
-    uint32_t v = s[min(max(k - lo, 0), hi - lo - 1)];
+  const uint32_t* s;  // LDS copy of g[lo, lo + size)
+  int lo;
+  uint32_t size;
+  // code k of the segment
+  __device__ __forceinline__ uint32_t operator[](int k) const { return rel(static_cast<uint32_t>(k - lo)); }
+  // code lo + u (an index below lo wraps above size)
+  __device__ __forceinline__ uint32_t rel(uint32_t u) const {
+    uint32_t v = s[min(u, size - 1)];
     asm volatile("" : "+v"(v));
-    if (k < lo || k >= hi) v = g[k];
+    if (u >= size) v = g[static_cast<int>(u) + lo];
     return v;
   }
 };
+
+// x * d for a direction d of +1 or -1 given as m = (d < 0 ? -1 : 0): two
+// full-rate bit operations instead of a (quarter-rate) 64-bit multiply-add.
+__device__ __forceinline__ int signed_step(int x, int m) { return (x ^ m) - m; }
 
 constexpr int kTreeHalo = 384;  // codes cached on each side of the workgroup's nodes
 
@@ -325,15 +333,18 @@ __device__ __forceinline__ uint32_t agree_below(int t) {
 __device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, int d, int l,
                                          uint32_t thr, uint32_t ci) {
   int s = 0;
+  const int m = d < 0 ? -1 : 0;
+  const uint32_t wi = static_cast<uint32_t>(i - codes.lo);  // i in the window's frame
   const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
   const int scut = (d == -1) ? i - 1 : n - i - 1;
   uint32_t t = (static_cast<uint32_t>(l) + 1) / 2;
   for (uint32_t div = 2; div <= max_div; div <<= 1) {
-    if (s + static_cast<int>(t) <= scut && (ci ^ codes[i + (s + static_cast<int>(t)) * d]) < thr)
+    if (s + static_cast<int>(t) <= scut &&
+        (ci ^ codes.rel(wi + signed_step(s + static_cast<int>(t), m))) < thr)
       s += static_cast<int>(t);
     t = (static_cast<uint32_t>(l) + (div << 1) - 1) / (div << 1);
   }
-  return i + s * d + min(d, 0);
+  return i + signed_step(s, m) + min(d, 0);
 }
 
 // One thread per node i.  With keys below 2^31 (the largest is the last)
@@ -351,9 +362,9 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     ref_tile(a, gcodes, sg.base, b0, n);
     return;
   }
-  CodeWindow codes{gcodes, s_codes, max(b0 - kTreeHalo, 0), min(b0 + kElemBlock + kTreeHalo, n + 1)};
-  for (int k = codes.lo + static_cast<int>(threadIdx.x); k < codes.hi; k += kElemBlock)
-    s_codes[k - codes.lo] = gcodes[k];
+  const int wlo = max(b0 - kTreeHalo, 0), whi = min(b0 + kElemBlock + kTreeHalo, n + 1);
+  CodeWindow codes{gcodes, s_codes, wlo, static_cast<uint32_t>(whi - wlo)};
+  for (int k = wlo + static_cast<int>(threadIdx.x); k < whi; k += kElemBlock) s_codes[k - wlo] = gcodes[k];
   __syncthreads();
   const int i = static_cast<int>(gi - sg.base);
   if (i >= n) return;
@@ -368,14 +379,17 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     // sign(delta(i, i+1) - delta(i, i-1)) = sign of the clz difference
     const int cp = __clz(ci ^ codes[i + 1]), cm = __clz(ci ^ codes[i - 1]);
     d = (cp > cm) - (cp < cm);
+    const int m = d < 0 ? -1 : 0;
+    const uint32_t wi = static_cast<uint32_t>(i - codes.lo);  // i in the window's frame
     const uint32_t thr = agree_below(delta8(ci, codes[i - d]));
-    int lmax = 2;
-    while (i + lmax * d >= 0 && i + lmax * d <= n && (ci ^ codes[i + lmax * d]) < thr) lmax *= 2;
+    // the probe i + lmax*d is inside [0, n] while lmax <= cutoff
     const int cutoff = (d == -1) ? i : n - i;
+    int lmax = 2;
+    while (lmax <= cutoff && (ci ^ codes.rel(wi + signed_step(lmax, m))) < thr) lmax *= 2;
     for (int t = lmax / 2; t >= 1; t /= 2)
-      if (l + t <= cutoff && (ci ^ codes[i + (l + t) * d]) < thr) l += t;
+      if (l + t <= cutoff && (ci ^ codes.rel(wi + signed_step(l + t, m))) < thr) l += t;
   }
-  const int j = i + l * d;
+  const int j = i + (d < 0 ? -l : l);
   const int dnode = delta8(ci, codes[j]);
   const int gamma = brt_split(codes, i, n, d, l, agree_below(dnode), ci);
   const bool ll = min(i, j) == gamma;
