@@ -382,12 +382,20 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     const int m = d < 0 ? -1 : 0;
     const uint32_t wi = static_cast<uint32_t>(i - codes.lo);  // i in the window's frame
     const uint32_t thr = agree_below(delta8(ci, codes[i - d]));
-    // the probe i + lmax*d is inside [0, n] while lmax <= cutoff
+    // The range end: l is the largest x <= cutoff with delta(i, i + x*d) >
+    // delta_min, and that predicate only turns false as x grows, so any
+    // bound lmax where it is false (or past cutoff) leads the binary search
+    // to the same l as the reference's doubling.  The bound grows 4-fold,
+    // which shortens the wave's longest search (its largest node's).
+    // The probe i + lmax*d is inside [0, n] while lmax <= cutoff.
     const int cutoff = (d == -1) ? i : n - i;
-    int lmax = 2;
-    while (lmax <= cutoff && (ci ^ codes.rel(wi + signed_step(lmax, m))) < thr) lmax *= 2;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-      if (l + t <= cutoff && (ci ^ codes.rel(wi + signed_step(l + t, m))) < thr) l += t;
+    uint32_t lmax = 4;
+    while (lmax <= static_cast<uint32_t>(cutoff) &&
+           (ci ^ codes.rel(wi + signed_step(static_cast<int>(lmax), m))) < thr)
+      lmax = lmax < (1u << 29) ? lmax << 2 : (1u << 31);
+    for (uint32_t t = lmax / 2; t >= 1; t /= 2)
+      if (l + static_cast<int>(t) <= cutoff && (ci ^ codes.rel(wi + signed_step(l + static_cast<int>(t), m))) < thr)
+        l += static_cast<int>(t);
   }
   const int j = i + (d < 0 ? -l : l);
   const int dnode = delta8(ci, codes[j]);
