@@ -208,13 +208,13 @@ void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float mi
 }
 
 // Pass 0's exclusive digit prefix of every tile of a single-cloud sort, in
-// place of the tile counts: one workgroup per digit scans that digit's row of
-// the digit-major table, 1024 tiles per step (coalesced), carrying the total.
-__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ tile_counts,
-                                                    uint32_t n_tiles) {
-  __shared__ uint32_t s_w[1024 / kWave];
+// place of the tile counts: one workgroup of 1024 threads scans one digit's
+// row of the digit-major table, 1024 tiles per step (coalesced), carrying
+// the total.  s_w holds 16 words.
+__device__ __forceinline__ void tile_scan_row(uint32_t* tile_counts, uint32_t n_tiles, uint32_t digit,
+                                              uint32_t* s_w) {
   const uint32_t t = threadIdx.x;
-  uint32_t* row = tile_counts + static_cast<size_t>(blockIdx.x) * n_tiles;
+  uint32_t* row = tile_counts + static_cast<size_t>(digit) * n_tiles;
   uint32_t carry = 0;
   for (uint32_t base = 0; base < n_tiles; base += 1024) {
     const uint32_t k = base + t;
@@ -256,23 +256,48 @@ __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t* s_tmp, uint32_
   return off + incl - v;
 }
 
-// Digit starts: one workgroup per (segment, pass).  Sums the histogram copies,
-// clears them for the next sort, and writes the exclusive scan over digits.
+// Digit starts of segment s, pass p: sums the histogram copies, clears them
+// for the next sort, and writes the exclusive scan over digits (thread d <
+// 256 owns digit d; every thread of the block calls it).
+__device__ __forceinline__ void digit_starts(uint32_t* hist, uint32_t* digit_start, uint32_t n_passes,
+                                             uint32_t s, uint32_t p, uint32_t* s_tmp) {
+  const uint32_t d = threadIdx.x;
+  uint32_t c = 0;
+  if (d < 256) {
+#pragma unroll
+    for (uint32_t k = 0; k < kHistCopies; ++k) {
+      uint32_t* h = hist + (static_cast<size_t>(s) * kHistCopies + k) * (kMaxPasses * 256) + p * 256 + d;
+      c += *h;
+      *h = 0;
+    }
+  }
+  uint32_t total;
+  const uint32_t ex = scan256(c, s_tmp, &total);
+  if (d < 256) digit_start[(static_cast<size_t>(s) * n_passes + p) * 256 + d] = ex;
+}
+
+// Digit starts: one workgroup per (segment, pass).
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
                                                     uint32_t* __restrict__ digit_start,
                                                     uint32_t n_passes) {
   __shared__ uint32_t s_tmp[4];
-  const uint32_t s = blockIdx.x, p = blockIdx.y, d = threadIdx.x;
-  uint32_t c = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < kHistCopies; ++k) {
-    uint32_t* h = hist + (static_cast<size_t>(s) * kHistCopies + k) * (kMaxPasses * 256) + p * 256 + d;
-    c += *h;
-    *h = 0;
+  digit_starts(hist, digit_start, n_passes, blockIdx.x, blockIdx.y, s_tmp);
+}
+
+// A single-cloud sort's two small scans in one launch: workgroups 0..255
+// each scan one digit's row of the pass-0 tile table, the next n_passes
+// workgroups compute the digit starts of one pass each.
+__global__ __launch_bounds__(1024) void k_start_scans(uint32_t* __restrict__ hist,
+                                                      uint32_t* __restrict__ digit_start,
+                                                      uint32_t n_passes,
+                                                      uint32_t* __restrict__ tile_counts,
+                                                      uint32_t n_tiles) {
+  __shared__ uint32_t s_tmp[1024 / kWave];
+  if (blockIdx.x < 256) {
+    tile_scan_row(tile_counts, n_tiles, blockIdx.x, s_tmp);
+  } else {
+    digit_starts(hist, digit_start, n_passes, 0, blockIdx.x - 256, s_tmp);
   }
-  uint32_t total;
-  const uint32_t ex = scan256(c, s_tmp, &total);
-  digit_start[(static_cast<size_t>(s) * n_passes + p) * 256 + d] = ex;
 }
 
 // ---------------------------------------------------------------------------
@@ -524,7 +549,6 @@ void launch_table_pass(const SortLaunch& L, uint32_t n_passes, uint32_t* dst, hi
   constexpr uint32_t TILE = BLOCK * KPT;
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
   constexpr auto kern = k_onesweep<BLOCK, KPT, 8, true>;
-  hipLaunchKernelGGL(k_tile_scan, dim3(256), dim3(1024), 0, stream, L.tile_counts, n_tiles);
   hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, L.keys_in,
                      dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, L.tile_counts, n_tiles);
 }
@@ -575,8 +599,14 @@ void launch_sort(const SortLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
                        L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist,
                        sort_tile_shift(variant), table ? L.tile_counts : nullptr);
-  hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
-                     L.digit_start, n_passes);
+  if (table) {
+    const uint32_t n_tiles = (L.n_index + (1u << sort_tile_shift(variant)) - 1) >> sort_tile_shift(variant);
+    hipLaunchKernelGGL(k_start_scans, dim3(256 + n_passes), dim3(1024), 0, stream, L.hist, L.digit_start,
+                       n_passes, L.tile_counts, n_tiles);
+  } else {
+    hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
+                       L.digit_start, n_passes);
+  }
   if (table) {
     with_variant(variant, [&](auto V) {
       launch_table_pass<kSortVariants[V].block, kSortVariants[V].kpt>(

@@ -381,7 +381,8 @@ __global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs s
     d = (cp > cm) - (cp < cm);
     const int m = d < 0 ? -1 : 0;
     const uint32_t wi = static_cast<uint32_t>(i - codes.lo);  // i in the window's frame
-    const uint32_t thr = agree_below(delta8(ci, codes[i - d]));
+    // delta(i, i - d) is the other neighbour's: clz - 1 (no uint8 wrap below 2^31)
+    const uint32_t thr = agree_below((d > 0 ? cm : cp) - 1);
     // The range end: l is the largest x <= cutoff with delta(i, i + x*d) >
     // delta_min, and that predicate only turns false as x grows, so any
     // bound lmax where it is false (or past cutoff) leads the binary search
