@@ -135,6 +135,45 @@ def test_sort_every_shape(gpu, n):
         assert np.array_equal(d_out.numpy(), np.sort(k)), v
 
 
+def test_sort_every_shape_full_size(gpu):
+    """At 2^24 keys every tile shape has more tiles than resident workgroups,
+    so each workgroup of a pass sorts several tiles in turn."""
+    rw = gpu
+    L = rw.lib()
+    n = 1 << 24
+    rng = np.random.default_rng(24)
+    k = rng.integers(0, 1 << 30, size=n, dtype=np.uint64).astype(np.uint32)
+    want = np.sort(k)
+    d_in = rw.DeviceArray.from_numpy(k)
+    d_out = rw.DeviceArray(n, np.uint32)
+    ws = rw.Workspace(n)
+    for v in range(L.rwh_sort_variant_count()):
+        print(f"case sort shape {v} n={n}", flush=True)
+        ws.set_sort_variant(v)
+        rw.check(L.rwh_sort_u32(ws.ptr, d_in.ptr, d_out.ptr, n, 30, None))
+        assert np.array_equal(d_out.numpy(), want), v
+
+
+def test_batch_sorts_many_tiles(gpu, orc, pool):
+    """A batch large enough that its sort, compaction and scan launches loop
+    over tiles: every cloud's sorted and unique codes equal the oracle's."""
+    rw = gpu
+    rng = np.random.default_rng(12)
+    sizes = rng.integers(20000, 120000, size=150)
+    clouds = [rw.uniform_points(int(n), 500 + s) if s % 3 else rw.gmm_points(int(n), 500 + s, 16, 10.24)
+              for s, n in enumerate(sizes)]
+    print(f"case batch many clouds={len(clouds)} points={int(sizes.sum())}", flush=True)
+    b = rw.Batch(clouds)
+    b.run()
+    res = b.read()
+    for s, c in enumerate(clouds):
+        codes = np.sort(orc.morton(pool, c))
+        u = np.unique(codes)
+        assert np.array_equal(res[s]["morton_keys_alt"], codes), s
+        assert res[s]["n_unique"] == u.size, s
+        assert np.array_equal(res[s]["unique"], u), s
+
+
 @pytest.mark.parametrize("n", [0, 1, 2, 4096, 4097, 100000])
 def test_unique_u32(gpu, n):
     rw = gpu
