@@ -19,8 +19,6 @@
 //     a scan of the sweep's tile counts instead of the look-back.
 // Segments (clouds) are sorted independently in one launch: tiles never cross
 // a segment boundary and the look-back stops at the segment's first tile.
-#include <algorithm>
-#include <atomic>
 #include <type_traits>
 
 #include "rw_device.hpp"
@@ -505,29 +503,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-// Workgroups of a kernel that fit on the device at once (at most n_tiles):
-// the sort passes loop over their tiles, so a larger grid would only queue.
-// Cached per kernel and device (a benign race computes the same value
-// twice); any grid is correct, this one is the fastest.
-template <auto Kernel>
-uint32_t resident_grid(int block, uint32_t n_tiles) {
-  constexpr int kMaxDevices = 64;
-  static std::atomic<uint32_t> cache[kMaxDevices];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
-  uint32_t g = cache[dev].load(std::memory_order_relaxed);
-  if (g == 0) {
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, block, 0) != hipSuccess || per_cu < 1)
-      per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 1;
-    g = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(cus);
-    cache[dev].store(g, std::memory_order_relaxed);
-  }
-  return std::max<uint32_t>(1, std::min(g, n_tiles));
-}
-
 // Calls f with the variant index as a compile-time constant.
 template <class F>
 void with_variant(int v, F&& f) {

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <atomic>
+
 #include "rw_device.hpp"
 
 namespace rw {
@@ -59,6 +62,30 @@ static_assert(kScanTile == kChunk, "segmented scans tile by chunk");
 
 // Elementwise tiles.
 constexpr int kElemBlock = 256;
+
+// Workgroups of a kernel that fit on the device at once (at most n_tiles):
+// the look-back kernels loop over their tiles, so a larger grid would only
+// queue.
+// Cached per kernel and device (a benign race computes the same value
+// twice); any grid is correct, this one is the fastest.
+template <auto Kernel>
+uint32_t resident_grid(int block, uint32_t n_tiles) {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<uint32_t> cache[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  uint32_t g = cache[dev].load(std::memory_order_relaxed);
+  if (g == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, block, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 1;
+    g = static_cast<uint32_t>(per_cu) * static_cast<uint32_t>(cus);
+    cache[dev].store(g, std::memory_order_relaxed);
+  }
+  return std::max<uint32_t>(1, std::min(g, n_tiles));
+}
 
 // Grid of n_tiles workgroups padded for xcd_tile().
 inline uint32_t xcd_grid(uint32_t n_tiles) { return (n_tiles + kXcds - 1) / kXcds * kXcds; }
