@@ -32,6 +32,21 @@ namespace rw {
 // the unit's counts are added to copy (u % kHistCopies) of its segment's
 // histogram, which spreads the global atomics over many cache lines.
 // ---------------------------------------------------------------------------
+// Adds one to h[d] for every active lane.  When all active lanes hold the
+// same digit (sorted or clustered keys: consecutive keys share their high
+// digits) one lane adds their number instead: 64 atomics on one LDS address
+// would otherwise be serialised.
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d) {
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+  if (ballot(d != d0) == 0) {
+    const uint64_t act = ballot(true);
+    if (lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1))
+      atomicAdd(&h[d0], static_cast<uint32_t>(__popcll(act)));
+  } else {
+    atomicAdd(&h[d], 1u);
+  }
+}
+
 // Pass 0's digit counts of the sort tiles a histogram unit covers (the unit
 // starts at a multiple of the tile size), from LDS to the digit-major table
 // tile_counts[digit][tile] of a sort of n keys.
@@ -107,7 +122,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 #pragma unroll
           for (uint32_t p = 0; p < kMaxPasses; ++p)
             if (p < n_passes && (p > 0 || !per_tile))
-              atomicAdd(&s_h[p * 256 + ((k[c] >> (8 * p)) & mask[p])], 1u);
+              hist_add(s_h + p * 256, (k[c] >> (8 * p)) & mask[p]);
           if (per_tile) atomicAdd(&tc[k[c] & mask[0]], 1u);
         }
       }
@@ -115,7 +130,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
       const uint32_t k = keys[i];
       for (uint32_t p = per_tile ? 1 : 0; p < n_passes; ++p)
-        atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+        hist_add(s_h + p * 256, (k >> (8 * p)) & mask[p]);
       if (per_tile) atomicAdd(&s_tc[((i - first) >> tile_shift) * 256 + (k & mask[0])], 1u);
     }
   }
@@ -176,7 +191,7 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
 #pragma unroll
       for (uint32_t p = 0; p < kMaxPasses; ++p)
         if (p < n_passes && (p > 0 || !per_tile))
-          atomicAdd(&s_h[p * 256 + ((k >> (8 * p)) & mask[p])], 1u);
+          hist_add(s_h + p * 256, (k >> (8 * p)) & mask[p]);
       if (per_tile) atomicAdd(&s_tc[((i - first) >> tile_shift) * 256 + (k & mask[0])], 1u);
     }
   }

@@ -154,6 +154,40 @@ def test_sort_every_shape_full_size(gpu):
         assert np.array_equal(d_out.numpy(), want), v
 
 
+@pytest.mark.parametrize("order", ["sorted", "reversed", "runs"])
+def test_sort_ordered_keys(gpu, order):
+    """Already ordered keys: whole waves share a digit in the histogram sweep."""
+    rw = gpu
+    L = rw.lib()
+    n = (1 << 22) + 77
+    rng = np.random.default_rng(3)
+    k = np.sort(rng.integers(0, 1 << 30, size=n, dtype=np.uint64).astype(np.uint32))
+    if order == "reversed":
+        k = k[::-1].copy()
+    elif order == "runs":
+        k = np.repeat(k[: n // 1000], 1000)[:n]
+        k = np.concatenate([k, np.zeros(n - k.size, np.uint32)])
+    d_in = rw.DeviceArray.from_numpy(k)
+    d_out = rw.DeviceArray(n, np.uint32)
+    ws = rw.Workspace(n)
+    print(f"case sort ordered {order} n={n}", flush=True)
+    rw.check(L.rwh_sort_u32(ws.ptr, d_in.ptr, d_out.ptr, n, 30, None))
+    assert np.array_equal(d_out.numpy(), np.sort(k))
+
+
+def test_build_morton_ordered_points(gpu, orc, pool):
+    """Points already in Morton order (codes arrive sorted at the sweep)."""
+    rw = gpu
+    n = 1 << 21
+    p = rw.uniform_points(n, 41)
+    codes = orc.morton(pool, p)
+    p = np.ascontiguousarray(p[np.argsort(codes, kind="stable")])
+    print(f"case build morton-ordered n={n}", flush=True)
+    g_counts, g = rw.build_octree(p, strict=False)
+    (nu, nb, no, rc), o, _ = orc.build(pool, p)
+    compare(rw, n, g_counts, g, (nu, nb, no), o, "morton-ordered")
+
+
 def test_batch_sorts_many_tiles(gpu, orc, pool):
     """A batch large enough that its sort, compaction and scan launches loop
     over tiles: every cloud's sorted and unique codes equal the oracle's."""
