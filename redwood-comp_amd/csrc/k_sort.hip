@@ -32,19 +32,39 @@ namespace rw {
 // the unit's counts are added to copy (u % kHistCopies) of its segment's
 // histogram, which spreads the global atomics over many cache lines.
 // ---------------------------------------------------------------------------
-// Adds one to h[d] for every active lane.  When all active lanes hold the
-// same digit (sorted or clustered keys: consecutive keys share their high
+// True when every active lane holds the same x (wave-uniform).
+__device__ __forceinline__ bool wave_same(uint32_t x) {
+  return ballot(x != __builtin_amdgcn_readfirstlane(x)) == 0;
+}
+
+// Adds one to h[d] for every active lane.  When all active lanes hit the
+// same counter (sorted or clustered keys: consecutive keys share their high
 // digits) one lane adds their number instead: 64 atomics on one LDS address
-// would otherwise be serialised.
-__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d) {
-  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-  if (ballot(d != d0) == 0) {
+// would otherwise be serialised.  try_wave (wave-uniform) skips the check
+// (the tail of a histogram unit, a few keys).
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool try_wave = true) {
+  if (try_wave && wave_same(d)) {
     const uint64_t act = ballot(true);
     if (lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1))
-      atomicAdd(&h[d0], static_cast<uint32_t>(__popcll(act)));
+      atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
   } else {
     atomicAdd(&h[d], 1u);
   }
+}
+
+// atomicAdd(&h[d], 1) for every active lane, returning each lane's old
+// value (distinct ranks, in no particular order within the wave), with one
+// atomic when all active lanes hold the same digit (checked only when
+// try_wave, as for hist_add).
+__device__ __forceinline__ uint32_t rank_add(uint32_t* h, uint32_t d, bool try_wave) {
+  if (try_wave && wave_same(d)) {
+    const uint64_t act = ballot(true);
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
+    return __shfl(base, leader, kWave) + popc_below(act);
+  }
+  return atomicAdd(&h[d], 1u);
 }
 
 // Pass 0's digit counts of the sort tiles a histogram unit covers (the unit
@@ -116,22 +136,22 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
         if (i0 + r * kHistBlock >= n4) break;
         const uint32_t k[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
         // the four keys share a tile (tiles hold a multiple of four keys)
-        uint32_t* tc = s_tc + (((4 * (i0 + r * kHistBlock)) >> tile_shift) * 256);
+        const uint32_t tc_row = ((4 * (i0 + r * kHistBlock)) >> tile_shift) * 256;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
           for (uint32_t p = 0; p < kMaxPasses; ++p)
             if (p < n_passes && (p > 0 || !per_tile))
-              hist_add(s_h + p * 256, (k[c] >> (8 * p)) & mask[p]);
-          if (per_tile) atomicAdd(&tc[k[c] & mask[0]], 1u);
+              hist_add(s_h, p * 256 + ((k[c] >> (8 * p)) & mask[p]));
+          if (per_tile) hist_add(s_tc, tc_row + (k[c] & mask[0]));
         }
       }
     }
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
       const uint32_t k = keys[i];
       for (uint32_t p = per_tile ? 1 : 0; p < n_passes; ++p)
-        hist_add(s_h + p * 256, (k >> (8 * p)) & mask[p]);
-      if (per_tile) atomicAdd(&s_tc[((i - first) >> tile_shift) * 256 + (k & mask[0])], 1u);
+        hist_add(s_h, p * 256 + ((k >> (8 * p)) & mask[p]), false);
+      if (per_tile) hist_add(s_tc, ((i - first) >> tile_shift) * 256 + (k & mask[0]), false);
     }
   }
   __syncthreads();
@@ -191,8 +211,8 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
 #pragma unroll
       for (uint32_t p = 0; p < kMaxPasses; ++p)
         if (p < n_passes && (p > 0 || !per_tile))
-          hist_add(s_h + p * 256, (k >> (8 * p)) & mask[p]);
-      if (per_tile) atomicAdd(&s_tc[((i - first) >> tile_shift) * 256 + (k & mask[0])], 1u);
+          hist_add(s_h, p * 256 + ((k >> (8 * p)) & mask[p]));
+      if (per_tile) hist_add(s_tc, ((i - first) >> tile_shift) * 256 + (k & mask[0]));
     }
   }
   __syncthreads();
@@ -405,7 +425,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       } else {
 #pragma unroll
         for (int k = 0; k < KPT; ++k)
-          if (n_valid == TILE || wbase + k * kWave < lim) atomicAdd(&s_count[(keys[k] >> shift) & dmask], 1u);
+          if (n_valid == TILE || wbase + k * kWave < lim) hist_add(s_count, (keys[k] >> shift) & dmask);
         __syncthreads();
       }
       if (!kTable && tid < 256) {
@@ -424,12 +444,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       // Ranks within the wave, per digit counter of the wave.
       uint32_t* wh = s_whist[wave];
       uint32_t rank[KPT];
+      // pass 0: a wave whose first keys share their digit may hold ordered keys
+      const bool ordered = kFirst && wave_same((keys[0] >> shift) & dmask);
 #pragma unroll
       for (int k = 0; k < KPT; ++k) {
         const uint32_t d = (keys[k] >> shift) & dmask;
         if (kFirst) {
           // padding keys take no rank: they are not staged
-          rank[k] = (n_valid == TILE || wbase + k * kWave < lim) ? atomicAdd(&wh[d], 1u) : 0u;
+          rank[k] = 0;
+          if (n_valid == TILE || wbase + k * kWave < lim) rank[k] = rank_add(wh, d, ordered);
           continue;
         }
         // Stable multisplit: lanes holding the same digit find each other
