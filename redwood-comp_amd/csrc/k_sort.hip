@@ -40,8 +40,8 @@ __device__ __forceinline__ bool wave_same(uint32_t x) {
 // Adds one to h[d] for every active lane.  When all active lanes hit the
 // same counter (sorted or clustered keys: consecutive keys share their high
 // digits) one lane adds their number instead: 64 atomics on one LDS address
-// would otherwise be serialised.  try_wave (wave-uniform) skips the check
-// (the tail of a histogram unit, a few keys).
+// would otherwise be serialised.  try_wave (wave-uniform) false skips the
+// check: the caller saw no digit the wave shares (or only a few keys).
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool try_wave = true) {
   if (try_wave && wave_same(d)) {
     const uint64_t act = ballot(true);
@@ -137,14 +137,22 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
         const uint32_t k[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
         // the four keys share a tile (tiles hold a multiple of four keys)
         const uint32_t tc_row = ((4 * (i0 + r * kHistBlock)) >> tile_shift) * 256;
+        // keys that are ordered, or share their low bits, show as a top or
+        // a low digit common to the whole wave
+        const bool shared = wave_same(k[0] >> (8 * (n_passes - 1))) || wave_same(k[0] & mask[0]);
+        // one branch per group of keys; the plain counts stay a tight loop
+        const auto count4 = [&](auto check) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+          for (int c = 0; c < 4; ++c) {
 #pragma unroll
-          for (uint32_t p = 0; p < kMaxPasses; ++p)
-            if (p < n_passes && (p > 0 || !per_tile))
-              hist_add(s_h, p * 256 + ((k[c] >> (8 * p)) & mask[p]));
-          if (per_tile) hist_add(s_tc, tc_row + (k[c] & mask[0]));
-        }
+            for (uint32_t p = 0; p < kMaxPasses; ++p)
+              if (p < n_passes && (p > 0 || !per_tile))
+                hist_add(s_h, p * 256 + ((k[c] >> (8 * p)) & mask[p]), decltype(check)::value);
+            if (per_tile) hist_add(s_tc, tc_row + (k[c] & mask[0]), decltype(check)::value);
+          }
+        };
+        if (shared) count4(std::true_type{});
+        else count4(std::false_type{});
       }
     }
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
@@ -208,11 +216,14 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
       if (i >= end) break;
       const uint32_t k = morton_encode(q[r].x, q[r].y, q[r].z, quant);
       codes[i] = k;
+      // points that are ordered, or share their low bits, show as a top or a
+      // low digit common to the whole wave
+      const bool shared = wave_same(k >> (8 * (n_passes - 1))) || wave_same(k & mask[0]);
 #pragma unroll
       for (uint32_t p = 0; p < kMaxPasses; ++p)
         if (p < n_passes && (p > 0 || !per_tile))
-          hist_add(s_h, p * 256 + ((k >> (8 * p)) & mask[p]));
-      if (per_tile) hist_add(s_tc, ((i - first) >> tile_shift) * 256 + (k & mask[0]));
+          hist_add(s_h, p * 256 + ((k >> (8 * p)) & mask[p]), shared);
+      if (per_tile) hist_add(s_tc, ((i - first) >> tile_shift) * 256 + (k & mask[0]), shared);
     }
   }
   __syncthreads();
@@ -423,9 +434,11 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       if (kTable) {
         if (tid < 256) excl = tile_excl[static_cast<size_t>(tid) * n_tiles + tile];
       } else {
+        // a wave whose first keys share their digit may hold ordered keys
+        const bool shared = wave_same((keys[0] >> shift) & dmask);
 #pragma unroll
         for (int k = 0; k < KPT; ++k)
-          if (n_valid == TILE || wbase + k * kWave < lim) hist_add(s_count, (keys[k] >> shift) & dmask);
+          if (n_valid == TILE || wbase + k * kWave < lim) hist_add(s_count, (keys[k] >> shift) & dmask, shared);
         __syncthreads();
       }
       if (!kTable && tid < 256) {
@@ -445,16 +458,24 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       uint32_t* wh = s_whist[wave];
       uint32_t rank[KPT];
       // pass 0: a wave whose first keys share their digit may hold ordered keys
-      const bool ordered = kFirst && wave_same((keys[0] >> shift) & dmask);
+      if (kFirst) {
+        // padding keys take no rank: they are not staged.  One branch per
+        // tile: the plain ranks stay a tight loop.
+        const auto ranks = [&](auto check) {
+#pragma unroll
+          for (int k = 0; k < KPT; ++k) {
+            rank[k] = 0;
+            if (n_valid == TILE || wbase + k * kWave < lim)
+              rank[k] = rank_add(wh, (keys[k] >> shift) & dmask, decltype(check)::value);
+          }
+        };
+        if (wave_same((keys[0] >> shift) & dmask)) ranks(std::true_type{});
+        else ranks(std::false_type{});
+      }
 #pragma unroll
       for (int k = 0; k < KPT; ++k) {
         const uint32_t d = (keys[k] >> shift) & dmask;
-        if (kFirst) {
-          // padding keys take no rank: they are not staged
-          rank[k] = 0;
-          if (n_valid == TILE || wbase + k * kWave < lim) rank[k] = rank_add(wh, d, ordered);
-          continue;
-        }
+        if (kFirst) continue;
         // Stable multisplit: lanes holding the same digit find each other
         // with BITS ballots (a lane differs from this one where some ballot
         // differs from this lane's own bit); the lowest lane of each group
