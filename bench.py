@@ -65,7 +65,7 @@ def enqueue_stages(rw, ws, tb, n, stream, events=None):
         lambda: L.rwh_morton(s.points, n, 0.0, 1024.0, s.morton_keys, stream),
         lambda: L.rwh_sort_u32(ws.ptr, s.morton_keys, s.morton_keys_alt, n, 30, stream),
         lambda: L.rwh_unique_u32(ws.ptr, s.morton_keys_alt, n, s.morton_keys, nu_p, stream),
-        lambda: L.rwh_radix_tree(s.morton_keys, nu_p, n, s.prefix_n, s.has_leaf_left,
+        lambda: L.rwh_radix_tree(ws.ptr, s.morton_keys, nu_p, n, s.prefix_n, s.has_leaf_left,
                                  s.has_leaf_right, s.left_child, s.parents, stream),
         lambda: L.rwh_edge_count(s.prefix_n, s.parents, nu_p, n, s.edge_count, stream),
         lambda: L.rwh_prefix_sum(ws.ptr, s.edge_count, nu_p, n, s.edge_offset, no_p, stream),

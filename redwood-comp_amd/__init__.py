@@ -74,7 +74,7 @@ ABI = {
     "rwh_morton": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_sort_u32": (_I, [_P, _P, _P, _U32, _U32, _P]),
     "rwh_unique_u32": (_I, [_P, _P, _U32, _P, _P, _P]),
-    "rwh_radix_tree": (_I, [_P, _P, _U32, _P, _P, _P, _P, _P, _P]),
+    "rwh_radix_tree": (_I, [_P, _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
     "rwh_edge_count": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rwh_prefix_sum": (_I, [_P, _P, _P, _U32, _P, _P, _P]),
     "rwh_octree": (_I, [_P, _P, _P, _U32, _F, _F, _P]),

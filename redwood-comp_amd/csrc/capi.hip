@@ -96,6 +96,7 @@ struct rwh_workspace {
   uint32_t* ctrl = nullptr;    // 3 users x kCtrlWords
   uint32_t* counts = nullptr;  // n_unique[max_segs], n_oct[max_segs]
   uint32_t* tables = nullptr;  // seg_base, seg_n, oct_base, oct_cap [max_segs each], chunk_seg
+  uint32_t* tree_list = nullptr;  // radix tree: edge-crossing nodes per tile
   uint32_t* h_stage = nullptr;  // pinned staging of tables
   uint32_t* h_counts = nullptr;  // pinned counts read-back
   hipEvent_t upload_done = nullptr;
@@ -364,6 +365,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   const size_t o_ctrl = take(sizeof(uint32_t) * 3 * kCtrlWords);
   const size_t o_cnt = take(sizeof(uint32_t) * 2ull * max_segments);
   const size_t o_tab = take(sizeof(uint32_t) * ws->table_words());
+  const size_t o_tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
   ws->bytes = off;
   char* base = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), off);
@@ -383,6 +385,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   ws->ctrl = reinterpret_cast<uint32_t*>(base + o_ctrl);
   ws->counts = reinterpret_cast<uint32_t*>(base + o_cnt);
   ws->tables = reinterpret_cast<uint32_t*>(base + o_tab);
+  ws->tree_list = reinterpret_cast<uint32_t*>(base + o_tl);
   // Look-back words, histograms and counters must start at zero; later
   // launches keep them consistent by themselves.
   e = hipMemset(base, 0, off);
@@ -475,17 +478,19 @@ int rwh_unique_u32(rwh_workspace* ws, const uint32_t* d_sorted, uint32_t n, uint
   return launched();
 }
 
-int rwh_radix_tree(const uint32_t* d_codes, const uint32_t* d_n_unique, uint32_t n_max,
+int rwh_radix_tree(rwh_workspace* ws, const uint32_t* d_codes, const uint32_t* d_n_unique, uint32_t n_max,
                    uint8_t* d_prefix_n, uint8_t* d_has_leaf_left, uint8_t* d_has_leaf_right,
                    int32_t* d_left_child, int32_t* d_parents, rwh_stream_t stream) {
   if (injected(4)) return RWH_ERR_INJECTED;
   RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
+  RWH_CHECK(ws, "null workspace");
+  RWH_CHECK(n_max <= ws->max_points, "n_max above the workspace's capacity");
   if (n_max == 0) return RWH_OK;
   RWH_CHECK(d_codes && d_n_unique && d_prefix_n && d_has_leaf_left && d_has_leaf_right &&
                 d_left_child && d_parents,
             "null buffer");
   TreeArgs a{d_codes, d_prefix_n, d_has_leaf_left, d_has_leaf_right, d_left_child, d_parents};
-  launch_radix_tree(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
+  launch_radix_tree(a, single_seg_dev(d_n_unique), n_max, ws->tree_list, static_cast<hipStream_t>(stream));
   return launched();
 }
 
@@ -562,7 +567,7 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
                          st, true));
   }
   RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, stream));
-  RWH_TRY(rwh_radix_tree(bufs->morton_keys, d_nu, n, bufs->prefix_n, bufs->has_leaf_left,
+  RWH_TRY(rwh_radix_tree(ws, bufs->morton_keys, d_nu, n, bufs->prefix_n, bufs->has_leaf_left,
                          bufs->has_leaf_right, bufs->left_child, bufs->parents, stream));
   // S5 + S6 fused: the counts are computed inside the scan and written beside it
   if (n == 0) {
@@ -638,7 +643,7 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
   RWH_TRY(launched());
   TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
               bufs->left_child, bufs->parents};
-  launch_radix_tree(ta, uni, n_index, st);
+  launch_radix_tree(ta, uni, n_index, ws->tree_list, st);
   launch_edge_scan(bufs->prefix_n, bufs->parents, bufs->edge_count, bufs->edge_offset, uni, n_index,
                    ws->n_oct(), ws->lb_scan, ws->ctrl + 2 * kCtrlWords, st);
   OctArgs oa = oct_args(bufs, min_coord, range);

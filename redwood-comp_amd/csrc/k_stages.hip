@@ -264,22 +264,24 @@ __device__ RefNode ref_node(const uint32_t* codes, int i, int n) {
   return r;
 }
 
-// The whole segment with the reference's search.  Such a tree need not be a
+// The reference's search over a whole segment.  Such a tree need not be a
 // proper binary tree (a parent slot can have several claimers), so parent[]
-// is the maximum claimer over zeroed slots (D7) built with atomics by the
-// segment's first workgroup alone (zeroing, barrier, claims); the other
-// outputs are written by each node's own thread.
-__device__ void ref_tile(const TreeArgs& a, const uint32_t* codes, uint32_t base, int b0, int n) {
-  const int tid = static_cast<int>(threadIdx.x);
-  if (b0 + tid < n) {
-    const int i = b0 + tid;
+// is the maximum claimer over zeroed slots (D7), built with atomics by the
+// segment's first workgroup alone (ref_parents: zeroing, barrier, claims);
+// the other outputs are written by each node's own thread (ref_nodes).
+__device__ void ref_nodes(const TreeArgs& a, const uint32_t* codes, uint32_t base, int b_begin, int b_end,
+                          int n) {
+  for (int i = b_begin + static_cast<int>(threadIdx.x); i < b_end; i += static_cast<int>(blockDim.x)) {
     const RefNode r = ref_node(codes, i, n);
     a.prefix_n[base + i] = static_cast<uint8_t>(r.dnode);
     a.left_child[base + i] = r.gamma;
     a.has_leaf_left[base + i] = r.ll ? 1 : 0;
     a.has_leaf_right[base + i] = r.lr ? 1 : 0;
   }
-  if (b0 != 0) return;
+}
+
+__device__ void ref_parents(const TreeArgs& a, const uint32_t* codes, uint32_t base, int n) {
+  const int tid = static_cast<int>(threadIdx.x);
   int32_t* parent = a.parent + base;
   for (int i = tid; i < n; i += static_cast<int>(blockDim.x)) parent[i] = 0;
   __syncthreads();
@@ -292,139 +294,293 @@ __device__ void ref_tile(const TreeArgs& a, const uint32_t* codes, uint32_t base
   }
 }
 
-// ---- Karras search through an LDS window (keys below 2^31) ---------------
-// Codes of a segment, read through an LDS copy of the window around the
-// workgroup's nodes (the searches of most nodes stay inside it) and from
-// global memory outside it.  The LDS read is issued unconditionally (at a
-// clamped index; the empty asm statement pins its result) and the global one
-// only outside the window, so the common case stays a ds_read instead of both
-// becoming one flat load.
-struct CodeWindow {
-  const uint32_t* g;  // segment's codes
-  const uint32_t* s;  // LDS copy of g[lo, lo + size)
-  int lo;
-  uint32_t size;
-  // code k of the segment
-  __device__ __forceinline__ uint32_t operator[](int k) const { return rel(static_cast<uint32_t>(k - lo)); }
-  // code lo + u (an index below lo wraps above size)
-  __device__ __forceinline__ uint32_t rel(uint32_t u) const {
-    uint32_t v = s[min(u, size - 1)];
-    asm volatile("" : "+v"(v));
-    if (u >= size) v = g[static_cast<int>(u) + lo];
-    return v;
-  }
-};
-
-// x * d for a direction d of +1 or -1 given as m = (d < 0 ? -1 : 0): two
-// full-rate bit operations instead of a (quarter-rate) 64-bit multiply-add.
-__device__ __forceinline__ int signed_step(int x, int m) { return (x ^ m) - m; }
-
-constexpr int kTreeHalo = 384;  // codes cached on each side of the workgroup's nodes
-
-// With every key below 2^31, clz(a ^ b) >= 1, so delta8 = clz - 1 has no uint8
-// wrap and  delta8(ci, x) > t  <=>  (ci ^ x) < agree_below(t):  one xor and one
-// compare per probe instead of the clz arithmetic.
+// ---- keys below 2^31: the tree grown from its leaves ------------------------
+// With every key below 2^31, clz(a ^ b) >= 1, so delta8 = clz - 1 has no
+// uint8 wrap and  delta8(ci, x) > t  <=>  (ci ^ x) < agree_below(t).
 __device__ __forceinline__ uint32_t agree_below(int t) {
   return t >= 31 ? 0u : (1u << (30 - t));
 }
 
-// Split search of the reference (func_brt.hpp:90-104) for node i with range
-// length l, direction d, thr = agree_below(delta of the node); returns gamma.
-__device__ __forceinline__ int brt_split(const CodeWindow& codes, int i, int n, int d, int l,
-                                         uint32_t thr, uint32_t ci) {
-  int s = 0;
-  const int m = d < 0 ? -1 : 0;
-  const uint32_t wi = static_cast<uint32_t>(i - codes.lo);  // i in the window's frame
-  const uint32_t max_div = 1u << log2_ceil_u32(static_cast<uint32_t>(l));
-  const int scut = (d == -1) ? i - 1 : n - i - 1;
-  uint32_t t = (static_cast<uint32_t>(l) + 1) / 2;
-  for (uint32_t div = 2; div <= max_div; div <<= 1) {
-    if (s + static_cast<int>(t) <= scut &&
-        (ci ^ codes.rel(wi + signed_step(s + static_cast<int>(t), m))) < thr)
-      s += static_cast<int>(t);
-    t = (static_cast<uint32_t>(l) + (div << 1) - 1) / (div << 1);
-  }
-  return i + signed_step(s, m) + min(d, 0);
-}
+// x * d for a direction d of +1 or -1 given as m = (d < 0 ? -1 : 0).
+__device__ __forceinline__ int signed_step(int x, int m) { return (x ^ m) - m; }
 
-// One thread per node i.  With keys below 2^31 (the largest is the last)
-// the tree is a proper binary tree and every parent slot has one claimer,
-// stored plainly; otherwise the segment takes the reference path above.
-__global__ __launch_bounds__(kElemBlock) void k_radix_tree(TreeArgs a, SegArgs seg) {
-  __shared__ uint32_t s_codes[kElemBlock + 2 * kTreeHalo];
-  const uint32_t gi = blockIdx.x * kElemBlock + threadIdx.x;
-  const Seg sg = seg_of(seg, blockIdx.x * kElemBlock);
-  const int n = static_cast<int>(sg.count);  // n_brt; codes hold n + 1 keys
-  const int b0 = static_cast<int>(blockIdx.x * kElemBlock - sg.base);
-  if (b0 >= n) return;  // whole workgroup past the segment's nodes
-  const uint32_t* __restrict__ gcodes = a.codes + sg.base;
-  if (gcodes[n] >= 0x80000000u) {  // keys with bit 31 set: the reference's own search
-    ref_tile(a, gcodes, sg.base, b0, n);
+// Over unique keys the reference's searches (func_brt.hpp:57-104) find the
+// canonical binary radix tree: node i's range [A, B] is the largest run of
+// keys around i that share a prefix longer than i's other neighbour does,
+// its split gamma is where that run's keys first differ, and i is the end
+// of the range next to its parent's split (B for a left child, A for a
+// right child).  The same tree grows from the leaves (Apetrei, "Fast and
+// Simple Agglomerative LBVH Construction", 2014): a node [a, b] is the left
+// child of a parent split at b when delta(b, b+1) > delta(a-1, a), else the
+// right child of a parent split at a-1 (the two never tie over unique keys;
+// deltas outside [0, n] are -1), and the second child to arrive at a split
+// forms the parent.  Each node is formed once, by one thread, with one LDS
+// exchange, instead of three binary searches per node.
+//
+// k_tree_local grows every node whose range lies inside one tile of
+// kTreeTile leaves, in LDS, and writes the tile's outputs coalesced; the
+// nodes whose range crosses a tile edge (at most 62 per tile: those holding
+// the pair of keys at one of its two edges, at most one per prefix length
+// and edge, plus node 0) are listed per tile, and k_tree_cross computes
+// them with the searches above, eight lanes per node.
+
+// LDS slot of an interior split: 0 while empty, else the first child's far
+// end (its end away from the split) - lo + 1, and that end's outer delta + 1
+// in the upper half.
+__global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs seg, uint32_t* __restrict__ list) {
+  constexpr int T = kTreeTile;
+  __shared__ uint32_t s_ks[T + 2];  // keys lo-1 .. hi at [k - lo + 1]; then the split slots
+  __shared__ uint8_t s_e[T + 2];    // delta(k, k+1) (clz) + 1 for k in [lo-1, hi-1] at [k - lo + 1]; 0 outside
+  __shared__ uint16_t s_lc[T];      // node lo + x: its split - lo
+  __shared__ uint16_t s_par[T];     // node lo + x: its parent - lo (0xffff: not formed here)
+  __shared__ uint8_t s_meta[T];     // node lo + x: 0x80 formed here | leaf left << 6 | leaf right << 5 | prefix
+  __shared__ uint32_t s_n_cross;
+  const int tid = static_cast<int>(threadIdx.x);
+  const uint32_t first = blockIdx.x * static_cast<uint32_t>(T);
+  const Seg sg = seg_of(seg, first);
+  const int n = static_cast<int>(sg.count);  // internal nodes; leaves (keys) 0..n
+  const int lo = static_cast<int>(first - sg.base);
+  if (lo >= n) return;  // whole tile past the segment's nodes
+  const uint32_t* __restrict__ codes = a.codes + sg.base;
+  uint32_t* __restrict__ my_list = list + static_cast<size_t>(blockIdx.x) * kTreeListStride;
+  if (codes[n] >= 0x80000000u) {  // keys with bit 31 set: the reference's own search
+    ref_nodes(a, codes, sg.base, lo, min(lo + T, n), n);
+    if (lo == 0) ref_parents(a, codes, sg.base, n);
     return;
   }
-  const int wlo = max(b0 - kTreeHalo, 0), whi = min(b0 + kElemBlock + kTreeHalo, n + 1);
-  CodeWindow codes{gcodes, s_codes, wlo, static_cast<uint32_t>(whi - wlo)};
-  for (int k = wlo + static_cast<int>(threadIdx.x); k < whi; k += kElemBlock) s_codes[k - wlo] = gcodes[k];
-  __syncthreads();
-  const int i = static_cast<int>(gi - sg.base);
-  if (i >= n) return;
-  int32_t* __restrict__ parent = a.parent + sg.base;
-  const uint32_t ci = codes[i];
-
-  int d, l = 0;
-  if (i == 0) {
-    d = 1;
-    l = n - 1;  // Q2
-  } else {
-    // sign(delta(i, i+1) - delta(i, i-1)) = sign of the clz difference
-    const int cp = __clz(ci ^ codes[i + 1]), cm = __clz(ci ^ codes[i - 1]);
-    d = (cp > cm) - (cp < cm);
-    const int m = d < 0 ? -1 : 0;
-    const uint32_t wi = static_cast<uint32_t>(i - codes.lo);  // i in the window's frame
-    // delta(i, i - d) is the other neighbour's: clz - 1 (no uint8 wrap below 2^31)
-    const uint32_t thr = agree_below((d > 0 ? cm : cp) - 1);
-    // The range end: l is the largest x <= cutoff with delta(i, i + x*d) >
-    // delta_min, and that predicate only turns false as x grows, so any
-    // bound lmax where it is false (or past cutoff) leads the binary search
-    // to the same l as the reference's doubling.  The bound grows 4-fold,
-    // which shortens the wave's longest search (its largest node's).
-    // The probe i + lmax*d is inside [0, n] while lmax <= cutoff.
-    const int cutoff = (d == -1) ? i : n - i;
-    uint32_t lmax = 4;
-    while (lmax <= static_cast<uint32_t>(cutoff) &&
-           (ci ^ codes.rel(wi + signed_step(static_cast<int>(lmax), m))) < thr)
-      lmax = lmax < (1u << 29) ? lmax << 2 : (1u << 31);
-    for (uint32_t t = lmax / 2; t >= 1; t /= 2)
-      if (l + static_cast<int>(t) <= cutoff && (ci ^ codes.rel(wi + signed_step(l + static_cast<int>(t), m))) < thr)
-        l += static_cast<int>(t);
+  // leaves [lo, hi): the segment's last tile also takes leaf n
+  const int hi = (lo + T < n) ? lo + T : n + 1;
+  const int k_end = min(hi, n);
+  {
+    // every load in flight before the first LDS store
+    constexpr int kLoads = (T + 2 + kTreeBlock - 1) / kTreeBlock;
+    const int k0 = max(lo - 1, 0) + tid;
+    uint32_t v[kLoads];
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) v[j] = (k0 + j * kTreeBlock <= k_end) ? codes[k0 + j * kTreeBlock] : 0u;
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j)
+      if (k0 + j * kTreeBlock <= k_end) s_ks[k0 + j * kTreeBlock - lo + 1] = v[j];
   }
-  const int j = i + (d < 0 ? -l : l);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < (T + 2 + kTreeBlock - 1) / kTreeBlock; ++j) {
+    const int k = lo - 1 + tid + j * kTreeBlock;
+    if (k < hi)
+      s_e[k - lo + 1] = (k < 0 || k >= n) ? uint8_t{0}
+                                           : static_cast<uint8_t>(__clz(s_ks[k - lo + 1] ^ s_ks[k - lo + 2]) + 1);
+  }
+  __syncthreads();
+  uint32_t* s_slot = s_ks;  // split g in [lo, hi-2] at [g - lo]
+  for (int x = tid; x < T; x += kTreeBlock) {
+    s_slot[x] = 0;
+    s_par[x] = 0xffff;
+    s_meta[x] = 0;
+  }
+  if (tid == 0) s_n_cross = 0;
+  __syncthreads();
+
+  // Each wave grows the nodes of its block of leaves: a lane whose climb has
+  // ended takes the block's next leaf, so the wave's lanes stay busy; a lane
+  // climbs while it is the second child at a split, and stops at a split on
+  // the tile's edge (that parent is computed by k_tree_cross).  Nodes are
+  // held as tile offsets [na, nb] with their outer deltas biased by one:
+  // ea = delta(na-1, na) + 1, eb = delta(nb, nb+1) + 1.  A child leaves at the
+  // split its far end (+1) and that end's outer delta, so the sibling forms
+  // the parent without reading LDS: one exchange per level.
+  constexpr int kWaves = kTreeBlock / kWave;
+  const int wave = tid / kWave;
+  const int n_leaf = hi - lo;  // at most T + 1
+  const int wlo = min(wave * (T / kWaves), n_leaf);
+  const int whi = wave == kWaves - 1 ? n_leaf : min(wlo + T / kWaves, n_leaf);
+  const uint32_t n_split = static_cast<uint32_t>(n_leaf - 1);  // interior splits: offsets [0, n_leaf - 2]
+  int next = wlo;
+  int na = 0, nb = 0, ea = 0, eb = 0;
+  bool have = false;
+  for (int guard = 0; guard < kTreeClimbGuard; ++guard) {
+    const uint64_t need = ballot(!have);
+    if (!have) {
+      const int k = next + static_cast<int>(popc_below(need));
+      if (k < whi) {
+        na = nb = k;
+        ea = s_e[k];
+        eb = s_e[k + 1];
+        have = true;
+      }
+    }
+    next += static_cast<int>(__popcll(need));
+    if (ballot(have) == 0) break;
+    if (!have) continue;
+    const bool right = eb > ea;
+    const int g = right ? nb : na - 1;
+    if (static_cast<uint32_t>(g) >= n_split) {  // the split is an edge of the tile
+      have = false;
+      continue;
+    }
+    const uint32_t mine = right ? (static_cast<uint32_t>(na + 1) | static_cast<uint32_t>(ea) << 16)
+                                : (static_cast<uint32_t>(nb + 1) | static_cast<uint32_t>(eb) << 16);
+    const uint32_t other = atomicExch(&s_slot[g], mine);
+    if (other == 0) {  // first child at the split: the sibling forms the parent
+      have = false;
+      continue;
+    }
+    const int ep = right ? eb : ea;  // delta at the split + 1: the parent's prefix + 2
+    const int sib = static_cast<int>(other & 0xffffu) - 1;
+    const int es = static_cast<int>(other >> 16);
+    if (right) {
+      nb = sib, eb = es;
+    } else {
+      na = sib, ea = es;
+    }
+    const bool ll = g == na, lr = g + 1 == nb;
+    if ((ea | eb) == 0) {  // the complete root [0, n] (lo == 0): index 0, whose record is Q2's
+      if (!ll) s_par[g] = 0;
+      if (!lr) s_par[g + 1] = 0;
+      s_par[0] = 0;
+      have = false;
+      continue;
+    }
+    const int i = eb > ea ? nb : na;
+    s_meta[i] = static_cast<uint8_t>(0x80u | (ll ? 0x40u : 0u) | (lr ? 0x20u : 0u) | static_cast<uint32_t>(ep - 2));
+    s_lc[i] = static_cast<uint16_t>(g);
+    if (!ll) s_par[g] = static_cast<uint16_t>(i);
+    if (!lr) s_par[g + 1] = static_cast<uint16_t>(i);
+  }
+  __syncthreads();
+
+  // The tile's nodes, coalesced; those formed elsewhere get placeholders that
+  // k_tree_cross overwrites, and are listed for it.
+  const int m = k_end - lo;
+  uint8_t* __restrict__ pfx = a.prefix_n + sg.base;
+  uint8_t* __restrict__ hll = a.has_leaf_left + sg.base;
+  uint8_t* __restrict__ hlr = a.has_leaf_right + sg.base;
+  int32_t* __restrict__ lc = a.left_child + sg.base;
+  int32_t* __restrict__ par = a.parent + sg.base;
+  for (int x = tid; x < m; x += kTreeBlock) {
+    const uint32_t meta = s_meta[x];
+    const uint32_t p = s_par[x];
+    const int i = lo + x;
+    pfx[i] = static_cast<uint8_t>(meta & 31u);
+    hll[i] = static_cast<uint8_t>((meta >> 6) & 1u);
+    hlr[i] = static_cast<uint8_t>((meta >> 5) & 1u);
+    lc[i] = lo + static_cast<int>(s_lc[x]);
+    par[i] = p == 0xffffu ? -1 : lo + static_cast<int>(p);
+    if (!(meta & 0x80u)) {
+      const uint32_t at = atomicAdd(&s_n_cross, 1u);
+      if (at < static_cast<uint32_t>(kTreeListCap)) my_list[1 + at] = static_cast<uint32_t>(i);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) my_list[0] = min(s_n_cross, static_cast<uint32_t>(kTreeListCap));
+}
+
+// Largest x in [lo, hi] with p(x), for p true at lo and true-then-false over
+// [lo, hi], by the kCrossGroup lanes of one group (lane r of the group, its
+// lanes at bit gshift of the wave's ballot): each round probes kCrossGroup
+// evenly spaced points and keeps the span after the last true one.  All
+// lanes of the group call it with the same lo and hi.
+template <class P>
+__device__ __forceinline__ int group_last_true(const P& p, int lo, int hi, int r, int gshift) {
+  while (lo < hi) {
+    const int step = (hi - lo + kCrossGroup - 1) / kCrossGroup;
+    const int x = lo + (r + 1) * step;
+    const bool t = x <= hi && p(x);
+    const uint32_t bits = static_cast<uint32_t>(ballot(t) >> gshift) & ((1u << kCrossGroup) - 1u);
+    lo += __popc(bits) * step;
+    hi = min(lo + step - 1, hi);
+  }
+  return lo;
+}
+
+// Node i (i >= 1) of a tree over keys below 2^31, by one group: its range
+// end and split with the reference's predicates, then its record and its
+// internal children's parent entries (lane 0 of the group).
+__device__ void cross_node(const TreeArgs& a, uint32_t base, const uint32_t* __restrict__ codes, int n, int i,
+                           int r, int gshift) {
+  const uint32_t ci = codes[i];
+  const int cp = __clz(ci ^ codes[i + 1]), cm = __clz(ci ^ codes[i - 1]);
+  const int d = cp > cm ? 1 : -1;  // never equal over unique keys
+  const int m = d < 0 ? -1 : 0;
+  // delta(i, i+x*d) > delta(i, i-d) for x in [0, l]; x <= cutoff keeps the probe in [0, n]
+  const uint32_t thr = agree_below((d > 0 ? cm : cp) - 1);
+  const int cutoff = d < 0 ? i : n - i;
+  const auto in_range = [&](int x) { return (ci ^ codes[i + signed_step(x, m)]) < thr; };
+  // bracket l: lane r probes 16^r (x = 1 always holds)
+  const int x16 = 1 << (4 * min(r, 7));
+  const bool t16 = r < 8 && x16 <= cutoff && in_range(x16);
+  const int k = max(__popc(static_cast<uint32_t>(ballot(t16) >> gshift) & ((1u << kCrossGroup) - 1u)), 1);
+  const int blo = 1 << (4 * (k - 1));
+  const int bhi = k < 8 ? min((1 << (4 * k)) - 1, cutoff) : cutoff;
+  const int l = group_last_true(in_range, blo, bhi, r, gshift);
+  const int j = i + signed_step(l, m);
   const int dnode = delta8(ci, codes[j]);
-  const int gamma = brt_split(codes, i, n, d, l, agree_below(dnode), ci);
-  const bool ll = min(i, j) == gamma;
-  const bool lr = max(i, j) == gamma + 1;
-  a.prefix_n[sg.base + i] = static_cast<uint8_t>(dnode);
-  a.left_child[sg.base + i] = gamma;
-  a.has_leaf_left[sg.base + i] = ll ? 1 : 0;
-  a.has_leaf_right[sg.base + i] = lr ? 1 : 0;
-  if (i > 0) {
-    if (!ll) parent[gamma] = i;
-    if (!lr) parent[gamma + 1] = i;
-  } else {
-    parent[0] = 0;
-    // children of the root over the complete range [0, n]
-    const int dfull = delta8(ci, codes[n]);
-    const int g = brt_split(codes, 0, n, 1, n, agree_below(dfull), ci);
-    if (g != 0) parent[g] = 0;
-    if (g + 1 < n) parent[g + 1] = 0;
+  const uint32_t thr2 = agree_below(dnode);
+  const int s = group_last_true([&](int x) { return (ci ^ codes[i + signed_step(x, m)]) < thr2; }, 0, l - 1,
+                                r, gshift);
+  const int gamma = i + signed_step(s, m) + min(d, 0);
+  if (r == 0) {
+    const bool ll = min(i, j) == gamma, lr = max(i, j) == gamma + 1;
+    a.prefix_n[base + i] = static_cast<uint8_t>(dnode);
+    a.left_child[base + i] = gamma;
+    a.has_leaf_left[base + i] = ll ? 1 : 0;
+    a.has_leaf_right[base + i] = lr ? 1 : 0;
+    if (!ll) a.parent[base + gamma] = i;
+    if (!lr) a.parent[base + gamma + 1] = i;
   }
 }
 
-void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
-  const uint32_t grid = (n_index + kElemBlock - 1) / kElemBlock;
+// Node 0: the Q2 record over [0, n-1], parent[0] = 0, and 0 for the internal
+// children of the complete root over [0, n] (the slots no node i >= 1 claims).
+__device__ void cross_root(const TreeArgs& a, uint32_t base, const uint32_t* __restrict__ codes, int n, int r,
+                           int gshift) {
+  const uint32_t c0 = codes[0];
+  const int l = n - 1;  // Q2
+  const int dnode = delta8(c0, codes[l]);
+  const uint32_t thr = agree_below(dnode);
+  const int gamma = group_last_true([&](int x) { return (c0 ^ codes[x]) < thr; }, 0, max(l - 1, 0), r, gshift);
+  const uint32_t thr_full = agree_below(delta8(c0, codes[n]));
+  const int g_full = group_last_true([&](int x) { return (c0 ^ codes[x]) < thr_full; }, 0, n - 1, r, gshift);
+  if (r == 0) {
+    a.prefix_n[base] = static_cast<uint8_t>(dnode);
+    a.left_child[base] = gamma;
+    a.has_leaf_left[base] = gamma == 0 ? 1 : 0;
+    a.has_leaf_right[base] = l == gamma + 1 ? 1 : 0;
+    a.parent[base] = 0;
+    if (g_full != 0) a.parent[base + g_full] = 0;
+    if (g_full + 1 < n) a.parent[base + g_full + 1] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kCrossBlock) void k_tree_cross(TreeArgs a, SegArgs seg,
+                                                           const uint32_t* __restrict__ list) {
+  const uint32_t first = blockIdx.x * static_cast<uint32_t>(kTreeTile);
+  const Seg sg = seg_of(seg, first);
+  const int n = static_cast<int>(sg.count);
+  const int lo = static_cast<int>(first - sg.base);
+  if (lo >= n) return;
+  const uint32_t* __restrict__ codes = a.codes + sg.base;
+  if (codes[n] >= 0x80000000u) return;  // done by the reference's search
+  const uint32_t* __restrict__ my_list = list + static_cast<size_t>(blockIdx.x) * kTreeListStride;
+  const int cnt = static_cast<int>(my_list[0]);
+  const int tid = static_cast<int>(threadIdx.x);
+  const int grp = tid / kCrossGroup, r = tid % kCrossGroup;
+  const int gshift = (tid % kWave) & ~(kCrossGroup - 1);
+  for (int e = grp; e < cnt; e += kCrossBlock / kCrossGroup) {
+    const int i = static_cast<int>(my_list[1 + e]);
+    if (i == 0) {
+      cross_root(a, sg.base, codes, n, r, gshift);
+    } else {
+      cross_node(a, sg.base, codes, n, i, r, gshift);
+    }
+  }
+}
+
+void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, uint32_t* list, hipStream_t stream) {
+  const uint32_t grid = (n_index + kTreeTile - 1) / kTreeTile;
   if (grid == 0) return;
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_radix_tree, dim3(grid), dim3(kElemBlock), 0, stream, a, seg);
+  hipLaunchKernelGGL(k_tree_local, dim3(grid), dim3(kTreeBlock), 0, stream, a, seg, list);
+  hipLaunchKernelGGL(k_tree_cross, dim3(grid), dim3(kCrossBlock), 0, stream, a, seg, list);
 }
 
 // ---------------------------------------------------------------------------

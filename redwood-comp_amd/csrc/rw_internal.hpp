@@ -63,6 +63,24 @@ static_assert(kScanTile == kChunk, "segmented scans tile by chunk");
 // Elementwise tiles.
 constexpr int kElemBlock = 256;
 
+// Radix tree (S4): tiles of leaves grown in LDS, then the nodes that cross a
+// tile edge (listed per tile) by groups of lanes.
+constexpr int kTreeTile = 4096;   // leaves per tile (the segment's last tile also takes leaf n)
+constexpr int kTreeBlock = 512;
+constexpr int kTreeListCap = 64;  // edge-crossing nodes per tile: at most 31 per edge, plus node 0
+constexpr int kTreeListStride = kTreeListCap + 1;  // count, then the nodes
+// loop bound of a wave's climbs: every iteration takes a leaf, forms a node
+// or ends a climb, and a tile has at most kTreeTile + 1 leaves and nodes
+constexpr int kTreeClimbGuard = 3 * kTreeTile + 8;
+constexpr int kCrossGroup = 8;    // lanes per edge-crossing node
+constexpr int kCrossBlock = 256;  // four waves: 32 nodes at a time
+static_assert(kChunk % kTreeTile == 0, "a tree tile never straddles two clouds");
+static_assert(kTreeTile < 0xffff, "LDS node offsets are 16-bit with 0xffff reserved");
+static_assert(kWave % kCrossGroup == 0 && kCrossGroup == 8, "the range bracket probes 16^r on 8 lanes");
+inline uint32_t tree_list_words(uint32_t n_index) {
+  return ((n_index + kTreeTile - 1) / kTreeTile) * static_cast<uint32_t>(kTreeListStride);
+}
+
 // Workgroups of a kernel that fit on the device at once (at most n_tiles):
 // the look-back kernels loop over their tiles, so a larger grid would only
 // queue.
@@ -134,7 +152,7 @@ struct TreeArgs {
   int32_t* parent;
 };
 // seg.count = n_unique per segment with count_sub = 1, i.e. the live count is n_brt
-void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
+void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, uint32_t* list, hipStream_t stream);
 void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
                        SegArgs seg, uint32_t n_index, hipStream_t stream);
 // Exclusive scan of the live count (n_brt) per segment; total per segment.

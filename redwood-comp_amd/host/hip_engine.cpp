@@ -258,7 +258,7 @@ std::future<void> run_stage3(AppData& a) {
 
 std::future<void> run_stage4(AppData& a) {
   return stage(a, "run_stage4", [&](Engine& e, DeviceState& s) {
-    return rwh_radix_tree(a.get_unique_morton_keys(), d_n_unique(s), a.n_input,
+    return rwh_radix_tree(s.ws, a.get_unique_morton_keys(), d_n_unique(s), a.n_input,
                           a.brt.u_prefix_n.data(), a.brt.u_has_leaf_left.data(),
                           a.brt.u_has_leaf_right.data(), a.brt.u_left_child.data(),
                           a.brt.u_parents.data(), e.stream());

@@ -257,13 +257,53 @@ def test_radix_tree_stage(gpu, orc, pool, kind, n):
     out = {"prefix_n": rw.DeviceArray(nu, np.uint8), "has_leaf_left": rw.DeviceArray(nu, np.uint8),
            "has_leaf_right": rw.DeviceArray(nu, np.uint8), "left_child": rw.DeviceArray(nu, np.int32),
            "parents": rw.DeviceArray(nu, np.int32, zero=True)}
-    rw.check(L.rwh_radix_tree(d_k.ptr, d_nu.ptr, nu, out["prefix_n"].ptr, out["has_leaf_left"].ptr,
+    ws = rw.Workspace(nu)
+    rw.check(L.rwh_radix_tree(ws.ptr, d_k.ptr, d_nu.ptr, nu, out["prefix_n"].ptr, out["has_leaf_left"].ptr,
                               out["has_leaf_right"].ptr, out["left_child"].ptr, out["parents"].ptr,
                               None))
     rw.check(L.rwh_stream_sync(None))
     want = orc.radix_tree(pool, k)
     for name, arr in out.items():
         assert np.array_equal(arr.numpy(nu - 1), want[name]), name
+
+
+def exact_keys(rng, kind, nu):
+    """nu sorted unique keys below 2^30."""
+    if kind == "k30":
+        k = rng.choice(1 << 30, size=nu, replace=False)
+    elif kind == "stride":  # a regular tree: every split at a power of two
+        k = np.arange(nu, dtype=np.uint64) << 7
+    else:  # clustered: 16 tight clusters, long chains of shared prefixes
+        centers = rng.choice(1 << 10, size=16, replace=False).astype(np.uint64) << 20
+        per = -(-nu // 16)
+        k = np.concatenate([c + rng.choice(1 << 18, size=per, replace=False) for c in centers])
+        k = np.unique(k)[:nu]
+    return np.sort(np.asarray(k, np.uint64)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("nu", [4095, 4096, 4097, 4098, 8191, 8192, 8193, 12289, 40961, 2 ** 20 + 1])
+@pytest.mark.parametrize("kind", ["k30", "stride", "clustered"])
+def test_radix_tree_tile_edges(gpu, orc, pool, kind, nu):
+    """S4 at unique-key counts around multiples of the tree tile (the last tile
+    takes the extra leaf; nodes cross tile edges), against the oracle."""
+    rw = gpu
+    L = rw.lib()
+    k = exact_keys(np.random.default_rng(nu + len(kind)), kind, nu)
+    assert k.size == nu and np.all(np.diff(k.astype(np.int64)) > 0)
+    d_k = rw.DeviceArray.from_numpy(k)
+    d_nu = rw.DeviceArray.from_numpy(np.array([nu], np.uint32))
+    ws = rw.Workspace(nu)
+    out = {"prefix_n": rw.DeviceArray(nu, np.uint8), "has_leaf_left": rw.DeviceArray(nu, np.uint8),
+           "has_leaf_right": rw.DeviceArray(nu, np.uint8), "left_child": rw.DeviceArray(nu, np.int32),
+           "parents": rw.DeviceArray(nu, np.int32)}
+    for _ in range(2):  # a second call over the first's outputs and lists
+        rw.check(L.rwh_radix_tree(ws.ptr, d_k.ptr, d_nu.ptr, nu, out["prefix_n"].ptr,
+                                  out["has_leaf_left"].ptr, out["has_leaf_right"].ptr,
+                                  out["left_child"].ptr, out["parents"].ptr, None))
+        rw.check(L.rwh_stream_sync(None))
+        want = orc.radix_tree(pool, k)
+        for name, arr in out.items():
+            assert np.array_equal(arr.numpy(nu - 1), want[name]), name
 
 
 def test_repeated_builds_are_identical(gpu):
