@@ -116,7 +116,8 @@ int rwh_unique_u32(rwh_workspace* ws, const uint32_t* d_sorted, uint32_t n, uint
  * Radix-tree nodes i in [0, n_unique-1) over the unique codes; n_unique is
  * read from device memory; n_max bounds it (buffers hold n_max entries, and
  * n_max is at most the workspace's max_points).  The workspace holds the
- * per-tile lists of nodes that cross a tile edge. */
+ * per-tile lists of nodes that cross a tile edge.  The five node buffers are
+ * 16-byte aligned. */
 int rwh_radix_tree(rwh_workspace* ws, const uint32_t* d_codes, const uint32_t* d_n_unique, uint32_t n_max,
                    uint8_t* d_prefix_n, uint8_t* d_has_leaf_left, uint8_t* d_has_leaf_right,
                    int32_t* d_left_child, int32_t* d_parents, rwh_stream_t stream);

@@ -489,6 +489,9 @@ int rwh_radix_tree(rwh_workspace* ws, const uint32_t* d_codes, const uint32_t* d
   RWH_CHECK(d_codes && d_n_unique && d_prefix_n && d_has_leaf_left && d_has_leaf_right &&
                 d_left_child && d_parents,
             "null buffer");
+  RWH_CHECK(aligned16(d_prefix_n) && aligned16(d_has_leaf_left) && aligned16(d_has_leaf_right) &&
+                aligned16(d_left_child) && aligned16(d_parents),
+            "node buffers must be 16-byte aligned");
   TreeArgs a{d_codes, d_prefix_n, d_has_leaf_left, d_has_leaf_right, d_left_child, d_parents};
   launch_radix_tree(a, single_seg_dev(d_n_unique), n_max, ws->tree_list, static_cast<hipStream_t>(stream));
   return launched();

@@ -331,9 +331,9 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   constexpr int T = kTreeTile;
   __shared__ uint32_t s_ks[T + 2];  // keys lo-1 .. hi at [k - lo + 1]; then the split slots
   __shared__ uint8_t s_e[T + 2];    // delta(k, k+1) (clz) + 1 for k in [lo-1, hi-1] at [k - lo + 1]; 0 outside
-  __shared__ uint16_t s_lc[T];      // node lo + x: its split - lo
-  __shared__ uint16_t s_par[T];     // node lo + x: its parent - lo (0xffff: not formed here)
-  __shared__ uint8_t s_meta[T];     // node lo + x: 0x80 formed here | leaf left << 6 | leaf right << 5 | prefix
+  // node lo + x: its split - lo | (0x80 formed here | leaf left << 6 | leaf right << 5 | prefix) << 16
+  __shared__ alignas(16) uint32_t s_rec[T];
+  __shared__ alignas(16) uint16_t s_par[T];  // node lo + x: its parent - lo (0xffff: not formed here)
   __shared__ uint32_t s_n_cross;
   const int tid = static_cast<int>(threadIdx.x);
   const uint32_t first = blockIdx.x * static_cast<uint32_t>(T);
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   for (int x = tid; x < T; x += kTreeBlock) {
     s_slot[x] = 0;
     s_par[x] = 0xffff;
-    s_meta[x] = 0;
+    s_rec[x] = 0;
   }
   if (tid == 0) s_n_cross = 0;
   __syncthreads();
@@ -383,11 +383,16 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   // Each wave grows the nodes of its block of leaves: a lane whose climb has
   // ended takes the block's next leaf, so the wave's lanes stay busy; a lane
   // climbs while it is the second child at a split, and stops at a split on
-  // the tile's edge (that parent is computed by k_tree_cross).  Nodes are
-  // held as tile offsets [na, nb] with their outer deltas biased by one:
-  // ea = delta(na-1, na) + 1, eb = delta(nb, nb+1) + 1.  A child leaves at the
-  // split its far end (+1) and that end's outer delta, so the sibling forms
-  // the parent without reading LDS: one exchange per level.
+  // the tile's edge (that parent is computed by k_tree_cross).  A node
+  // [na, nb] (tile offsets) is held as two words, each end with its outer
+  // delta biased by one:
+  //   pa = kEnd | (delta(na-1, na) + 1) << 16 | na
+  //   pb = kEnd | (delta(nb, nb+1) + 1) << 16 | (nb + 1)
+  // The biased deltas sit above the offsets, so pb > pa exactly when the
+  // node is its parent's left child (the deltas never tie below the root).
+  // A child leaves its far end's word at the split; the sibling takes it as
+  // its own far end, so a level costs one LDS exchange and no LDS read.
+  constexpr uint32_t kEnd = 0x80000000u;  // never 0: an empty slot is 0
   constexpr int kWaves = kTreeBlock / kWave;
   const int wave = tid / kWave;
   const int n_leaf = hi - lo;  // at most T + 1
@@ -395,79 +400,102 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   const int whi = wave == kWaves - 1 ? n_leaf : min(wlo + T / kWaves, n_leaf);
   const uint32_t n_split = static_cast<uint32_t>(n_leaf - 1);  // interior splits: offsets [0, n_leaf - 2]
   int next = wlo;
-  int na = 0, nb = 0, ea = 0, eb = 0;
+  uint32_t pa = 0, pb = 0;
   bool have = false;
   for (int guard = 0; guard < kTreeClimbGuard; ++guard) {
     const uint64_t need = ballot(!have);
     if (!have) {
       const int k = next + static_cast<int>(popc_below(need));
       if (k < whi) {
-        na = nb = k;
-        ea = s_e[k];
-        eb = s_e[k + 1];
+        pa = kEnd | static_cast<uint32_t>(s_e[k]) << 16 | static_cast<uint32_t>(k);
+        pb = kEnd | static_cast<uint32_t>(s_e[k + 1]) << 16 | static_cast<uint32_t>(k + 1);
         have = true;
       }
     }
     next += static_cast<int>(__popcll(need));
     if (ballot(have) == 0) break;
     if (!have) continue;
-    const bool right = eb > ea;
-    const int g = right ? nb : na - 1;
+    const bool right = pb > pa;
+    const uint32_t near = right ? pb : pa;
+    const int g = static_cast<int>(near & 0xffffu) - 1;  // the parent's split
     if (static_cast<uint32_t>(g) >= n_split) {  // the split is an edge of the tile
       have = false;
       continue;
     }
-    const uint32_t mine = right ? (static_cast<uint32_t>(na + 1) | static_cast<uint32_t>(ea) << 16)
-                                : (static_cast<uint32_t>(nb + 1) | static_cast<uint32_t>(eb) << 16);
-    const uint32_t other = atomicExch(&s_slot[g], mine);
+    const uint32_t other = atomicExch(&s_slot[g], right ? pa : pb);
     if (other == 0) {  // first child at the split: the sibling forms the parent
       have = false;
       continue;
     }
-    const int ep = right ? eb : ea;  // delta at the split + 1: the parent's prefix + 2
-    const int sib = static_cast<int>(other & 0xffffu) - 1;
-    const int es = static_cast<int>(other >> 16);
     if (right) {
-      nb = sib, eb = es;
+      pb = other;
     } else {
-      na = sib, ea = es;
+      pa = other;
     }
+    const int na = static_cast<int>(pa & 0xffffu), nb = static_cast<int>(pb & 0xffffu) - 1;
     const bool ll = g == na, lr = g + 1 == nb;
-    if ((ea | eb) == 0) {  // the complete root [0, n] (lo == 0): index 0, whose record is Q2's
+    if (((pa | pb) & 0x00ff0000u) == 0) {  // the complete root [0, n] (lo == 0): index 0, whose record is Q2's
       if (!ll) s_par[g] = 0;
       if (!lr) s_par[g + 1] = 0;
       s_par[0] = 0;
       have = false;
       continue;
     }
-    const int i = eb > ea ? nb : na;
-    s_meta[i] = static_cast<uint8_t>(0x80u | (ll ? 0x40u : 0u) | (lr ? 0x20u : 0u) | static_cast<uint32_t>(ep - 2));
-    s_lc[i] = static_cast<uint16_t>(g);
+    const int i = pb > pa ? nb : na;
+    const uint32_t ep = (near >> 16) & 0xffu;  // delta at the split + 1: the parent's prefix + 2
+    s_rec[i] = static_cast<uint32_t>(g) | (0x80u | (ll ? 0x40u : 0u) | (lr ? 0x20u : 0u) | (ep - 2)) << 16;
     if (!ll) s_par[g] = static_cast<uint16_t>(i);
     if (!lr) s_par[g + 1] = static_cast<uint16_t>(i);
   }
   __syncthreads();
 
-  // The tile's nodes, coalesced; those formed elsewhere get placeholders that
-  // k_tree_cross overwrites, and are listed for it.
+  // The tile's nodes, four per thread with 16- and 4-byte stores (the
+  // buffers are 16-byte aligned and lo + sg.base is a multiple of 4); those
+  // formed elsewhere get placeholders that k_tree_cross overwrites, and are
+  // listed for it.
   const int m = k_end - lo;
-  uint8_t* __restrict__ pfx = a.prefix_n + sg.base;
-  uint8_t* __restrict__ hll = a.has_leaf_left + sg.base;
-  uint8_t* __restrict__ hlr = a.has_leaf_right + sg.base;
-  int32_t* __restrict__ lc = a.left_child + sg.base;
-  int32_t* __restrict__ par = a.parent + sg.base;
-  for (int x = tid; x < m; x += kTreeBlock) {
-    const uint32_t meta = s_meta[x];
-    const uint32_t p = s_par[x];
-    const int i = lo + x;
-    pfx[i] = static_cast<uint8_t>(meta & 31u);
-    hll[i] = static_cast<uint8_t>((meta >> 6) & 1u);
-    hlr[i] = static_cast<uint8_t>((meta >> 5) & 1u);
-    lc[i] = lo + static_cast<int>(s_lc[x]);
-    par[i] = p == 0xffffu ? -1 : lo + static_cast<int>(p);
-    if (!(meta & 0x80u)) {
-      const uint32_t at = atomicAdd(&s_n_cross, 1u);
-      if (at < static_cast<uint32_t>(kTreeListCap)) my_list[1 + at] = static_cast<uint32_t>(i);
+  uint8_t* __restrict__ pfx = a.prefix_n + sg.base + lo;
+  uint8_t* __restrict__ hll = a.has_leaf_left + sg.base + lo;
+  uint8_t* __restrict__ hlr = a.has_leaf_right + sg.base + lo;
+  int32_t* __restrict__ lc = a.left_child + sg.base + lo;
+  int32_t* __restrict__ par = a.parent + sg.base + lo;
+  const auto list_node = [&](int x) {
+    const uint32_t at = atomicAdd(&s_n_cross, 1u);
+    if (at < static_cast<uint32_t>(kTreeListCap)) my_list[1 + at] = static_cast<uint32_t>(lo + x);
+  };
+  const auto parent_of = [&](uint32_t p) { return p == 0xffffu ? -1 : lo + static_cast<int>(p); };
+  for (int x = 4 * tid; x < m; x += 4 * kTreeBlock) {
+    if (x + 4 <= m) {
+      const uint4 r4 = *reinterpret_cast<const uint4*>(&s_rec[x]);
+      const uint2 p2 = *reinterpret_cast<const uint2*>(&s_par[x]);
+      // the four meta bytes, node x in the low byte
+      const uint32_t meta = __builtin_amdgcn_perm(r4.y, r4.x, 0x0c0c0602u) |
+                            __builtin_amdgcn_perm(r4.w, r4.z, 0x06020c0cu);
+      *reinterpret_cast<uint32_t*>(pfx + x) = meta & 0x1f1f1f1fu;
+      *reinterpret_cast<uint32_t*>(hll + x) = (meta >> 6) & 0x01010101u;
+      *reinterpret_cast<uint32_t*>(hlr + x) = (meta >> 5) & 0x01010101u;
+      *reinterpret_cast<int4*>(lc + x) =
+          make_int4(lo + static_cast<int>(r4.x & 0xffffu), lo + static_cast<int>(r4.y & 0xffffu),
+                    lo + static_cast<int>(r4.z & 0xffffu), lo + static_cast<int>(r4.w & 0xffffu));
+      *reinterpret_cast<int4*>(par + x) = make_int4(parent_of(p2.x & 0xffffu), parent_of(p2.x >> 16),
+                                                     parent_of(p2.y & 0xffffu), parent_of(p2.y >> 16));
+      const uint32_t cross = ~meta & 0x80808080u;
+      if (cross) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (cross & (0x80u << (8 * c))) list_node(x + c);
+      }
+    } else {
+      for (int c = 0; x + c < m; ++c) {
+        const uint32_t rec = s_rec[x + c];
+        const uint32_t meta = (rec >> 16) & 0xffu;
+        pfx[x + c] = static_cast<uint8_t>(meta & 31u);
+        hll[x + c] = static_cast<uint8_t>((meta >> 6) & 1u);
+        hlr[x + c] = static_cast<uint8_t>((meta >> 5) & 1u);
+        lc[x + c] = lo + static_cast<int>(rec & 0xffffu);
+        par[x + c] = parent_of(s_par[x + c]);
+        if (!(meta & 0x80u)) list_node(x + c);
+      }
     }
   }
   __syncthreads();
