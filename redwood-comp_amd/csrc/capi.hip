@@ -570,15 +570,17 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
                          st, true));
   }
   RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, stream));
-  RWH_TRY(rwh_radix_tree(ws, bufs->morton_keys, d_nu, n, bufs->prefix_n, bufs->has_leaf_left,
-                         bufs->has_leaf_right, bufs->left_child, bufs->parents, stream));
-  // S5 + S6 fused: the counts are computed inside the scan and written beside it
+  if (injected(4) || injected(5)) return RWH_ERR_INJECTED;
   if (n == 0) {
     RWH_HIP(hipMemsetAsync(d_no, 0, sizeof(uint32_t), static_cast<hipStream_t>(stream)));
   } else {
-    launch_edge_scan(bufs->prefix_n, bufs->parents, bufs->edge_count, bufs->edge_offset,
-                     single_seg_dev(d_nu), n, d_no, ws->lb_scan, ws->ctrl + 2 * kCtrlWords,
-                     static_cast<hipStream_t>(stream));
+    // S4 with S5: the tree kernels write the edge counts beside the tree
+    // (Morton codes are below 2^31); S6 scans them
+    TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
+                bufs->left_child, bufs->parents, bufs->edge_count};
+    launch_radix_tree(ta, single_seg_dev(d_nu), n, ws->tree_list, static_cast<hipStream_t>(stream));
+    launch_prefix_sum(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->lb_scan,
+                      ws->ctrl + 2 * kCtrlWords, static_cast<hipStream_t>(stream));
     RWH_TRY(launched());
   }
   RWH_TRY(rwh_octree(bufs, d_nu, d_no, n, min_coord, range, stream));
@@ -645,10 +647,10 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
                 ws->lb_unique, ws->ctrl + 1 * kCtrlWords, st);
   RWH_TRY(launched());
   TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
-              bufs->left_child, bufs->parents};
+              bufs->left_child, bufs->parents, bufs->edge_count};
   launch_radix_tree(ta, uni, n_index, ws->tree_list, st);
-  launch_edge_scan(bufs->prefix_n, bufs->parents, bufs->edge_count, bufs->edge_offset, uni, n_index,
-                   ws->n_oct(), ws->lb_scan, ws->ctrl + 2 * kCtrlWords, st);
+  launch_prefix_sum(bufs->edge_count, bufs->edge_offset, uni, n_index, ws->n_oct(), ws->lb_scan,
+                    ws->ctrl + 2 * kCtrlWords, st);
   OctArgs oa = oct_args(bufs, min_coord, range);
   oa.n_oct = ws->n_oct();
   oa.oct_base = ws->oct_base();

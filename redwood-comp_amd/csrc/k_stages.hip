@@ -464,6 +464,15 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
     if (at < static_cast<uint32_t>(kTreeListCap)) my_list[1 + at] = static_cast<uint32_t>(lo + x);
   };
   const auto parent_of = [&](uint32_t p) { return p == 0xffffu ? -1 : lo + static_cast<int>(p); };
+  // S5's count prefix/3 - parent's prefix/3 when the parent was formed here
+  // and is not node 0 (whose prefix is the Q2 record's); else a placeholder
+  // that k_tree_cross overwrites (the parent crosses the tile's edge, or is
+  // the root)
+  int32_t* __restrict__ ec = a.edge_count ? a.edge_count + sg.base + lo : nullptr;
+  const auto count_of = [&](uint32_t rec, uint32_t p) {
+    if (p == 0xffffu || lo + static_cast<int>(p) == 0) return 0;
+    return static_cast<int>((rec >> 16) & 31u) / 3 - static_cast<int>((s_rec[p] >> 16) & 31u) / 3;
+  };
   for (int x = 4 * tid; x < m; x += 4 * kTreeBlock) {
     if (x + 4 <= m) {
       const uint4 r4 = *reinterpret_cast<const uint4*>(&s_rec[x]);
@@ -479,6 +488,9 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
                     lo + static_cast<int>(r4.z & 0xffffu), lo + static_cast<int>(r4.w & 0xffffu));
       *reinterpret_cast<int4*>(par + x) = make_int4(parent_of(p2.x & 0xffffu), parent_of(p2.x >> 16),
                                                      parent_of(p2.y & 0xffffu), parent_of(p2.y >> 16));
+      if (ec)
+        *reinterpret_cast<int4*>(ec + x) = make_int4(count_of(r4.x, p2.x & 0xffffu), count_of(r4.y, p2.x >> 16),
+                                                     count_of(r4.z, p2.y & 0xffffu), count_of(r4.w, p2.y >> 16));
       const uint32_t cross = ~meta & 0x80808080u;
       if (cross) {
 #pragma unroll
@@ -494,6 +506,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
         hlr[x + c] = static_cast<uint8_t>((meta >> 5) & 1u);
         lc[x + c] = lo + static_cast<int>(rec & 0xffffu);
         par[x + c] = parent_of(s_par[x + c]);
+        if (ec) ec[x + c] = count_of(rec, s_par[x + c]);
         if (!(meta & 0x80u)) list_node(x + c);
       }
     }
@@ -541,7 +554,8 @@ __device__ void cross_node(const TreeArgs& a, uint32_t base, const uint32_t* __r
   const int bhi = k < 8 ? min((1 << (4 * k)) - 1, cutoff) : cutoff;
   const int l = group_last_true(in_range, blo, bhi, r, gshift);
   const int j = i + signed_step(l, m);
-  const int dnode = delta8(ci, codes[j]);
+  const uint32_t cj = codes[j];
+  const int dnode = delta8(ci, cj);
   const uint32_t thr2 = agree_below(dnode);
   const int s = group_last_true([&](int x) { return (ci ^ codes[i + signed_step(x, m)]) < thr2; }, 0, l - 1,
                                 r, gshift);
@@ -554,6 +568,11 @@ __device__ void cross_node(const TreeArgs& a, uint32_t base, const uint32_t* __r
     a.has_leaf_right[base + i] = lr ? 1 : 0;
     if (!ll) a.parent[base + gamma] = i;
     if (!lr) a.parent[base + gamma + 1] = i;
+    if (a.edge_count) {  // the internal children's counts: their prefixes from their end keys
+      const uint32_t c_lo = d > 0 ? ci : cj, c_hi = d > 0 ? cj : ci;
+      if (!ll) a.edge_count[base + gamma] = delta8(c_lo, codes[gamma]) / 3 - dnode / 3;
+      if (!lr) a.edge_count[base + gamma + 1] = delta8(codes[gamma + 1], c_hi) / 3 - dnode / 3;
+    }
   }
 }
 
@@ -576,6 +595,11 @@ __device__ void cross_root(const TreeArgs& a, uint32_t base, const uint32_t* __r
     a.parent[base] = 0;
     if (g_full != 0) a.parent[base + g_full] = 0;
     if (g_full + 1 < n) a.parent[base + g_full + 1] = 0;
+    if (a.edge_count) {  // count[0] = 0; the complete root's children count against node 0's Q2 prefix
+      a.edge_count[base] = 0;
+      if (g_full != 0) a.edge_count[base + g_full] = delta8(c0, codes[g_full]) / 3 - dnode / 3;
+      if (g_full + 1 < n) a.edge_count[base + g_full + 1] = delta8(codes[g_full + 1], codes[n]) / 3 - dnode / 3;
+    }
   }
 }
 
@@ -643,29 +667,11 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // a one-workgroup scan there; D4).  Single pass with decoupled look-back;
 // the segment's last tile writes the total (n_octree_nodes).
 // ---------------------------------------------------------------------------
-// kFromTree: the fused build's S5 + S6.  The counts are not read but computed
-// from the tree (the S5 formula above) and written beside the offsets, which
-// saves the count array's round trip through HBM and one launch.
-struct EdgeSource {
-  const uint8_t* prefix_n;
-  const int32_t* parent;
-  int32_t* edge_count;  // written
-};
-
-__device__ __forceinline__ uint32_t edge_count_at(const EdgeSource& es, const Seg& sg, uint32_t gi,
-                                                  uint32_t pfx, int32_t par) {
-  if (gi == sg.base) return 0u;  // count[0] = 0
-  return static_cast<uint32_t>(static_cast<int>(pfx / 3) -
-                               static_cast<int>(es.prefix_n[sg.base + par] / 3));
-}
-
-template <bool kFromTree, int BLOCK, int KPT>
-__global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict__ in,
-                                                           int32_t* __restrict__ out, SegArgs seg,
-                                                           uint32_t* __restrict__ total,
-                                                           uint64_t* __restrict__ lookback,
-                                                           uint32_t* __restrict__ ctrl,
-                                                           EdgeSource es) {
+template <int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                      SegArgs seg, uint32_t* __restrict__ total,
+                                                      uint64_t* __restrict__ lookback,
+                                                      uint32_t* __restrict__ ctrl) {
   constexpr int G = KPT / 4;
   constexpr uint32_t TILE = BLOCK * KPT;
   __shared__ uint32_t s_w[BLOCK / kWave];
@@ -693,36 +699,7 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict_
     uint32_t v[G][4];
     uint32_t gsum[G];
     const bool full = first + TILE <= lim;
-    if constexpr (kFromTree) {
-      if (full) {  // 16-byte parent loads, 4-byte prefix loads, gathered parent prefixes
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const uint32_t i0 = wbase + 4 * (g * kWave + lane);
-          const int4 par = *reinterpret_cast<const int4*>(es.parent + i0);
-          const uint32_t pf = *reinterpret_cast<const uint32_t*>(es.prefix_n + i0);
-          v[g][0] = edge_count_at(es, sg, i0 + 0, pf & 0xffu, par.x);
-          v[g][1] = edge_count_at(es, sg, i0 + 1, (pf >> 8) & 0xffu, par.y);
-          v[g][2] = edge_count_at(es, sg, i0 + 2, (pf >> 16) & 0xffu, par.z);
-          v[g][3] = edge_count_at(es, sg, i0 + 3, pf >> 24, par.w);
-          *reinterpret_cast<int4*>(es.edge_count + i0) =
-              make_int4(static_cast<int>(v[g][0]), static_cast<int>(v[g][1]),
-                        static_cast<int>(v[g][2]), static_cast<int>(v[g][3]));
-        }
-      } else {
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const uint32_t i0 = wbase + 4 * (g * kWave + lane);
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            v[g][c] = 0u;
-            if (i0 + c < lim) {
-              v[g][c] = edge_count_at(es, sg, i0 + c, es.prefix_n[i0 + c], es.parent[i0 + c]);
-              es.edge_count[i0 + c] = static_cast<int>(v[g][c]);
-            }
-          }
-        }
-      }
-    } else if (full) {  // plain 16-byte loads
+    if (full) {  // plain 16-byte loads
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int4 q = *reinterpret_cast<const int4*>(in + wbase + 4 * (g * kWave + lane));
@@ -776,32 +753,19 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict_
 
 // Launches the scan of the live count of each segment on tiles of BLOCK x KPT
 // (one cloud: the larger tile).
-template <bool kFromTree>
-void launch_scan(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
-                 uint64_t* lookback, uint32_t* ctrl, EdgeSource es, hipStream_t stream) {
+void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
+                       uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
   if (n_index == 0) return;
   seg.count_sub = 1;
   if (seg.n_segs == 1) {
     const uint32_t n_tiles = (n_index + kScanBigTile - 1) / kScanBigTile;
-    hipLaunchKernelGGL((k_prefix_sum<kFromTree, kScanBigBlock, kScanBigKpt>), dim3(n_tiles), dim3(kScanBigBlock),
-                       0, stream, in, out, seg, total, lookback, ctrl, es);
+    hipLaunchKernelGGL((k_prefix_sum<kScanBigBlock, kScanBigKpt>), dim3(n_tiles), dim3(kScanBigBlock), 0,
+                       stream, in, out, seg, total, lookback, ctrl);
     return;
   }
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL((k_prefix_sum<kFromTree, kScanBlock, kScanKpt>), dim3(n_tiles), dim3(kScanBlock), 0,
-                     stream, in, out, seg, total, lookback, ctrl, es);
-}
-
-void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
-                       uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
-  launch_scan<false>(in, out, seg, n_index, total, lookback, ctrl, EdgeSource{}, stream);
-}
-
-void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
-                      int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
-                      uint64_t* lookback, uint32_t* ctrl, hipStream_t stream) {
-  launch_scan<true>(nullptr, out, seg, n_index, total, lookback, ctrl,
-                    EdgeSource{prefix_n, parent, edge_count}, stream);
+  hipLaunchKernelGGL((k_prefix_sum<kScanBlock, kScanKpt>), dim3(n_tiles), dim3(kScanBlock), 0, stream, in,
+                     out, seg, total, lookback, ctrl);
 }
 
 }  // namespace rw

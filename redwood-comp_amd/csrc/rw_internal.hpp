@@ -150,6 +150,8 @@ struct TreeArgs {
   uint8_t* has_leaf_right;
   int32_t* left_child;
   int32_t* parent;
+  // optional: S5's edge counts, written beside the tree (keys below 2^31)
+  int32_t* edge_count = nullptr;
 };
 // seg.count = n_unique per segment with count_sub = 1, i.e. the live count is n_brt
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, uint32_t* list, hipStream_t stream);
@@ -158,11 +160,6 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // Exclusive scan of the live count (n_brt) per segment; total per segment.
 void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
                        uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
-// S5 + S6 in one pass: edge counts computed from prefix_n / parent, written,
-// and scanned into out.
-void launch_edge_scan(const uint8_t* prefix_n, const int32_t* parent, int32_t* edge_count,
-                      int32_t* out, SegArgs seg, uint32_t n_index, uint32_t* total,
-                      uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
 
 struct OctArgs {
   const int32_t* edge_offset;
