@@ -334,6 +334,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   // node lo + x: its split - lo | (0x80 formed here | leaf left << 6 | leaf right << 5 | prefix) << 16
   __shared__ alignas(16) uint32_t s_rec[T];
   __shared__ alignas(16) uint16_t s_par[T];  // node lo + x: its parent - lo (0xffff: not formed here)
+  __shared__ uint16_t s_pairs[kTreeBlock / kWave][kTreePairCap];  // per wave: left leaves of leaf pairs
   __shared__ uint32_t s_n_cross;
   const int tid = static_cast<int>(threadIdx.x);
   const uint32_t first = blockIdx.x * static_cast<uint32_t>(T);
@@ -399,16 +400,58 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   const int wlo = min(wave * (T / kWaves), n_leaf);
   const int whi = wave == kWaves - 1 ? n_leaf : min(wlo + T / kWaves, n_leaf);
   const uint32_t n_split = static_cast<uint32_t>(n_leaf - 1);  // interior splits: offsets [0, n_leaf - 2]
-  int next = wlo;
+  // First every leaf arrives at its parent's split, 64 at a time with no
+  // divergence; two leaves that meet form their parent at once (both
+  // children leaves) and it is queued for the climb.  A single leaf's sibling
+  // is an internal node, formed later, so every leaf is done here.
+  uint16_t* pairs = s_pairs[wave];
+  int n_pairs = 0;
+  for (int k0 = wlo; k0 < whi; k0 += kWave) {
+    const int k = k0 + static_cast<int>(tid % kWave);
+    bool queued = false;
+    int left = 0;  // the queued pair's left leaf
+    if (k < whi) {
+      const uint32_t e0 = s_e[k], e1 = s_e[k + 1];
+      const bool right = e1 > e0;
+      const int g = right ? k : k - 1;
+      if (static_cast<uint32_t>(g) < n_split) {
+        const uint32_t other = atomicExch(&s_slot[g], right ? (kEnd | e0 << 16 | static_cast<uint32_t>(k))
+                                                            : (kEnd | e1 << 16 | static_cast<uint32_t>(k + 1)));
+        if (other != 0) {  // the sibling is a leaf: the parent is [g, g + 1]
+          const uint32_t es = (other >> 16) & 0xffu;
+          const uint32_t ea = right ? e0 : es, eb = right ? es : e1;
+          const uint32_t ep = right ? e1 : e0;  // delta at the split + 1
+          if ((ea | eb) == 0) {  // the complete root [0, 1]: index 0, whose record is Q2's
+            s_par[0] = 0;
+          } else {
+            s_rec[eb > ea ? g + 1 : g] = static_cast<uint32_t>(g) | (0xe0u | (ep - 2)) << 16;
+            queued = true;
+            left = g;
+          }
+        }
+      }
+    }
+    const uint64_t qm = ballot(queued);
+    const int at = n_pairs + static_cast<int>(popc_below(qm));
+    if (queued && at < kTreePairCap) pairs[at] = static_cast<uint16_t>(left);
+    n_pairs += static_cast<int>(__popcll(qm));
+  }
+  __syncthreads();  // every leaf has arrived: from here on only internal nodes do
+
+  // Then the climb: a lane takes the next queued pair and climbs while it is
+  // the second child at a split.
+  n_pairs = min(n_pairs, kTreePairCap);
+  int next = 0;
   uint32_t pa = 0, pb = 0;
   bool have = false;
   for (int guard = 0; guard < kTreeClimbGuard; ++guard) {
     const uint64_t need = ballot(!have);
     if (!have) {
-      const int k = next + static_cast<int>(popc_below(need));
-      if (k < whi) {
+      const int q = next + static_cast<int>(popc_below(need));
+      if (q < n_pairs) {
+        const int k = pairs[q];  // the pair [k, k + 1]
         pa = kEnd | static_cast<uint32_t>(s_e[k]) << 16 | static_cast<uint32_t>(k);
-        pb = kEnd | static_cast<uint32_t>(s_e[k + 1]) << 16 | static_cast<uint32_t>(k + 1);
+        pb = kEnd | static_cast<uint32_t>(s_e[k + 2]) << 16 | static_cast<uint32_t>(k + 2);
         have = true;
       }
     }

@@ -72,6 +72,9 @@ constexpr int kTreeListStride = kTreeListCap + 1;  // count, then the nodes
 // loop bound of a wave's climbs: every iteration takes a leaf, forms a node
 // or ends a climb, and a tile has at most kTreeTile + 1 leaves and nodes
 constexpr int kTreeClimbGuard = 3 * kTreeTile + 8;
+// leaf pairs a wave queues: its block's kTreeTile / waves (+1) leaves pair up
+// at most half each, plus one pair across each end of the block
+constexpr int kTreePairCap = kTreeTile / (kTreeBlock / kWave) / 2 + 4;
 constexpr int kCrossGroup = 8;    // lanes per edge-crossing node
 constexpr int kCrossBlock = 256;  // four waves: 32 nodes at a time
 static_assert(kChunk % kTreeTile == 0, "a tree tile never straddles two clouds");
