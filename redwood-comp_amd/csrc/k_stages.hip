@@ -569,9 +569,40 @@ __device__ __forceinline__ int group_last_true(const P& p, int lo, int hi, int r
     const int step = (hi - lo + kCrossGroup - 1) / kCrossGroup;
     const int x = lo + (r + 1) * step;
     const bool t = x <= hi && p(x);
-    const uint32_t bits = static_cast<uint32_t>(ballot(t) >> gshift) & ((1u << kCrossGroup) - 1u);
+    const uint32_t bits = static_cast<uint32_t>(ballot(t) >> gshift) & kCrossMask;
     lo += __popc(bits) * step;
     hi = min(lo + step - 1, hi);
+  }
+  return lo;
+}
+
+// The same search for the predicate (ci ^ key(x)) < thr with key(x) =
+// codes[i + x*d] (m = d < 0 ? -1 : 0), which also keeps the probed keys at
+// the ends: on entry klo = key(lo) and khi = key(hi + 1) (if known); on
+// return klo = key(x) and khi = key(x + 1) when x < hi on entry, or the
+// entry khi otherwise.
+__device__ __forceinline__ int group_search(const uint32_t* __restrict__ codes, int i, int m, uint32_t ci,
+                                            uint32_t thr, int lo, int hi, uint32_t& klo, uint32_t& khi, int r,
+                                            int gshift) {
+  while (lo < hi) {
+    const int step = (hi - lo + kCrossGroup - 1) / kCrossGroup;
+    const int x = lo + (r + 1) * step;
+    uint32_t key = 0;
+    bool t = false;
+    if (x <= hi) {
+      key = codes[i + signed_step(x, m)];
+      t = (ci ^ key) < thr;
+    }
+    const int k = __popc(static_cast<uint32_t>(ballot(t) >> gshift) & kCrossMask);
+    // the last true probe (lane k-1) and the first false one (lane k)
+    const uint32_t key_t = __shfl(key, gshift + max(k - 1, 0), kWave);
+    const uint32_t key_f = __shfl(key, gshift + min(k, kCrossGroup - 1), kWave);
+    const int nlo = lo + k * step;
+    const int nhi = min(nlo + step - 1, hi);
+    if (k > 0) klo = key_t;
+    if (nhi < hi) khi = key_f;  // nhi + 1 = nlo + step is lane k's probe
+    lo = nlo;
+    hi = nhi;
   }
   return lo;
 }
@@ -588,20 +619,25 @@ __device__ void cross_node(const TreeArgs& a, uint32_t base, const uint32_t* __r
   // delta(i, i+x*d) > delta(i, i-d) for x in [0, l]; x <= cutoff keeps the probe in [0, n]
   const uint32_t thr = agree_below((d > 0 ? cm : cp) - 1);
   const int cutoff = d < 0 ? i : n - i;
-  const auto in_range = [&](int x) { return (ci ^ codes[i + signed_step(x, m)]) < thr; };
-  // bracket l: lane r probes 16^r (x = 1 always holds)
-  const int x16 = 1 << (4 * min(r, 7));
-  const bool t16 = r < 8 && x16 <= cutoff && in_range(x16);
-  const int k = max(__popc(static_cast<uint32_t>(ballot(t16) >> gshift) & ((1u << kCrossGroup) - 1u)), 1);
-  const int blo = 1 << (4 * (k - 1));
-  const int bhi = k < 8 ? min((1 << (4 * k)) - 1, cutoff) : cutoff;
-  const int l = group_last_true(in_range, blo, bhi, r, gshift);
+  // bracket l: lane r probes 2^(r * kCrossBracket) (x = 1 always holds)
+  const int x4 = 1 << (kCrossBracket * r);
+  uint32_t key4 = 0;
+  bool t4 = false;
+  if (x4 <= cutoff) {
+    key4 = codes[i + signed_step(x4, m)];
+    t4 = (ci ^ key4) < thr;
+  }
+  const int k = max(__popc(static_cast<uint32_t>(ballot(t4) >> gshift) & kCrossMask), 1);
+  const int blo = 1 << (kCrossBracket * (k - 1));
+  const int bhi = k < kCrossGroup ? min((1 << (kCrossBracket * k)) - 1, cutoff) : cutoff;
+  uint32_t cj = __shfl(key4, gshift + k - 1, kWave), unused = 0;
+  const int l = group_search(codes, i, m, ci, thr, blo, bhi, cj, unused, r, gshift);
   const int j = i + signed_step(l, m);
-  const uint32_t cj = codes[j];
   const int dnode = delta8(ci, cj);
-  const uint32_t thr2 = agree_below(dnode);
-  const int s = group_last_true([&](int x) { return (ci ^ codes[i + signed_step(x, m)]) < thr2; }, 0, l - 1,
-                                r, gshift);
+  // the split: the largest s in [0, l) with delta(i, i+s*d) > dnode; the keys
+  // at s and s + 1 are those at gamma and gamma + 1
+  uint32_t ks = ci, ks1 = cj;
+  const int s = group_search(codes, i, m, ci, agree_below(dnode), 0, l - 1, ks, ks1, r, gshift);
   const int gamma = i + signed_step(s, m) + min(d, 0);
   if (r == 0) {
     const bool ll = min(i, j) == gamma, lr = max(i, j) == gamma + 1;
@@ -613,8 +649,9 @@ __device__ void cross_node(const TreeArgs& a, uint32_t base, const uint32_t* __r
     if (!lr) a.parent[base + gamma + 1] = i;
     if (a.edge_count) {  // the internal children's counts: their prefixes from their end keys
       const uint32_t c_lo = d > 0 ? ci : cj, c_hi = d > 0 ? cj : ci;
-      if (!ll) a.edge_count[base + gamma] = delta8(c_lo, codes[gamma]) / 3 - dnode / 3;
-      if (!lr) a.edge_count[base + gamma + 1] = delta8(codes[gamma + 1], c_hi) / 3 - dnode / 3;
+      const uint32_t c_g = d > 0 ? ks : ks1, c_g1 = d > 0 ? ks1 : ks;  // keys at gamma, gamma + 1
+      if (!ll) a.edge_count[base + gamma] = delta8(c_lo, c_g) / 3 - dnode / 3;
+      if (!lr) a.edge_count[base + gamma + 1] = delta8(c_g1, c_hi) / 3 - dnode / 3;
     }
   }
 }

@@ -76,10 +76,12 @@ constexpr int kTreeClimbGuard = 3 * kTreeTile + 8;
 // at most half each, plus one pair across each end of the block
 constexpr int kTreePairCap = kTreeTile / (kTreeBlock / kWave) / 2 + 4;
 constexpr int kCrossGroup = 8;    // lanes per edge-crossing node
+constexpr uint32_t kCrossMask = (1u << kCrossGroup) - 1u;
+constexpr int kCrossBracket = 32 / kCrossGroup;  // the range bracket: lane r probes 2^(r * this)
 constexpr int kCrossBlock = 256;  // four waves: 32 nodes at a time
 static_assert(kChunk % kTreeTile == 0, "a tree tile never straddles two clouds");
 static_assert(kTreeTile < 0xffff, "LDS node offsets are 16-bit with 0xffff reserved");
-static_assert(kWave % kCrossGroup == 0 && kCrossGroup == 8, "the range bracket probes 16^r on 8 lanes");
+static_assert(kWave % kCrossGroup == 0 && kCrossBracket * (kCrossGroup - 1) <= 30, "bracket probes below 2^31");
 inline uint32_t tree_list_words(uint32_t n_index) {
   return ((n_index + kTreeTile - 1) / kTreeTile) * static_cast<uint32_t>(kTreeListStride);
 }
