@@ -79,22 +79,65 @@ def enqueue_stages(rw, ws, tb, n, stream, events=None):
         events[len(calls)].record(stream)
 
 
-def cpu_baseline(pts, budget_s=12.0):
+STAGES = ("S1_morton", "S2_sort", "S3_unique", "S4_radix_tree", "S5_edge_count", "S6_prefix_sum",
+          "S7_octree")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_runs(orc, pool, pts, max_runs, budget_s, warm=True):
+    """Median whole-build time (s) and median per-stage times (ms, the oracle's
+    own steady_clock) over up to max_runs builds within budget_s."""
+    if warm:
+        orc.build(pool, pts)
+    times, stages = [], []
+    t_all = time.perf_counter()
+    while len(times) < max_runs and (len(times) == 0 or (time.perf_counter() - t_all) < budget_s):
+        t0 = time.perf_counter()
+        _, _, ms = orc.build(pool, pts)
+        times.append(time.perf_counter() - t0)
+        stages.append(np.asarray(ms, np.float64))
+    st = np.median(np.stack(stages), axis=0)
+    return float(np.median(times)), {k: round(float(v), 3) for k, v in zip(STAGES, st)}, len(times)
+
+
+def cpu_baseline(rw, pts, budget_s=12.0):
+    """The pthread oracle on the host: the full 2^24 workload on up to 16
+    threads (the reported value), with per-stage times, one thread when a
+    build is short enough, and the reference's own CPU case (307,200 uniform
+    points, seed 114514) on all threads and on one."""
     orc = ge.load_oracle()
     threads = min(16, os.cpu_count() or 1)
     pool = orc.Pool(threads)
-    orc.build(pool, pts)  # warm-up
-    times = []
-    t_all = time.perf_counter()
-    while len(times) < 5 and (time.perf_counter() - t_all) < budget_s:
-        t0 = time.perf_counter()
-        orc.build(pool, pts)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": round(pts.shape[0] / t / 1e6, 3), "unit": "Mpts/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{pts.shape[0]} uniform points (the full workload), all 7 stages of the "
-                      f"pthread oracle, median of {len(times)} runs after 1 warm-up"}
+    t, stages, runs = cpu_runs(orc, pool, pts, 5, budget_s)
+    n = pts.shape[0]
+    out = {"value": round(n / t / 1e6, 3), "unit": "Mpts/s", "cores": threads, "kind": "port",
+           "sample": f"{n} uniform points (the full workload), all 7 stages of the pthread oracle, "
+                     f"median of {runs} runs after 1 warm-up",
+           "cpu_model": cpu_model(), "stages_ms": stages}
+    one = orc.Pool(1)
+    if t < 0.5:  # one thread at the full size: one run
+        t1, stages1, _ = cpu_runs(orc, one, pts, 1, 0.0, warm=False)
+        out["one_thread"] = {"value": round(n / t1 / 1e6, 3), "ms": round(t1 * 1e3, 2), "stages_ms": stages1}
+    small = rw.uniform_points(307200, 114514)
+    ts, stages_s, runs_s = cpu_runs(orc, pool, small, 10, 2.0)
+    ts1, stages_s1, runs_s1 = cpu_runs(orc, one, small, 10, 3.0)
+    out["reference_case"] = {
+        "points": int(small.shape[0]), "seed": 114514,
+        "all_threads": {"threads": threads, "value": round(small.shape[0] / ts / 1e6, 3),
+                        "ms": round(ts * 1e3, 3), "runs": runs_s, "stages_ms": stages_s},
+        "one_thread": {"value": round(small.shape[0] / ts1 / 1e6, 3), "ms": round(ts1 * 1e3, 3),
+                       "runs": runs_s1, "stages_ms": stages_s1}}
+    return out
 
 
 def batched_section(rw, reps=5):
@@ -295,7 +338,7 @@ def main():
     if not args.no_serving:
         out["serving"] = serving_section(rw)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pts)
+        out["cpu_baseline"] = cpu_baseline(rw, pts)
     print(json.dumps(out))
 
 
