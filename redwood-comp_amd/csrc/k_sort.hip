@@ -375,6 +375,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   __shared__ uint32_t s_whist[WAVES][256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_count[256];  // the tile's digit counts (valid keys)
+  __shared__ uint32_t s_pre[256];    // the tile's digit starts (+ prefixes with kTable)
   __shared__ uint32_t s_tmp[4];
   __shared__ Grab s_grab;  // the tile taken next
 
@@ -398,9 +399,23 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   uint32_t tile = kTable ? blockIdx.x : s_grab.index;
 
   uint32_t keys[KPT];
+  uint32_t rank[KPT];
+  uint32_t* wh = s_whist[wave];  // this wave's digit counters
+  // The segment of the tile whose keys are loaded, kept for its iteration:
+  // looked up there, after the scatter, its read would wait for every store
+  // of the scatter as well (one in-order counter for loads and stores).
+  // So are the digit starts of its segment and, with kTable, the tile's
+  // prefixes (thread d < 256: digit d), loaded before the keys and moved to
+  // s_pre where the keys are first used.  A kTable sort has one segment,
+  // found once.
+  Seg g = {};
+  uint32_t tdig = 0, texcl = 0;
+  const Seg g_table = kTable ? seg_of(seg, 0) : Seg{};
   const auto load_keys = [&](uint32_t t) {
     const uint32_t f = t * TILE;
-    const Seg g = seg_of(seg, f);
+    if (kTable && tid < 256) texcl = tile_excl[static_cast<size_t>(tid) * n_tiles + t];
+    g = kTable ? g_table : seg_of(seg, f);
+    if (tid < 256) tdig = digit_start[(static_cast<size_t>(g.id) * n_passes + pass) * 256 + tid];
     const uint32_t l = g.base + g.count;
     if (f >= l) return;  // a segment's padding
     const uint32_t wb = f + wave * (KPT * kWave) + lane;
@@ -416,34 +431,93 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       }
     }
   };
-  if (tile < n_tiles) load_keys(tile);
+  // The digit counts of the loaded tile (LDS atomics into s_count), taken
+  // before its iteration so that the aggregate is published first thing.
+  // For every tile after the first this runs behind the previous tile's
+  // scatter, in straight-line code: the wait for the keys then leaves the
+  // scatter's stores in flight.
+  const auto count_keys = [&](uint32_t t) {
+    if (tid < 256) s_pre[tid] = tdig;
+    const uint32_t f = t * TILE;
+    const uint32_t l = g.base + g.count;
+    if (f >= l) {  // a segment's padding: no keys were loaded
+      __builtin_amdgcn_s_waitcnt(0);
+      return;
+    }
+    const uint32_t wb = f + wave * (KPT * kWave) + lane;
+    // a wave whose first keys share their digit may hold ordered keys
+    const bool shared = wave_same((keys[0] >> shift) & dmask);
+    if (f + TILE <= l) {
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) hist_add(s_count, (keys[k] >> shift) & dmask, shared);
+    } else {
+      // the last tile of a segment.  Every key is read here (a select that
+      // changes nothing): on no path does a key load stay in flight into the
+      // iteration, where waiting for it would also wait for the stores
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) {
+        const bool in_seg = wb + k * kWave < l;
+        keys[k] = in_seg ? keys[k] : 0xffffffffu;
+        if (in_seg) hist_add(s_count, (keys[k] >> shift) & dmask, shared);
+      }
+    }
+  };
+  // Pass 0 ranks, per digit counter of the wave (LDS atomics; padding keys
+  // take no rank: they are not staged).  With kTable they are taken before
+  // the iteration, like the counts above.  One branch per tile: the plain
+  // ranks stay a tight loop.
+  const auto rank_first = [&](uint32_t t) {
+    if (kTable && tid < 256) s_pre[tid] = tdig + texcl;
+    const uint32_t f = t * TILE;
+    const uint32_t l = g.base + g.count;
+    if (f >= l) return;  // a segment's padding
+    const uint32_t wb = f + wave * (KPT * kWave) + lane;
+    const auto ranks = [&](auto check) {
+      if (f + TILE <= l) {
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) rank[k] = rank_add(wh, (keys[k] >> shift) & dmask, decltype(check)::value);
+      } else {
+        // every key read, as in count_keys
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) {
+          const bool in_seg = wb + k * kWave < l;
+          keys[k] = in_seg ? keys[k] : 0xffffffffu;
+          rank[k] = 0;
+          if (in_seg) rank[k] = rank_add(wh, (keys[k] >> shift) & dmask, decltype(check)::value);
+        }
+      }
+    };
+    // a wave whose first keys share their digit may hold ordered keys
+    if (wave_same((keys[0] >> shift) & dmask)) ranks(std::true_type{});
+    else ranks(std::false_type{});
+  };
+  if (tile < n_tiles) {
+    load_keys(tile);
+    if (!kTable) {
+      count_keys(tile);
+      __syncthreads();
+    } else {
+      rank_first(tile);
+    }
+  }
 
   while (tile < n_tiles) {
     const uint32_t first = tile * TILE;
-    const Seg sg = seg_of(seg, first);
+    const Seg sg = g;
     const uint32_t lim = sg.base + sg.count;
     const bool live = first < lim;  // else a segment's padding (uniform across the workgroup)
     const uint32_t n_valid = live ? min(TILE, lim - first) : 0u;
     const uint32_t wbase = first + wave * (KPT * kWave) + lane;
     if (live) {
       const uint32_t tile_in_seg = (first - sg.base) / TILE;
-      // The tile's digit counts first (LDS atomics), so that its aggregate
-      // is published, and its own look-back done, before the ranking:
-      // successors waiting on this tile are released early.
-      uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles
-      if (kTable) {
-        if (tid < 256) excl = tile_excl[static_cast<size_t>(tid) * n_tiles + tile];
-      } else {
-        // a wave whose first keys share their digit may hold ordered keys
-        const bool shared = wave_same((keys[0] >> shift) & dmask);
-#pragma unroll
-        for (int k = 0; k < KPT; ++k)
-          if (n_valid == TILE || wbase + k * kWave < lim) hist_add(s_count, (keys[k] >> shift) & dmask, shared);
-        __syncthreads();
-      }
+      // The tile's digit counts (taken before the iteration) are published,
+      // and its own look-back done, before the ranking: successors waiting
+      // on this tile are released early.
+      uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles (look-back)
       if (!kTable && tid < 256) {
         const uint32_t d = tid;
         const uint32_t cnt_valid = s_count[d];
+        s_count[d] = 0;  // for the next tile's counts (after two barriers)
         uint64_t* slot = &lookback[static_cast<size_t>(tile) * 256 + d];
         if (tile_in_seg == 0) {
           lb_store(slot, lb_pack(epoch, kIncl, cnt_valid));
@@ -455,23 +529,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       }
 
       // Ranks within the wave, per digit counter of the wave.
-      uint32_t* wh = s_whist[wave];
-      uint32_t rank[KPT];
-      // pass 0: a wave whose first keys share their digit may hold ordered keys
-      if (kFirst) {
-        // padding keys take no rank: they are not staged.  One branch per
-        // tile: the plain ranks stay a tight loop.
-        const auto ranks = [&](auto check) {
-#pragma unroll
-          for (int k = 0; k < KPT; ++k) {
-            rank[k] = 0;
-            if (n_valid == TILE || wbase + k * kWave < lim)
-              rank[k] = rank_add(wh, (keys[k] >> shift) & dmask, decltype(check)::value);
-          }
-        };
-        if (wave_same((keys[0] >> shift) & dmask)) ranks(std::true_type{});
-        else ranks(std::false_type{});
-      }
+      if (kFirst && !kTable) rank_first(tile);
 #pragma unroll
       for (int k = 0; k < KPT; ++k) {
         const uint32_t d = (keys[k] >> shift) & dmask;
@@ -529,8 +587,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
           s_whist[w][tid] = run;
           run += t;
         }
-        s_gbase[tid] = sg.base + digit_start[(static_cast<size_t>(sg.id) * n_passes + pass) * 256 + tid] +
-                       excl - tstart;
+        s_gbase[tid] = sg.base + s_pre[tid] + excl - tstart;
       }
       __syncthreads();
 
@@ -540,19 +597,34 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
         const uint32_t d = (keys[k] >> shift) & dmask;
         if (!kFirst || n_valid == TILE || wbase + k * kWave < lim) s_stage[wh[d] + rank[k]] = keys[k];
       }
+      // from here on only this wave uses its counters: cleared for the next tile
+      for (uint32_t i = lane; i < 256; i += kWave) wh[i] = 0;
     }
     // The next tile: taken now, its keys loading during the scatter.
     if (!kTable && tid == 0) s_grab = grab_tiles(ctrl, n_grabs);
     __syncthreads();
     const uint32_t next = kTable ? tile + gridDim.x : s_grab.index;
-    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
-    if (tid < 256) s_count[tid] = 0;
     if (next < n_tiles) load_keys(next);
 
-    // Scatter: consecutive staged keys of one digit go to consecutive addresses.
-    for (uint32_t p = tid; p < n_valid; p += BLOCK) {
+    // Scatter: consecutive staged keys of one digit go to consecutive
+    // addresses.  A whole tile's loop is unrolled, so the next tile's first
+    // key waits only for its own load, not for these stores.
+    const auto put = [&](uint32_t p) {
       const uint32_t key = s_stage[p];
       out[s_gbase[(key >> shift) & dmask] + p] = key;
+    };
+    // (the next tile's counts, or pass 0 ranks, follow in each branch: after
+    // a merge the wait for its keys would have to assume that no store is in
+    // flight)
+    if (n_valid == TILE) {
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) put(tid + k * BLOCK);
+      if (!kTable && next < n_tiles) count_keys(next);
+      if (kTable && next < n_tiles) rank_first(next);
+    } else {
+      for (uint32_t p = tid; p < n_valid; p += BLOCK) put(p);
+      if (!kTable && next < n_tiles) count_keys(next);
+      if (kTable && next < n_tiles) rank_first(next);
     }
     __syncthreads();  // the staged tile and the counters are reused
     tile = next;
