@@ -97,6 +97,7 @@ struct rwh_workspace {
   uint32_t* counts = nullptr;  // n_unique[max_segs], n_oct[max_segs]
   uint32_t* tables = nullptr;  // seg_base, seg_n, oct_base, oct_cap [max_segs each], chunk_seg
   uint32_t* tree_list = nullptr;  // radix tree: edge-crossing nodes per tile
+  uint32_t* tree_sums = nullptr;  // radix tree: edge counts summed per tile (build path)
   uint32_t* h_stage = nullptr;  // pinned staging of tables
   uint32_t* h_counts = nullptr;  // pinned counts read-back
   hipEvent_t upload_done = nullptr;
@@ -366,6 +367,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   const size_t o_cnt = take(sizeof(uint32_t) * 2ull * max_segments);
   const size_t o_tab = take(sizeof(uint32_t) * ws->table_words());
   const size_t o_tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
+  const size_t o_ts = take(sizeof(uint32_t) * static_cast<size_t>(tree_tiles(mp)));
   ws->bytes = off;
   char* base = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), off);
@@ -386,6 +388,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   ws->counts = reinterpret_cast<uint32_t*>(base + o_cnt);
   ws->tables = reinterpret_cast<uint32_t*>(base + o_tab);
   ws->tree_list = reinterpret_cast<uint32_t*>(base + o_tl);
+  ws->tree_sums = reinterpret_cast<uint32_t*>(base + o_ts);
   // Look-back words, histograms and counters must start at zero; later
   // launches keep them consistent by themselves.
   e = hipMemset(base, 0, off);
@@ -575,12 +578,13 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
     RWH_HIP(hipMemsetAsync(d_no, 0, sizeof(uint32_t), static_cast<hipStream_t>(stream)));
   } else {
     // S4 with S5: the tree kernels write the edge counts beside the tree
-    // (Morton codes are below 2^31); S6 scans them
+    // (Morton codes are below 2^31) and their sums per tree tile; S6 scans
+    // the counts, each tile's prefix from those sums
     TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
-                bufs->left_child, bufs->parents, bufs->edge_count};
+                bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums};
     launch_radix_tree(ta, single_seg_dev(d_nu), n, ws->tree_list, static_cast<hipStream_t>(stream));
-    launch_prefix_sum(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->lb_scan,
-                      ws->ctrl + 2 * kCtrlWords, static_cast<hipStream_t>(stream));
+    launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->tree_sums,
+                            static_cast<hipStream_t>(stream));
     RWH_TRY(launched());
   }
   RWH_TRY(rwh_octree(bufs, d_nu, d_no, n, min_coord, range, stream));
@@ -647,10 +651,9 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
                 ws->lb_unique, ws->ctrl + 1 * kCtrlWords, st);
   RWH_TRY(launched());
   TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
-              bufs->left_child, bufs->parents, bufs->edge_count};
+              bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums};
   launch_radix_tree(ta, uni, n_index, ws->tree_list, st);
-  launch_prefix_sum(bufs->edge_count, bufs->edge_offset, uni, n_index, ws->n_oct(), ws->lb_scan,
-                    ws->ctrl + 2 * kCtrlWords, st);
+  launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, uni, n_index, ws->n_oct(), ws->tree_sums, st);
   OctArgs oa = oct_args(bufs, min_coord, range);
   oa.n_oct = ws->n_oct();
   oa.oct_base = ws->oct_base();

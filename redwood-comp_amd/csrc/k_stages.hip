@@ -336,6 +336,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   __shared__ alignas(16) uint16_t s_par[T];  // node lo + x: its parent - lo (0xffff: not formed here)
   __shared__ uint16_t s_pairs[kTreeBlock / kWave][kTreePairCap];  // per wave: left leaves of leaf pairs
   __shared__ uint32_t s_n_cross;
+  __shared__ uint32_t s_count_sum;  // the tile's edge counts, summed (with a.tile_sum)
   const int tid = static_cast<int>(threadIdx.x);
   const uint32_t first = blockIdx.x * static_cast<uint32_t>(T);
   const Seg sg = seg_of(seg, first);
@@ -378,7 +379,10 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
     s_par[x] = 0xffff;
     s_rec[x] = 0;
   }
-  if (tid == 0) s_n_cross = 0;
+  if (tid == 0) {
+    s_n_cross = 0;
+    s_count_sum = 0;
+  }
   __syncthreads();
 
   // Each wave grows the nodes of its block of leaves: a lane whose climb has
@@ -516,6 +520,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
     if (p == 0xffffu || lo + static_cast<int>(p) == 0) return 0;
     return static_cast<int>((rec >> 16) & 31u) / 3 - static_cast<int>((s_rec[p] >> 16) & 31u) / 3;
   };
+  uint32_t count_sum = 0;  // this thread's edge counts (placeholders add 0)
   for (int x = 4 * tid; x < m; x += 4 * kTreeBlock) {
     if (x + 4 <= m) {
       const uint4 r4 = *reinterpret_cast<const uint4*>(&s_rec[x]);
@@ -531,9 +536,12 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
                     lo + static_cast<int>(r4.z & 0xffffu), lo + static_cast<int>(r4.w & 0xffffu));
       *reinterpret_cast<int4*>(par + x) = make_int4(parent_of(p2.x & 0xffffu), parent_of(p2.x >> 16),
                                                      parent_of(p2.y & 0xffffu), parent_of(p2.y >> 16));
-      if (ec)
-        *reinterpret_cast<int4*>(ec + x) = make_int4(count_of(r4.x, p2.x & 0xffffu), count_of(r4.y, p2.x >> 16),
-                                                     count_of(r4.z, p2.y & 0xffffu), count_of(r4.w, p2.y >> 16));
+      if (ec) {
+        const int4 c4 = make_int4(count_of(r4.x, p2.x & 0xffffu), count_of(r4.y, p2.x >> 16),
+                                  count_of(r4.z, p2.y & 0xffffu), count_of(r4.w, p2.y >> 16));
+        *reinterpret_cast<int4*>(ec + x) = c4;
+        count_sum += static_cast<uint32_t>(c4.x + c4.y + c4.z + c4.w);
+      }
       const uint32_t cross = ~meta & 0x80808080u;
       if (cross) {
 #pragma unroll
@@ -549,13 +557,25 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
         hlr[x + c] = static_cast<uint8_t>((meta >> 5) & 1u);
         lc[x + c] = lo + static_cast<int>(rec & 0xffffu);
         par[x + c] = parent_of(s_par[x + c]);
-        if (ec) ec[x + c] = count_of(rec, s_par[x + c]);
+        if (ec) {
+          const int cc = count_of(rec, s_par[x + c]);
+          ec[x + c] = cc;
+          count_sum += static_cast<uint32_t>(cc);
+        }
         if (!(meta & 0x80u)) list_node(x + c);
       }
     }
   }
+  if (a.tile_sum) {
+    const uint32_t ws = wave_sum(count_sum);
+    if (tid % kWave == 0 && ws != 0) atomicAdd(&s_count_sum, ws);
+  }
   __syncthreads();
-  if (tid == 0) my_list[0] = min(s_n_cross, static_cast<uint32_t>(kTreeListCap));
+  if (tid == 0) {
+    my_list[0] = min(s_n_cross, static_cast<uint32_t>(kTreeListCap));
+    // k_tree_cross adds the counts it writes over this tile's placeholders
+    if (a.tile_sum) a.tile_sum[blockIdx.x] = s_count_sum;
+  }
 }
 
 // Largest x in [lo, hi] with p(x), for p true at lo and true-then-false over
@@ -607,6 +627,13 @@ __device__ __forceinline__ int group_search(const uint32_t* __restrict__ codes, 
   return lo;
 }
 
+// The edge count of node gi (index space) over k_tree_local's placeholder 0,
+// added to its tile's sum when the tile sums are kept.
+__device__ __forceinline__ void put_count(const TreeArgs& a, uint32_t gi, int count) {
+  a.edge_count[gi] = count;
+  if (a.tile_sum && count != 0) atomicAdd(&a.tile_sum[gi / kTreeTile], static_cast<uint32_t>(count));
+}
+
 // Node i (i >= 1) of a tree over keys below 2^31, by one group: its range
 // end and split with the reference's predicates, then its record and its
 // internal children's parent entries (lane 0 of the group).
@@ -650,8 +677,8 @@ __device__ void cross_node(const TreeArgs& a, uint32_t base, const uint32_t* __r
     if (a.edge_count) {  // the internal children's counts: their prefixes from their end keys
       const uint32_t c_lo = d > 0 ? ci : cj, c_hi = d > 0 ? cj : ci;
       const uint32_t c_g = d > 0 ? ks : ks1, c_g1 = d > 0 ? ks1 : ks;  // keys at gamma, gamma + 1
-      if (!ll) a.edge_count[base + gamma] = delta8(c_lo, c_g) / 3 - dnode / 3;
-      if (!lr) a.edge_count[base + gamma + 1] = delta8(c_g1, c_hi) / 3 - dnode / 3;
+      if (!ll) put_count(a, base + gamma, delta8(c_lo, c_g) / 3 - dnode / 3);
+      if (!lr) put_count(a, base + gamma + 1, delta8(c_g1, c_hi) / 3 - dnode / 3);
     }
   }
 }
@@ -677,8 +704,8 @@ __device__ void cross_root(const TreeArgs& a, uint32_t base, const uint32_t* __r
     if (g_full + 1 < n) a.parent[base + g_full + 1] = 0;
     if (a.edge_count) {  // count[0] = 0; the complete root's children count against node 0's Q2 prefix
       a.edge_count[base] = 0;
-      if (g_full != 0) a.edge_count[base + g_full] = delta8(c0, codes[g_full]) / 3 - dnode / 3;
-      if (g_full + 1 < n) a.edge_count[base + g_full + 1] = delta8(codes[g_full + 1], codes[n]) / 3 - dnode / 3;
+      if (g_full != 0) put_count(a, base + g_full, delta8(c0, codes[g_full]) / 3 - dnode / 3);
+      if (g_full + 1 < n) put_count(a, base + g_full + 1, delta8(codes[g_full + 1], codes[n]) / 3 - dnode / 3);
     }
   }
 }
@@ -744,29 +771,37 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 
 // ---------------------------------------------------------------------------
 // S6  exclusive prefix sum (reference: apps/tree/cuda/06_prefix_sum.cu:26-37,
-// a one-workgroup scan there; D4).  Single pass with decoupled look-back;
-// the segment's last tile writes the total (n_octree_nodes).
+// a one-workgroup scan there; D4).  Single pass; the segment's last tile
+// writes the total (n_octree_nodes).  A tile's prefix over the segment's
+// earlier tiles comes from a decoupled look-back, or (kTiles) from the sums
+// of the earlier tree tiles that the radix tree kernels left (tile_sum).
 // ---------------------------------------------------------------------------
-template <int BLOCK, int KPT>
+template <int BLOCK, int KPT, bool kTiles = false>
 __global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict__ in, int32_t* __restrict__ out,
                                                       SegArgs seg, uint32_t* __restrict__ total,
                                                       uint64_t* __restrict__ lookback,
-                                                      uint32_t* __restrict__ ctrl) {
+                                                      uint32_t* __restrict__ ctrl,
+                                                      const uint32_t* __restrict__ tile_sum) {
   constexpr int G = KPT / 4;
   constexpr uint32_t TILE = BLOCK * KPT;
+  static_assert(!kTiles || TILE % kTreeTile == 0, "a scan tile holds whole tree tiles");
   __shared__ uint32_t s_w[BLOCK / kWave];
+  __shared__ uint32_t s_w2[BLOCK / kWave];
   __shared__ uint32_t s_bcast;
   __shared__ Grab s_grab;
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave, lane = tid % kWave;
-  if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
-  __syncthreads();
-  const uint32_t epoch = s_grab.epoch;
-  // One tile per workgroup, in grab order: a tile only ever waits on tiles
-  // whose workgroups took their grab earlier and are therefore running, so
-  // the look-back spin ends whatever the residency of the grid.
+  if (!kTiles) {
+    if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
+    __syncthreads();
+  }
+  const uint32_t epoch = kTiles ? 0u : s_grab.epoch;
+  // One tile per workgroup.  With the look-back, in grab order: a tile only
+  // ever waits on tiles whose workgroups took their grab earlier and are
+  // therefore running, so the look-back spin ends whatever the residency of
+  // the grid.
   {
-    const uint32_t tile = s_grab.index;
+    const uint32_t tile = kTiles ? blockIdx.x : s_grab.index;
     const uint32_t first = tile * TILE;
     const Seg sg = seg_of(seg, first);
     const uint32_t lim = sg.base + sg.count;
@@ -793,6 +828,12 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict_
         for (int c = 0; c < 4; ++c) v[g][c] = (i0 + c < lim) ? static_cast<uint32_t>(in[i0 + c]) : 0u;
       }
     }
+    // kTiles: the segment's tree tiles before this tile, summed by the block
+    uint32_t before = 0;
+    if (kTiles) {
+      const uint32_t t0 = sg.base / kTreeTile, nt = (first - sg.base) / kTreeTile;
+      for (uint32_t j = tid; j < nt; j += BLOCK) before += tile_sum[t0 + j];
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) gsum[g] = v[g][0] + v[g][1] + v[g][2] + v[g][3];
     uint32_t wave_run = 0;
@@ -806,7 +847,12 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_sum(const int32_t* __restrict_
     uint32_t tile_total;
     const uint32_t wexcl = block_excl_scan<BLOCK>(lane == 0 ? wave_run : 0u, s_w, &tile_total);
     const uint32_t wave_off = __shfl(wexcl, 0, kWave);
-    const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+    uint32_t excl;
+    if (kTiles) {
+      block_excl_scan<BLOCK>(before, s_w2, &excl);
+    } else {
+      excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+    }
     const uint32_t base = excl + wave_off;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -840,12 +886,23 @@ void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_
   if (seg.n_segs == 1) {
     const uint32_t n_tiles = (n_index + kScanBigTile - 1) / kScanBigTile;
     hipLaunchKernelGGL((k_prefix_sum<kScanBigBlock, kScanBigKpt>), dim3(n_tiles), dim3(kScanBigBlock), 0,
-                       stream, in, out, seg, total, lookback, ctrl);
+                       stream, in, out, seg, total, lookback, ctrl, nullptr);
     return;
   }
   const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL((k_prefix_sum<kScanBlock, kScanKpt>), dim3(n_tiles), dim3(kScanBlock), 0, stream, in,
-                     out, seg, total, lookback, ctrl);
+                     out, seg, total, lookback, ctrl, nullptr);
+}
+
+// (No chain between tiles here, so the smaller tile of any cloud layout,
+// which keeps more of them in flight, is also the faster one for one cloud.)
+void launch_prefix_sum_tiles(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
+                             uint32_t* total, const uint32_t* tile_sum, hipStream_t stream) {
+  if (n_index == 0) return;
+  seg.count_sub = 1;
+  const uint32_t n_tiles = (n_index + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((k_prefix_sum<kScanBlock, kScanKpt, true>), dim3(n_tiles), dim3(kScanBlock), 0, stream,
+                     in, out, seg, total, nullptr, nullptr, tile_sum);
 }
 
 }  // namespace rw

@@ -157,6 +157,9 @@ struct TreeArgs {
   int32_t* parent;
   // optional: S5's edge counts, written beside the tree (keys below 2^31)
   int32_t* edge_count = nullptr;
+  // optional with edge_count: per tree tile (kTreeTile nodes of index space)
+  // the sum of its nodes' edge counts, for a scan without look-back
+  uint32_t* tile_sum = nullptr;
 };
 // seg.count = n_unique per segment with count_sub = 1, i.e. the live count is n_brt
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, uint32_t* list, hipStream_t stream);
@@ -165,6 +168,11 @@ void launch_edge_count(const uint8_t* prefix_n, const int32_t* parent, int32_t* 
 // Exclusive scan of the live count (n_brt) per segment; total per segment.
 void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
                        uint32_t* total, uint64_t* lookback, uint32_t* ctrl, hipStream_t stream);
+// The same scan of edge counts whose tree tile sums (TreeArgs::tile_sum) are
+// known: each tile adds up the sums of the segment's earlier tree tiles.
+void launch_prefix_sum_tiles(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
+                             uint32_t* total, const uint32_t* tile_sum, hipStream_t stream);
+inline uint32_t tree_tiles(uint32_t n_index) { return (n_index + kTreeTile - 1) / kTreeTile; }
 
 struct OctArgs {
   const int32_t* edge_offset;
