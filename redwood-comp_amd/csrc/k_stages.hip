@@ -99,6 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_unique(const uint32_t* __restrict__ i
   __shared__ uint32_t s_w[BLOCK / kWave];
   __shared__ uint32_t s_bcast;
   __shared__ Grab s_grab;
+  __shared__ uint32_t s_out[TILE];  // the tile's heads in output order
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave, lane = tid % kWave;
   if (tid == 0) s_grab = grab_tiles(ctrl, gridDim.x);
@@ -166,15 +167,20 @@ __global__ __launch_bounds__(BLOCK) void k_unique(const uint32_t* __restrict__ i
     uint32_t tile_total;
     const uint32_t wexcl = block_excl_scan<BLOCK>(lane == 0 ? wave_run : 0u, s_w, &tile_total);
     const uint32_t wave_off = __shfl(wexcl, 0, kWave);
-    const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
-    const uint32_t base = sg.base + excl + wave_off;
+    // The heads staged in LDS in output order (the look-back's barrier
+    // orders these stores before the reads), then written out by consecutive
+    // threads: the scattered head stores of a wave would each touch a
+    // quarter of a line.
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      uint32_t pos = base + gexcl[g];
+      uint32_t pos = wave_off + gexcl[g];
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        if ((flags[g] >> c) & 1u) out[pos++] = k[g][c];
+        if ((flags[g] >> c) & 1u) s_out[pos++] = k[g][c];
     }
+    const uint32_t excl = tile_prefix(tile_total, tile, tile_in_seg, epoch, lookback, &s_bcast);
+    uint32_t* __restrict__ dst = out + sg.base + excl;
+    for (uint32_t i = tid; i < tile_total; i += BLOCK) dst[i] = s_out[i];
     // the tile holding the segment's last key reports the count
     if (tid == 0 && lim - first <= TILE) n_unique[sg.id] = excl + tile_total;
   }
