@@ -363,6 +363,52 @@ def test_batch_equals_oracle_per_cloud(gpu, orc, pool, shape):
                 (nu, nb, no), o, f"cloud{s}")
 
 
+def distinct_cells(n, seed):
+    """n points in distinct cells of the 1024^3 lattice (cell centres): every
+    Morton code differs, so n_unique = n and the radix tree has n - 1 nodes."""
+    rng = np.random.default_rng(seed)
+    c = rng.choice(1 << 30, size=n, replace=False).astype(np.uint64)
+    p = np.empty((n, 4), np.float32)
+    p[:, 0] = (c & 1023).astype(np.float32) + 0.5
+    p[:, 1] = ((c >> 10) & 1023).astype(np.float32) + 0.5
+    p[:, 2] = (c >> 20).astype(np.float32) + 0.5
+    p[:, 3] = 0.0
+    return p
+
+
+# radix-tree node counts at the edges of the tree tiles (4096 nodes) and of
+# the scan tiles (8192 and 16384), where the per-tile edge-count sums meet
+TILE_EDGE_NODES = [4095, 4096, 4097, 8191, 8192, 8193, 12289, 16383, 16384, 16385, (1 << 20) + 1]
+
+
+@pytest.mark.parametrize("nb", TILE_EDGE_NODES)
+def test_build_tile_edge_sizes(gpu, orc, pool, nb):
+    rw = gpu
+    pts = distinct_cells(nb + 1, seed=nb)
+    print(f"case build tile edges n_brt={nb}", flush=True)
+    g_counts, g = rw.build_octree(pts, strict=False)
+    assert g_counts[0] == nb + 1
+    (nu, nb_o, no, rc), o, _ = orc.build(pool, pts)
+    assert rc == (-1 if no > rw.octree_capacity(nb + 1) else 0)
+    compare(rw, nb + 1, g_counts, g, (nu, nb_o, no), o, f"edges/{nb}")
+
+
+def test_batch_tile_edge_sizes(gpu, orc, pool):
+    rw = gpu
+    clouds = [distinct_cells(nb + 1, seed=7 * nb) for nb in TILE_EDGE_NODES[:-1]]
+    print("case batch tile edges", flush=True)
+    b = rw.Batch(clouds)
+    b.run()
+    res = b.read()
+    for s, c in enumerate(clouds):
+        (nu, nb, no, rc), o, _ = orc.build(pool, c)
+        r = res[s]
+        assert r["n_unique"] == c.shape[0]
+        g = dict(r, morton_keys=r["unique"])
+        compare(rw, c.shape[0], (r["n_unique"], max(r["n_unique"] - 1, 0), r["n_oct"]), g,
+                (nu, nb, no), o, f"cloud{s}")
+
+
 def test_batch_repeat_and_reuse(gpu):
     """The same workspace serves batches of different shapes back to back."""
     rw = gpu
