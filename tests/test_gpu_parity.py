@@ -208,12 +208,13 @@ def test_batch_sorts_many_tiles(gpu, orc, pool):
         assert np.array_equal(res[s]["unique"], u), s
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 4096, 4097, 100000])
+# sizes at the edges of the 16384-key compaction tiles, too
+@pytest.mark.parametrize("n", [0, 1, 2, 4096, 4097, 16383, 16384, 16385, 49153, 100000])
 def test_unique_u32(gpu, n):
     rw = gpu
     L = rw.lib()
     rng = np.random.default_rng(n)
-    for hi in (1, 3, 1 << 30):
+    for hi in (1, 3, 1 << 14, 1 << 30):
         k = np.sort(rng.integers(0, hi, size=n).astype(np.uint32))
         d_in = rw.DeviceArray.from_numpy(k) if n else rw.DeviceArray(1, np.uint32)
         d_out = rw.DeviceArray(max(n, 1), np.uint32)
