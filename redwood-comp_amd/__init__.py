@@ -106,7 +106,7 @@ HOST_ABI = {
     "rwt_server_destroy": (None, [_P]),
     "rwt_server_slots": (_U32, [_P]),
     "rwt_serve_load": (_I, [_P, _U32, _P, _P, C.c_double, _U32, C.c_uint64, _P]),
-    "rwt_serve_mix": (_I, [_P, _U32, _P, _P] + [_P] * 9 + [_P, _P]),
+    "rwt_serve_mix": (_I, [_P, _U32, _P, _P, _P, _P, _P]),
 }
 
 
@@ -202,6 +202,28 @@ def log_uniform_sizes(seed: int, count: int, lo_log2: float, hi_log2: float) -> 
     out = np.empty(count, dtype=np.uint32)
     host().rwt_log_uniform_sizes(seed, count, lo_log2, hi_log2, out.ctypes.data)
     return out
+
+
+def batch_mix_clouds(count: int = 512, seed: int = 7, lo_log2: float = 10, hi_log2: float = 20):
+    """The batched workload: count clouds with sizes log-uniform in
+    [2^lo, 2^hi] (seeded), even clouds uniform in [0,1024)^3, odd clouds a
+    16-cluster Gaussian mixture with sigma 10.24 (1% of the box).  Defaults:
+    512 clouds of 2^10..2^20 points, seed 7."""
+    sizes = log_uniform_sizes(seed, count, lo_log2, hi_log2)
+    clouds = [uniform_points(int(n), (seed << 20) + s) if s % 2 == 0 else
+              gmm_points(int(n), (seed << 20) + s, 16, 10.24) for s, n in enumerate(sizes)]
+    return sizes, clouds
+
+
+def serving_mix_clouds(count: int = 32, seed: int = 11, lo_log2: float = 12, hi_log2: float = 22):
+    """The serving workload's distinct requests: count clouds with sizes
+    log-uniform in [2^lo, 2^hi] (seeded), even clouds uniform, odd clouds the
+    clustered kind of the single-build workload (64-component Gaussian mixture,
+    sigma 20.48).  Defaults: 32 clouds of 2^12..2^22 points, seed 11."""
+    sizes = log_uniform_sizes(seed, count, lo_log2, hi_log2)
+    clouds = [uniform_points(int(n), (seed << 20) + s) if s % 2 == 0 else
+              gmm_points(int(n), (seed << 20) + s, 64, 20.48) for s, n in enumerate(sizes)]
+    return sizes, clouds
 
 
 # ---------------------------------------------------------------------------
@@ -411,14 +433,17 @@ class PinnedArray:
 
 def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
                  min_coord: float = MIN_COORD, range_: float = RANGE, link_leaf: bool = False,
-                 strict: bool = True):
+                 strict: bool = True, sort_variant: int | None = None):
     """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs).
     link_leaf also runs the opt-in leaf linking after S7.  strict=False returns the
     (capacity-clamped) outputs when n_octree_nodes exceeds the 0.6 n capacity
-    instead of raising."""
+    instead of raising.  sort_variant picks the sort's tile shape (a new
+    workspace's default otherwise)."""
     n = points.shape[0]
     cap = octree_capacity(n)
     ws = ws or Workspace(max(n, 1))
+    if sort_variant is not None:
+        ws.set_sort_variant(sort_variant)
     tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
     check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
     if link_leaf:
@@ -543,28 +568,32 @@ class Server:
             raise RwhError(rc, host().rwt_last_error().decode())
         return dict(zip(SERVE_STATS, stats.tolist()))
 
+    # the arrays a kept result carries, in rwt_serve_mix's field order
+    MIX_FIELDS = ("morton_keys_alt", "unique", "prefix_n", "has_leaf_left", "has_leaf_right",
+                  "left_child", "parents", "edge_count", "edge_offset", "children", "corner",
+                  "cell_size", "child_node_mask", "child_leaf_mask")
+
     def mix(self, clouds):
-        """All clouds submitted together, outputs kept; per-request results."""
+        """All clouds submitted together, outputs kept; per-request results
+        ((n_unique, n_brt, n_oct), arrays, batch size)."""
         k = len(clouds)
-        fields = {"morton_keys_alt": np.uint32, "unique": np.uint32, "prefix_n": np.uint8,
-                  "parents": np.int32, "edge_offset": np.int32, "children": np.int32,
-                  "corner": np.float32, "cell_size": np.float32, "child_node_mask": np.int32}
-        width = {"children": 8, "corner": 4}
         outs = []
         for c in clouds:
             n = c.shape[0]
             cap = max(octree_capacity(n), 1)
             o = {}
-            for f, dt in fields.items():
-                per_slot = f in ("children", "corner", "cell_size", "child_node_mask")
-                o[f] = np.zeros((cap if per_slot else max(n, 1)) * width.get(f, 1), dt)
+            for f in self.MIX_FIELDS:
+                src = "morton_keys" if f == "unique" else f
+                dt, width, per_point = TREE_FIELDS[src]
+                o[f] = np.zeros((max(n, 1) if per_point else cap) * width, dt)
             outs.append(o)
-        tables = [(C.c_void_p * k)(*[outs[r][f].ctypes.data for r in range(k)]) for f in fields]
+        table = (C.c_void_p * (len(self.MIX_FIELDS) * k))(
+            *[outs[r][f].ctypes.data for f in self.MIX_FIELDS for r in range(k)])
         ptrs = (C.c_void_p * k)(*[c.ctypes.data for c in clouds])
         sizes = np.array([c.shape[0] for c in clouds], np.uint32)
         counts = np.zeros(3 * k, np.uint32)
         bsz = np.zeros(k, np.uint32)
-        rc = host().rwt_serve_mix(self.ptr, k, ptrs, sizes.ctypes.data, *tables,
+        rc = host().rwt_serve_mix(self.ptr, k, ptrs, sizes.ctypes.data, table,
                                   counts.ctypes.data, bsz.ctypes.data)
         if rc != 0:
             raise RwhError(rc, host().rwt_last_error().decode())

@@ -164,29 +164,40 @@ int rwt_serve_load(void* server, uint32_t n_clouds, const float* const* clouds,
 // results of request r into the arrays at the r-th entry of each pointer
 // table (sizes as in rwt_dropin_build; counts3[r*3..]).  Returns 0, or 1 + r
 // for the first failed request.
+// Submits every cloud at once (keep_outputs) and copies each result back.
+// outs is field-major: outs[f * n_clouds + r] is request r's host array for
+// field f, in the order sorted, unique keys, prefix_n, has_leaf_left,
+// has_leaf_right, left_child, parents, edge_count, edge_offset, children,
+// corner, cell_size, child_node_mask, child_leaf_mask (sized as the AppData
+// buffers of that cloud).  counts3[3r..] = {n_unique, n_brt, n_octree_nodes};
+// batch_sizes[r] = clouds in the batch request r ran in.
 int rwt_serve_mix(void* server, uint32_t n_clouds, const float* const* clouds,
-                  const uint32_t* sizes, uint32_t* const* sorted, uint32_t* const* unique_keys,
-                  uint8_t* const* prefix_n, int32_t* const* parents, int32_t* const* edge_offset,
-                  int32_t* const* children, float* const* corner, float* const* cell_size,
-                  int32_t* const* child_node_mask, uint32_t* counts3, uint32_t* batch_sizes) {
+                  const uint32_t* sizes, void* const* outs, uint32_t* counts3, uint32_t* batch_sizes) {
   auto& s = *static_cast<redwood::serve::BuildServer*>(server);
   std::vector<std::future<redwood::serve::BuildResult>> f;
   for (uint32_t r = 0; r < n_clouds; ++r) f.push_back(s.submit(clouds[r], sizes[r], true));
   for (uint32_t r = 0; r < n_clouds; ++r) {
     try {
       const auto res = f[r].get();
-      const auto cp = [](const auto& v, auto* dst) {
+      int field = 0;
+      const auto cp = [&](const auto& v) {
+        void* dst = outs[static_cast<size_t>(field++) * n_clouds + r];
         if (!v.empty()) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
       };
-      cp(res.sorted, sorted[r]);
-      cp(res.unique_keys, unique_keys[r]);
-      cp(res.prefix_n, prefix_n[r]);
-      cp(res.parents, parents[r]);
-      cp(res.edge_offset, edge_offset[r]);
-      cp(res.children, children[r]);
-      cp(res.corner, corner[r]);
-      cp(res.cell_size, cell_size[r]);
-      cp(res.child_node_mask, child_node_mask[r]);
+      cp(res.sorted);
+      cp(res.unique_keys);
+      cp(res.prefix_n);
+      cp(res.has_leaf_left);
+      cp(res.has_leaf_right);
+      cp(res.left_child);
+      cp(res.parents);
+      cp(res.edge_count);
+      cp(res.edge_offset);
+      cp(res.children);
+      cp(res.corner);
+      cp(res.cell_size);
+      cp(res.child_node_mask);
+      cp(res.child_leaf_mask);
       counts3[3 * r] = res.n_unique;
       counts3[3 * r + 1] = res.n_unique > 0 ? res.n_unique - 1 : 0;
       counts3[3 * r + 2] = res.n_octree_nodes;

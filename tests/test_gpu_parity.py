@@ -8,32 +8,9 @@ comparison is still element by element against the oracle.
 import numpy as np
 import pytest
 
+from parity_helpers import bits, compare
+
 pytestmark = pytest.mark.gpu
-
-PER_POINT = ("prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parents",
-             "edge_count", "edge_offset")
-PER_SLOT = ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask")
-
-
-def bits(a):
-    a = np.ascontiguousarray(a)
-    return a.view(np.uint8)
-
-
-def compare(rw, n, g_counts, g, o_counts, o, tag=""):
-    nu, nb, no = g_counts
-    assert (nu, nb, no) == tuple(o_counts[:3]), (tag, g_counts, o_counts)
-    assert np.array_equal(g["morton_keys_alt"][:n], o["morton_keys_alt"][:n]), tag + " sorted"
-    assert np.array_equal(g["morton_keys"][:nu] if "morton_keys" in g else g["unique"][:nu],
-                          o["unique"][:nu]), tag + " unique"
-    for k in PER_POINT:
-        assert np.array_equal(g[k][:nb], o[k][:nb]), f"{tag} {k}"
-    rows = min(no, rw.octree_capacity(n))
-    for k in PER_SLOT:
-        assert np.array_equal(bits(g[k][:rows]), bits(o[k][:rows])), f"{tag} {k}"
-    if no == 0 and rw.octree_capacity(n) > 0 and nb > 0:
-        # D8: with no contributing node the root cell stays at slot 0
-        assert np.array_equal(bits(g["corner"][:1]), bits(o["corner"][:1]))
 
 
 def inputs(rw, kind, n, seed):
@@ -59,15 +36,19 @@ def inputs(rw, kind, n, seed):
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 100, 1000, 4095, 4096, 4097, 8193, 65537, 307200])
 @pytest.mark.parametrize("kind", ["uniform", "gmm", "dups", "lastfar", "firstfar"])
 def test_build_matches_oracle(gpu, orc, pool, n, kind):
+    """Whole builds with every compiled sort tile shape: the fused Morton +
+    histogram sweep counts pass 0 per tile of the selected shape (tiles of
+    4096, 8192 or 16384 keys), so each shape is its own code path."""
     rw = gpu
     pts = inputs(rw, kind, n, seed=n + 17)
-    print(f"case build kind={kind} n={n}", flush=True)
-    g_counts, g = rw.build_octree(pts, strict=False)
     (nu, nb, no, rc), o, _ = orc.build(pool, pts)
     # sparse clouds can need more octree nodes than the reference's 0.6 n slots:
     # both sides then report it and keep the first 0.6 n slots (D11)
     assert rc == (-1 if no > rw.octree_capacity(n) else 0)
-    compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}")
+    for v in range(len(rw.sort_variants())):
+        print(f"case build kind={kind} n={n} sort_variant={v}", flush=True)
+        g_counts, g = rw.build_octree(pts, strict=False, sort_variant=v)
+        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}")
 
 
 @pytest.mark.parametrize("mn,rg", [(0.0, 1024.0), (0.25, 0.5), (-64.0, 2.0 ** 20), (-3.7, 1000.0),
@@ -87,16 +68,26 @@ def test_build_box_matches_oracle(gpu, orc, pool, mn, rg):
     compare(rw, n, g_counts, g, (nu, nb, no), o, f"box {mn} {rg}")
 
 
+# the benchmark's shape (1024 x 16), the default (256 x 16) and the tuner's pick
+BENCH_SORT_VARIANT = 5
+
+
 @pytest.mark.parametrize("kind,n", [("uniform", 1 << 24), ("gmm", 1 << 24), ("gmm", 1 << 22)])
 def test_full_size_matches_oracle(gpu, orc, kind, n):
+    """The benchmarked configuration itself (2^24 uniform points, rwh_build_octree
+    with the 1024 x 16 sort tile: the fused sweep's 16384-key pass-0 table),
+    the clustered configurations, and the autotuner's shape for the size."""
     rw = gpu
-    print(f"case full kind={kind} n={n}", flush=True)
+    assert rw.sort_variants()[BENCH_SORT_VARIANT] == (1024, 16)
     pts = inputs(rw, kind, n, seed=114514 if kind == "uniform" else 1)
-    g_counts, g = rw.build_octree(pts)
     big = orc.Pool(16)
     (nu, nb, no, rc), o, _ = orc.build(big, pts)
     assert rc == 0
-    compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}")
+    ws = rw.Workspace(n)
+    for v in sorted({0, BENCH_SORT_VARIANT, rw.best_sort_variant(n)}):
+        print(f"case full kind={kind} n={n} sort_variant={v}", flush=True)
+        g_counts, g = rw.build_octree(pts, ws=ws, sort_variant=v)
+        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}")
 
 
 @pytest.mark.parametrize("n", [1, 3, 4, 1000, 4097, 123457, 1 << 20])

@@ -365,14 +365,12 @@ def test_octree_invariants(orc, pool, rw):
     assert np.all(owner >= 0)  # every slot has exactly one owner
     ch = o["children"][:no]
     mask = o["child_node_mask"][:no]
-    # mask bit set <=> that child slot holds a node index
-    for s in range(no):
-        for w in range(8):
-            assert bool(mask[s] >> w & 1) == (ch[s, w] != 0 or (s == 0 and False)) or ch[s, w] == 0
+    # a child slot whose mask bit is clear was never linked (stays 0)
+    for w in range(8):
+        assert np.all(ch[(mask >> w) & 1 == 0, w] == 0)
     assert np.all(o["child_leaf_mask"][:no] == 0)  # Q6
     # corner = decoded prefix of the owner's code at the slot's level; cell closed form
-    rng = np.random.default_rng(0)
-    for s in rng.choice(no, size=min(no, 3000), replace=False):
+    for s in range(no):
         i = owner[s]
         j = s - off[i]
         level = int(pn[i]) // 3 - j
@@ -381,6 +379,155 @@ def test_octree_invariants(orc, pool, rw):
         x, y, z = decode_numpy(prefix)
         assert o["corner"][s].tolist() == [x * 1.0, y * 1.0, z * 1.0, 1.0]
         assert o["cell_size"][s] * (1 << (level - root_level)) == 1024.0
+
+
+def slot_geometry(o, rows, root_level):
+    """Level and 30-bit Morton prefix of every octree slot, read back from its
+    corner and cell size (both pinned above by closed forms), for the box
+    [0, 1024): cell = 1024 / 2^(level - root_level), corner = grid origin."""
+    cell = o["cell_size"][:rows].astype(np.float64)
+    k = np.log2(1024.0 / cell)
+    assert np.all(k == np.round(k)), "cell sizes are powers of two of the box"
+    level = root_level + np.round(k).astype(np.int64)
+    g = o["corner"][:rows, :3]
+    gi = g.astype(np.int64)
+    assert np.all(g == gi), "corners are grid points"
+    code = (spread_magic(gi[:, 0]).astype(np.int64) | spread_magic(gi[:, 1]).astype(np.int64) << 1 |
+            spread_magic(gi[:, 2]).astype(np.int64) << 2)
+    return level, code
+
+
+def octant(code, level):
+    """Octant of a cell (or key) at `level` within its parent: the three Morton
+    bits of that level (the shift count taken mod 32 as the hardware does)."""
+    return (int(code) >> ((30 - 3 * int(level)) & 31)) & 7
+
+
+def contributing_ancestor(i, cnt, par):
+    """Nearest node at or above i with a non-zero edge count, walking parent[]
+    at most 31 steps (func_octree.hpp:83-92), for a walk that starts at i."""
+    rt, steps = i, 0
+    while cnt[rt] == 0:
+        rt = par[rt]
+        steps += 1
+        if steps > 30:
+            break
+    return rt
+
+
+def octree_clouds(rw):
+    out = [rw.uniform_points(20000, 7), rw.gmm_points(30000, 3, 8, 5.0),
+           rw.uniform_points(3000, 9)]
+    p = rw.uniform_points(5000, 11)
+    p[:, :3] = np.floor(p[:, :3] / 128) * 128 + 1  # dense lattice: few, long chains
+    out.append(p)
+    for kind in ("lastfar", "firstfar"):
+        p = rw.uniform_points(4000, 13)
+        p[:, :3] = p[:, :3] / 64.0 + (0.0 if kind == "lastfar" else 900.0)
+        p[-1, :3] = 1000.0 if kind == "lastfar" else 0.5
+        out.append(p)
+    return out
+
+
+def test_octree_links_follow_geometry(orc, pool, rw):
+    """Pins S7's child links and octant assignment (func_octree.hpp:56-111).
+    The expected rows are rebuilt from the pinned radix tree, edge counts and
+    offsets, with each link's octant read from the CHILD slot's own corner and
+    cell size (not from the oracle's shift of the radix node's code):
+      * slot off+j+1 of a chain links slot off+j;
+      * a chain's top links into the bottom slot of its nearest contributing
+        ancestor (Q5 walk), at the top's octant;
+      * collisions keep the largest child (D6), the mask ORs the octants.
+    Geometry: a child is one level below its parent slot, and its cell lies
+    inside the parent's cell except where the parent's prefix length is a
+    multiple of 3 (Q3: the octree level takes one bit more than the keys
+    share).  Slot 0 (Q4) is left out of the level checks."""
+    checked = q3 = 0
+    for pts in octree_clouds(rw):
+        (nu, nb, no, rc), o, _ = orc.build(pool, pts)
+        assert rc == 0 and nb > 1
+        rows = no
+        cnt, off = o["edge_count"][:nb], o["edge_offset"][:nb]
+        par, pn = o["parents"][:nb], o["prefix_n"][:nb]
+        root_level = int(pn[0]) // 3
+        level, code = slot_geometry(o, rows, root_level)
+        want = np.zeros((rows, 8), np.int64)
+        want_mask = np.zeros(rows, np.int64)
+
+        def link(parent_slot, child_slot, parent_node):
+            nonlocal q3
+            w = octant(code[child_slot], level[child_slot])
+            want[parent_slot, w] = max(want[parent_slot, w], child_slot)
+            want_mask[parent_slot] |= 1 << w
+            if parent_slot != 0:
+                assert level[child_slot] == level[parent_slot] + 1
+                sh = (30 - 3 * int(level[parent_slot])) & 31
+                if (code[child_slot] >> sh) != (code[parent_slot] >> sh):
+                    assert int(pn[parent_node]) % 3 == 0, "outside the parent only under Q3"
+                    q3 += 1
+
+        for i in range(1, nb):
+            for j in range(int(cnt[i]) - 1):
+                link(off[i] + j + 1, off[i] + j, i)
+            if cnt[i] > 0:
+                t = contributing_ancestor(int(par[i]), cnt, par)
+                link(int(off[t]), int(off[i]) + int(cnt[i]) - 1, t)
+        assert np.array_equal(o["children"][:rows], want)
+        assert np.array_equal(o["child_node_mask"][:rows], want_mask)
+        checked += rows
+    assert checked > 20000
+
+
+def test_link_leaf_follows_geometry(orc, pool, rw):
+    """Pins oracle_link_leaf (the reference's process_link_leaf,
+    func_octree.hpp:114-163 / 07_octree.cu:116-173, never launched there).
+    Every has_leaf flag of brt node i links key leaf = left_child + side into
+    the bottom slot of the nearest contributing node at or above i (or is
+    dropped when that slot lies outside the rows), at the octant of the
+    leaf's key one level below the bottom slot's level read from its cell
+    size; the child slot keeps the largest value (leaf indices and octree
+    slots alike, D6), child_leaf_mask stays 0 (set_leaf clears the bit,
+    func_octree.hpp:24).  The leaf's key lies in the bottom's cell except
+    under Q3, and one level below it (slot 0 aside, Q4)."""
+    links = q3 = 0
+    for pts in octree_clouds(rw):
+        (nu, nb, no, rc), o, _ = orc.build(pool, pts)
+        assert rc == 0
+        rows = no
+        codes = o["unique"][:nu]
+        cnt, off = o["edge_count"][:nb], o["edge_offset"][:nb]
+        par, pn, lc = o["parents"][:nb], o["prefix_n"][:nb], o["left_child"][:nb]
+        hl, hr = o["has_leaf_left"][:nb], o["has_leaf_right"][:nb]
+        root_level = int(pn[0]) // 3
+        level, code = slot_geometry(o, rows, root_level)
+        want = o["children"][:rows].astype(np.int64).copy()
+        dropped = 0
+        for i in range(nb):
+            for side, flag in ((0, hl[i]), (1, hr[i])):
+                if not flag:
+                    continue
+                leaf = int(lc[i]) + side
+                t = contributing_ancestor(i, cnt, par)
+                b = int(off[t])
+                if not 0 <= b < rows:
+                    dropped += 1
+                    continue
+                w = octant(codes[leaf], level[b] + 1)
+                want[b, w] = max(want[b, w], leaf)
+                links += 1
+                if b != 0:
+                    assert level[b] == int(pn[i]) // 3  # the leaf sits one level below the bottom
+                    sh = (30 - 3 * int(level[b])) & 31
+                    if (int(codes[leaf]) >> sh) != (code[b] >> sh):
+                        assert int(pn[t]) % 3 == 0, "outside the bottom's cell only under Q3"
+                        q3 += 1
+        n_flags = int(hl.sum()) + int(hr.sum())
+        assert links <= n_flags + links and dropped <= n_flags
+        orc.link_leaf(pool, o, (nu, nb, no))
+        assert np.array_equal(o["children"][:rows], want)
+        assert np.all(o["child_leaf_mask"][:rows] == 0)
+        assert n_flags > 0
+    assert links > 10000
 
 
 def test_build_is_thread_count_invariant(orc, rw):

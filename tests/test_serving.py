@@ -45,10 +45,11 @@ def test_mix_equals_alone(gpu):
         n = c.shape[0]
         assert np.array_equal(o["morton_keys_alt"][:n], g["morton_keys_alt"][:n])
         assert np.array_equal(o["unique"][:nu], g["morton_keys"][:nu])
-        for k in ("prefix_n", "parents", "edge_offset"):
+        for k in ("prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parents",
+                  "edge_count", "edge_offset"):
             assert np.array_equal(o[k][:nb], g[k][:nb]), (i, k)
         rows = min(no, rw.octree_capacity(n))
-        for k in ("children", "corner", "cell_size", "child_node_mask"):
+        for k in ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"):
             assert np.array_equal(o[k][:rows].view(np.uint8), g[k][:rows].view(np.uint8)), (i, k)
     srv.close()
 

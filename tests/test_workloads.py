@@ -1,0 +1,60 @@
+"""The batched and served workloads at their own sizes, against the CPU oracle.
+
+* The batched workload: 512 clouds, sizes log-uniform in [2^10, 2^20] (seed 7),
+  half uniform and half 16-cluster Gaussian, built in one segmented pass;
+  every cloud's 14 output arrays must equal the oracle's build of that cloud
+  alone (the reference builds one cloud per AppData, apps/tree/main.cpp:18-38).
+* The served workload: clouds of 2^12..2^22 points, uniform and clustered,
+  submitted together to the continuous-batching server with outputs kept;
+  each request's arrays must equal the oracle's, whatever batch it ran in.
+"""
+import pytest
+
+from parity_helpers import compare
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def big_pool(orc):
+    return orc.Pool(16)
+
+
+@pytest.mark.parametrize("variant", [0, 3])
+def test_batch_workload_matches_oracle(gpu, orc, big_pool, variant):
+    """variant: the sort tile shape of the batch (256 x 16, or 1024 x 8, the
+    chunk-sized tile)."""
+    rw = gpu
+    sizes, clouds = rw.batch_mix_clouds()
+    assert sizes.size == 512 and sizes.min() >= 1 << 10 and sizes.max() <= 1 << 20
+    print(f"case batch workload clouds=512 points={int(sizes.sum())} variant={variant}", flush=True)
+    b = rw.Batch(clouds)
+    b.ws.set_sort_variant(variant)
+    b.run()
+    res = b.read()
+    for s, c in enumerate(clouds):
+        (nu, nb, no, rc), o, _ = orc.build(big_pool, c)
+        assert rc == 0, s
+        r = res[s]
+        g = dict(r, morton_keys=r["unique"])
+        compare(rw, c.shape[0], (r["n_unique"], max(r["n_unique"] - 1, 0), r["n_oct"]), g,
+                (nu, nb, no), o, f"cloud{s}")
+
+
+def test_served_workload_matches_oracle(gpu, orc, big_pool):
+    rw = gpu
+    sizes, clouds = rw.serving_mix_clouds()
+    assert sizes.min() >= 1 << 12 and sizes.max() <= 1 << 22
+    print(f"case served workload requests={len(clouds)} points={int(sizes.sum())}", flush=True)
+    srv = rw.Server(pool_bytes=64 << 30, n_slots=3, slot_points=1 << 24, max_clouds=512)
+    try:
+        res = srv.mix(clouds)
+    finally:
+        srv.close()
+    assert max(b for _, _, b in res) > 1  # requests were coalesced into batches
+    for i, c in enumerate(clouds):
+        (nu, nb, no), g, _ = res[i]
+        (onu, onb, ono, rc), o, _ = orc.build(big_pool, c)
+        assert rc == 0, i
+        compare(rw, c.shape[0], (nu, nb, no), dict(g, morton_keys=g["unique"]), (onu, onb, ono), o,
+                f"request{i}")
