@@ -3,13 +3,18 @@
 Workload (the metric's configuration): one plain build of 2^24 float4 points
 uniform in [0,1024)^3 (seed 114514), 32-bit Morton keys, all seven stages,
 inputs resident in HBM.  A step is one whole build (rwh_build_octree); K steps
-are timed between stream synchronisations.  Per-stage device times come from
-HIP events recorded on the build's stream around each stage entry point;
-algorithmic bytes per stage give achieved GB/s against the 8 TB/s HBM peak.
-The CPU baseline is the repo's pthread oracle on the same points, on this
-host's cores.
+are timed between stream synchronisations.  The last build is then checked
+against the CPU oracle, array by array, outside the timed region (exit status
+1 on a mismatch).  Per-stage device times come from HIP events recorded on the
+build's stream around each stage entry point; algorithmic bytes per stage give
+achieved GB/s against the 8 TB/s HBM peak and against the measured
+device-to-device copy rate.  Beside the main line: the clustered single
+builds (2^24 and 2^22 points), the 512-cloud batched build against one plain
+build per cloud, the continuous-batching server under Poisson load, and the
+CPU pthread oracle on this host.
 
-Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Prints one JSON line (progress goes to stderr).
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
@@ -28,6 +33,18 @@ import __graft_entry__ as ge  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "octree build Mpts/s @2^24 + batched/plain speedup, HBM roofline frac, p99 latency"
+# CPU threads for the oracle: the host share of one GPU on the benchmark box
+CPU_THREAD_CAP = 16
+FIELDS = ("morton_keys_alt", "morton_keys", "prefix_n", "has_leaf_left", "has_leaf_right",
+          "left_child", "parents", "edge_count", "edge_offset", "children", "corner", "cell_size",
+          "child_node_mask", "child_leaf_mask")
+SLOT_FIELDS = ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask")
+STAGES = ("S1_morton", "S2_sort", "S3_unique", "S4_radix_tree", "S5_edge_count", "S6_prefix_sum",
+          "S7_octree")
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def stage_bytes(n, nu, nb, no):
@@ -43,15 +60,15 @@ def stage_bytes(n, nu, nb, no):
     }
 
 
-def measured_sort_traffic(log2n):
-    """HBM bytes of one sort call from the committed rocprofv3 counter summary
+def profile_figure(log2n, key):
+    """A per-call byte count from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE runs, calibrated by a 256 MiB copy); None for a
     workload that summary does not cover."""
     path = os.path.join(ROOT, "profiles", f"build_2p{log2n}_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("sort_hbm_bytes_per_call:"):
+        if line.startswith(key + ":"):
             return int(line.split(":")[1])
     return None
 
@@ -79,10 +96,6 @@ def enqueue_stages(rw, ws, tb, n, stream, events=None):
         events[len(calls)].record(stream)
 
 
-STAGES = ("S1_morton", "S2_sort", "S3_unique", "S4_radix_tree", "S5_edge_count", "S6_prefix_sum",
-          "S7_octree")
-
-
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -92,6 +105,221 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def oracle_threads():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return min(avail, CPU_THREAD_CAP), avail
+
+
+def live_arrays(o, n, nu, nb, rows, unique_key="morton_keys"):
+    out = {"morton_keys_alt": o["morton_keys_alt"][:n], "morton_keys": o[unique_key][:nu]}
+    for f in FIELDS[2:]:
+        out[f] = o[f][:rows] if f in SLOT_FIELDS else o[f][:nb]
+    return out
+
+
+def verify_build(rw, orc, pool, tb, pts, counts):
+    """Every output array of the build left in tb against the oracle's build
+    of the same points: (ok, list of mismatching arrays)."""
+    n = pts.shape[0]
+    nu, nb, no = counts
+    (onu, onb, ono, rc), o, _ = orc.build(pool, pts)
+    if (onu, onb, ono) != (nu, nb, no):
+        return False, [f"counts {counts} != oracle {(onu, onb, ono)}"]
+    rows = min(max(no, 1), rw.octree_capacity(n))
+    g = live_arrays(tb.read(n, rows), n, nu, nb, rows)
+    want = live_arrays(o, n, nu, nb, rows, unique_key="unique")
+    bad = [f for f in FIELDS if not np.array_equal(np.ascontiguousarray(g[f]).view(np.uint8),
+                                                   np.ascontiguousarray(want[f]).view(np.uint8))]
+    return not bad, bad
+
+
+def timed_builds(rw, ws, tb, n, stream, steps, warmup):
+    L = rw.lib()
+
+    def build():
+        rw.check(L.rwh_build_octree(ws.ptr, tb.ptr(), n, 0.0, 1024.0, stream.ptr), "build")
+
+    for _ in range(warmup):
+        build()
+    stream.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        build()
+    stream.sync()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def read_counts(rw, ws, cap, stream):
+    import ctypes as C
+    counts = (C.c_uint32 * 3)()
+    rw.check(rw.lib().rwh_read_counts(ws.ptr, cap, counts, stream.ptr), "counts")
+    return counts[0], counts[1], counts[2]
+
+
+def copy_rate(rw, stream, mib=256, reps=10):
+    """Device-to-device copy rate of the runtime (GB/s, read + write bytes)."""
+    L = rw.lib()
+    a = rw.DeviceArray(mib << 18, np.uint32, zero=True)
+    b = rw.DeviceArray(mib << 18, np.uint32, zero=True)
+    e0, e1 = rw.Event(), rw.Event()
+    rw.check(L.rwh_memcpy_d2d(b.ptr, a.ptr, mib << 20, stream.ptr), "copy")
+    e0.record(stream.ptr)
+    for _ in range(reps):
+        rw.check(L.rwh_memcpy_d2d(b.ptr, a.ptr, mib << 20, stream.ptr), "copy")
+    e1.record(stream.ptr)
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    return 2 * (mib << 20) / (ms * 1e-3) / 1e9
+
+
+def clustered_section(rw, orc, pool, steps, warmup):
+    """The clustered single builds: 64-component Gaussian mixture (sigma 20.48,
+    seed 1) at 2^24 and 2^22 points, each with the tuned sort shape, verified
+    against the oracle."""
+    out = {}
+    for log2n in (24, 22):
+        n = 1 << log2n
+        progress(f"clustered build 2^{log2n}")
+        pts = rw.gmm_points(n, 1, 64, 20.48)
+        variant = rw.best_sort_variant(n)
+        ws = rw.Workspace(n, sort_variant=variant)
+        cap = rw.octree_capacity(n)
+        tb = rw.TreeBuild(n, cap, rw.DeviceArray.from_numpy(pts.reshape(-1)))
+        stream = rw.Stream()
+        ms = timed_builds(rw, ws, tb, n, stream, steps, warmup)
+        counts = read_counts(rw, ws, cap, stream)
+        ok, bad = verify_build(rw, orc, pool, tb, pts, counts)
+        out[f"2^{log2n}"] = {"points": n, "ms": round(ms, 4), "mpts_per_s": round(n / (ms * 1e-3) / 1e6, 1),
+                             "n_unique": counts[0], "n_octree_nodes": counts[2],
+                             "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
+                             "verified": ok, "mismatches": bad}
+    return out
+
+
+def batched_section(rw, reps=5):
+    """512 clouds (sizes log-uniform in [2^10, 2^20], seed 7; even clouds uniform,
+    odd clouds 16-cluster Gaussian with sigma 10.24) built in one segmented pass,
+    against one plain build per cloud on the same resident buffers."""
+    import ctypes as C
+    L = rw.lib()
+    sizes, clouds = rw.batch_mix_clouds()
+    b = rw.Batch(clouds)
+    del clouds
+    total = int(sizes.sum())
+    k = int(sizes.size)
+    stream = rw.Stream()
+    e0, e1 = rw.Event(), rw.Event()
+    rw.check(L.rwh_batch_prepare(b.ws.ptr, b.sizes.ctypes.data, b.pb.ctypes.data, b.ob.ctypes.data,
+                                 b.oc.ctypes.data, k, stream.ptr), "prepare")
+
+    def timed(fn, n_reps):
+        fn()
+        stream.sync()
+        e0.record(stream.ptr)
+        for _ in range(n_reps):
+            fn()
+        e1.record(stream.ptr)
+        e1.sync()
+        return e0.elapsed_ms(e1) / n_reps
+
+    def enqueue():
+        rw.check(L.rwh_build_octree_batch_enqueue(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr), "batch")
+
+    # the batch's sort tile shape, tuned on the batch itself (shapes whose
+    # tile fits a chunk; larger ones fall back to the chunk-sized tile)
+    shape_ms = {}
+    for v, (blk, kpt) in enumerate(rw.sort_variants()):
+        if blk * kpt <= 8192:
+            b.ws.set_sort_variant(v)
+            shape_ms[v] = timed(enqueue, 3)
+    variant = min(shape_ms, key=shape_ms.get)
+    b.ws.set_sort_variant(variant)
+    t_enqueue = timed(enqueue, reps)
+    g = C.c_void_p()
+    rw.check(L.rwh_batch_graph_create(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr, C.byref(g)), "graph")
+    t_graph = timed(lambda: rw.check(L.rwh_graph_launch(g, stream.ptr), "graph launch"), reps)
+    nu_b, no_b = (x.copy() for x in b.ws.counts(k))
+    snap = {f: b.tb[f].numpy() for f in FIELDS}
+
+    # plain: one rwh_build_octree per cloud, outputs written to the same places
+    ws1 = rw.Workspace(int(sizes.max()))
+    base = b.tb.struct
+    views = []
+    for s in range(k):
+        v = rw.TreeBuffers()
+        for name, (dt, w, per_point) in rw.TREE_FIELDS.items():
+            off = (int(b.pb[s]) if per_point else int(b.ob[s])) * w * np.dtype(dt).itemsize
+            setattr(v, name, getattr(base, name) + off)
+        v.points = base.points + int(b.pb[s]) * 16
+        v.oct_capacity = int(b.oc[s])
+        views.append(v)
+
+    def plain_all():
+        for s in range(k):
+            rw.check(L.rwh_build_octree(ws1.ptr, C.addressof(views[s]), int(sizes[s]), 0.0, 1024.0,
+                                        stream.ptr), "plain")
+
+    t_plain = timed(plain_all, max(1, reps // 2))
+    # equal output: the plain builds rewrote every array the batched build
+    # wrote, in the same places (every byte of all 14 arrays compared)
+    same = all(np.array_equal(snap[f], b.tb[f].numpy()) for f in FIELDS)
+    rw.check(L.rwh_graph_destroy(g), "graph destroy")
+    return {"workload": "512 clouds, sizes log-uniform 2^10..2^20 (seed 7), half uniform, "
+                        "half 16-cluster Gaussian sigma 10.24, one segmented build",
+            "points": total, "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
+            "sort_variant_ms": {"{}x{}".format(*rw.sort_variants()[v]): round(t, 3) for v, t in shape_ms.items()},
+            "ms_batched": round(t_enqueue, 3), "ms_batched_graph": round(t_graph, 3),
+            "ms_plain": round(t_plain, 3),
+            "mpts_per_s_batched": round(total / (min(t_enqueue, t_graph) * 1e-3) / 1e6, 1),
+            "mpts_per_s_plain": round(total / (t_plain * 1e-3) / 1e6, 1),
+            "speedup": round(t_plain / min(t_enqueue, t_graph), 2), "equal_output": bool(same),
+            "octree_nodes": int(no_b.sum()), "unique_keys": int(nu_b.sum())}
+
+
+def serving_section(rw, rates=(200, 1000, 5000), seconds=2.0, max_requests=4000):
+    """Open-loop Poisson request streams against the continuous-batching server
+    and against one plain build per request, plus each server's capacity (all
+    requests queued at once)."""
+    sizes, clouds = rw.serving_mix_clouds()
+    pinned = []
+    for c in clouds:
+        p = rw.PinnedArray(c.shape)
+        p.array[:] = c
+        pinned.append(p)
+    del clouds
+    hosted = [p.array for p in pinned]
+    keys = ("throughput_rps", "mpts_per_s", "p50_ms", "p99_ms", "mean_batch", "batches", "rejected")
+    out = {"workload": "Poisson arrivals over 32 clouds with sizes log-uniform 2^12..2^22 (seed 11), "
+                       "half uniform, half 64-cluster Gaussian sigma 20.48; pinned host points "
+                       "uploaded per request; latency = arrival -> counts on host",
+           "pool_cap_GB": 64, "rates": {}}
+    srv = rw.Server(pool_bytes=64 << 30, n_slots=3, slot_points=1 << 26, max_clouds=512)
+    plain = rw.Server(pool_bytes=64 << 30, n_slots=1, slot_points=1 << 23, plain=True)
+    # capacity: every request queued at once, completions per second
+    cap_b = srv.load(hosted, 1e7, 2000, seed=1)
+    cap_p = plain.load(hosted, 1e7, 1000, seed=1)
+    out["capacity_rps"] = {"batched": round(cap_b["throughput_rps"], 1), "plain": round(cap_p["throughput_rps"], 1),
+                           "batched_mean_batch": round(cap_b["mean_batch"], 2)}
+    for r in rates:
+        progress(f"serving at {r} req/s")
+        n_req = int(min(max_requests, max(50, r * seconds)))
+        res = {}
+        for name, server in (("batched", srv), ("plain", plain)):
+            a = server.load(hosted, float(r), n_req, seed=r)
+            d = {k: round(a[k], 3) for k in keys if k in a}
+            # offered load above what the server completes: the latency then
+            # measures queue growth over the run, not a steady state
+            d["saturated"] = bool(a["throughput_rps"] < 0.95 * r)
+            res[name] = d
+        out["rates"][str(r)] = dict(requests=n_req, **res)
+    srv.close()
+    plain.close()
+    return out
 
 
 def cpu_runs(orc, pool, pts, max_runs, budget_s, warm=True):
@@ -110,24 +338,25 @@ def cpu_runs(orc, pool, pts, max_runs, budget_s, warm=True):
     return float(np.median(times)), {k: round(float(v), 3) for k, v in zip(STAGES, st)}, len(times)
 
 
-def cpu_baseline(rw, pts, budget_s=12.0):
-    """The pthread oracle on the host: the full 2^24 workload on up to 16
-    threads (the reported value), with per-stage times, one thread when a
-    build is short enough, and the reference's own CPU case (307,200 uniform
-    points, seed 114514) on all threads and on one."""
-    orc = ge.load_oracle()
-    threads = min(16, os.cpu_count() or 1)
+def cpu_baseline(rw, orc, pts, budget_s=12.0):
+    """The pthread oracle on the host: the full 2^24 workload on the CPUs of
+    one GPU's share (the reported value), with per-stage times, one thread,
+    and the reference's own CPU case (307,200 uniform points, seed 114514) on
+    all those threads and on one."""
+    threads, avail = oracle_threads()
     pool = orc.Pool(threads)
     t, stages, runs = cpu_runs(orc, pool, pts, 5, budget_s)
     n = pts.shape[0]
     out = {"value": round(n / t / 1e6, 3), "unit": "Mpts/s", "cores": threads, "kind": "port",
            "sample": f"{n} uniform points (the full workload), all 7 stages of the pthread oracle, "
                      f"median of {runs} runs after 1 warm-up",
+           "host_cpus": os.cpu_count(), "affinity_cpus": avail,
+           "threads_note": f"threads = min(CPUs this process may use, {CPU_THREAD_CAP}): the host "
+                           f"share of one GPU on the benchmark box",
            "cpu_model": cpu_model(), "stages_ms": stages}
     one = orc.Pool(1)
-    if t < 0.5:  # one thread at the full size: one run
-        t1, stages1, _ = cpu_runs(orc, one, pts, 1, 0.0, warm=False)
-        out["one_thread"] = {"value": round(n / t1 / 1e6, 3), "ms": round(t1 * 1e3, 2), "stages_ms": stages1}
+    t1, stages1, _ = cpu_runs(orc, one, pts, 1, 0.0, warm=False)
+    out["one_thread"] = {"value": round(n / t1 / 1e6, 3), "ms": round(t1 * 1e3, 2), "stages_ms": stages1}
     small = rw.uniform_points(307200, 114514)
     ts, stages_s, runs_s = cpu_runs(orc, pool, small, 10, 2.0)
     ts1, stages_s1, runs_s1 = cpu_runs(orc, one, small, 10, 3.0)
@@ -140,117 +369,20 @@ def cpu_baseline(rw, pts, budget_s=12.0):
     return out
 
 
-def batched_section(rw, reps=5):
-    """512 clouds (sizes log-uniform in [2^10, 2^20], seed 7; even clouds uniform,
-    odd clouds 16-cluster Gaussian with sigma 10.24) built in one segmented pass,
-    against one plain build per cloud on the same resident buffers."""
-    import ctypes as C
-    L = rw.lib()
-    sizes = rw.log_uniform_sizes(7, 512, 10, 20)
-    clouds = [rw.uniform_points(int(n), (7 << 20) + s) if s % 2 == 0 else
-              rw.gmm_points(int(n), (7 << 20) + s, 16, 10.24) for s, n in enumerate(sizes)]
-    b = rw.Batch(clouds)
-    del clouds
-    total = int(sizes.sum())
-    stream = rw.Stream()
-    e0, e1 = rw.Event(), rw.Event()
-    rw.check(L.rwh_batch_prepare(b.ws.ptr, b.sizes.ctypes.data, b.pb.ctypes.data, b.ob.ctypes.data,
-                                 b.oc.ctypes.data, 512, stream.ptr), "prepare")
-
-    def timed(fn, n_reps):
-        fn()
-        stream.sync()
-        e0.record(stream.ptr)
-        for _ in range(n_reps):
-            fn()
-        e1.record(stream.ptr)
-        e1.sync()
-        return e0.elapsed_ms(e1) / n_reps
-
-    t_enqueue = timed(lambda: rw.check(L.rwh_build_octree_batch_enqueue(
-        b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr), "batch"), reps)
-    g = C.c_void_p()
-    rw.check(L.rwh_batch_graph_create(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr, C.byref(g)), "graph")
-    t_graph = timed(lambda: rw.check(L.rwh_graph_launch(g, stream.ptr), "graph launch"), reps)
-    batched = {k: v.copy() for k, v in zip(("n_unique", "n_oct"), b.ws.counts(512))}
-    snap = {k: b.tb[k].numpy() for k in ("morton_keys_alt", "parents", "edge_offset", "children", "corner")}
-
-    # plain: one rwh_build_octree per cloud, outputs written to the same places
-    ws1 = rw.Workspace(int(sizes.max()))
-    base = b.tb.struct
-    views = []
-    for s in range(512):
-        v = rw.TreeBuffers()
-        for name, (dt, k, per_point) in rw.TREE_FIELDS.items():
-            off = (int(b.pb[s]) if per_point else int(b.ob[s])) * k * np.dtype(dt).itemsize
-            setattr(v, name, getattr(base, name) + off)
-        v.points = base.points + int(b.pb[s]) * 16
-        v.oct_capacity = int(b.oc[s])
-        views.append(v)
-
-    def plain_all():
-        for s in range(512):
-            rw.check(L.rwh_build_octree(ws1.ptr, C.addressof(views[s]), int(sizes[s]), 0.0, 1024.0,
-                                        stream.ptr), "plain")
-
-    t_plain = timed(plain_all, max(1, reps // 2))
-    # equal output: the plain builds rewrote every array the batched build wrote
-    same = all(np.array_equal(snap[k], b.tb[k].numpy()) for k in snap)
-    rw.check(L.rwh_graph_destroy(g), "graph destroy")
-    return {"workload": "512 clouds, sizes log-uniform 2^10..2^20 (seed 7), half uniform, "
-                        "half 16-cluster Gaussian sigma 10.24, one segmented build",
-            "points": total, "ms_batched": round(t_enqueue, 3), "ms_batched_graph": round(t_graph, 3),
-            "ms_plain": round(t_plain, 3),
-            "mpts_per_s_batched": round(total / (min(t_enqueue, t_graph) * 1e-3) / 1e6, 1),
-            "mpts_per_s_plain": round(total / (t_plain * 1e-3) / 1e6, 1),
-            "speedup": round(t_plain / min(t_enqueue, t_graph), 2), "equal_output": bool(same),
-            "octree_nodes": int(batched["n_oct"].sum())}
-
-
-def serving_section(rw, rates=(200, 1000, 5000), seconds=2.0, max_requests=4000):
-    """Open-loop Poisson request streams against the continuous-batching server
-    and against one plain build per request."""
-    sizes = rw.log_uniform_sizes(11, 32, 12, 22)
-    pinned = []
-    for i, n in enumerate(sizes):
-        p = rw.PinnedArray((int(n), 4))
-        p.array[:] = (rw.uniform_points(int(n), 900 + i) if i % 2 == 0 else
-                      rw.gmm_points(int(n), 900 + i, 16, 10.24))
-        pinned.append(p)
-    clouds = [p.array for p in pinned]
-    out = {"workload": "Poisson arrivals; 32 clouds with sizes log-uniform 2^12..2^22 (seed 11), "
-                       "half uniform, half 16-cluster Gaussian; pinned host points uploaded per "
-                       "request; latency = arrival -> counts on host",
-           "pool_cap_GB": 64, "rates": {}}
-    srv = rw.Server(pool_bytes=64 << 30, n_slots=3, slot_points=1 << 26, max_clouds=512)
-    plain = rw.Server(pool_bytes=64 << 30, n_slots=1, slot_points=1 << 23, plain=True)
-    for r in rates:
-        n_req = int(min(max_requests, max(50, r * seconds)))
-        a = srv.load(clouds, float(r), n_req, seed=r)
-        p = plain.load(clouds, float(r), n_req, seed=r)
-        out["rates"][str(r)] = {
-            "requests": n_req,
-            "batched": {k: round(a[k], 3) for k in ("throughput_rps", "mpts_per_s", "p50_ms", "p99_ms",
-                                                    "mean_batch", "batches", "rejected")},
-            "plain": {k: round(p[k], 3) for k in ("throughput_rps", "mpts_per_s", "p50_ms", "p99_ms",
-                                                  "rejected")}}
-    srv.close()
-    plain.close()
-    return out
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=24)
-    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-verify", action="store_true", help="skip the oracle check of the timed build")
+    ap.add_argument("--no-clustered", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
-    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (clouds and requests "
@@ -264,6 +396,7 @@ def main():
     n = 1 << args.log2n
     pts = rw.uniform_points(n, 114514)
     cap = rw.octree_capacity(n)
+    progress("timing the sort tile shapes")
     variant_ms = rw.measure_sort_variants(n, 5)
     variant = int(np.argmin(variant_ms)) if variant_ms else 0
     ws = rw.Workspace(n, sort_variant=variant)
@@ -271,25 +404,20 @@ def main():
     stream = rw.Stream()
     L = rw.lib()
 
-    def build():
-        rw.check(L.rwh_build_octree(ws.ptr, tb.ptr(), n, 0.0, 1024.0, stream.ptr), "build")
+    progress(f"timing {args.steps} builds of 2^{args.log2n} points")
+    ms_per_step = timed_builds(rw, ws, tb, n, stream, args.steps, args.warmup)
+    nu, nb, no = read_counts(rw, ws, cap, stream)
 
-    for _ in range(args.warmup):
-        build()
-    stream.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        build()
-    stream.sync()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-
-    import ctypes as C
-    counts = (C.c_uint32 * 3)()
-    rw.check(L.rwh_read_counts(ws.ptr, cap, counts, stream.ptr), "counts")
-    nu, nb, no = counts[0], counts[1], counts[2]
+    orc = None
+    threads, _ = oracle_threads()
+    verified, mismatches = None, []
+    if not args.no_verify:
+        progress("checking the last build against the oracle")
+        orc = ge.load_oracle()
+        verified, mismatches = verify_build(rw, orc, orc.Pool(threads), tb, pts, (nu, nb, no))
 
     # per-stage device time: events on the build stream around each stage
+    progress("per-stage times")
     evs = [rw.Event() for _ in range(8)]
     acc = np.zeros(7)
     for _ in range(args.stage_reps):
@@ -297,6 +425,7 @@ def main():
         evs[7].sync()
         acc += [evs[i].elapsed_ms(evs[i + 1]) for i in range(7)]
     stage_ms = acc / args.stage_reps
+    copy_gbs = copy_rate(rw, stream)
     if args.calib:
         a = rw.DeviceArray(64 << 20, np.uint32, zero=True)
         b = rw.DeviceArray(64 << 20, np.uint32, zero=True)
@@ -307,7 +436,10 @@ def main():
     for (name, b), ms in zip(sb.items(), stage_ms):
         gbs = b / (ms * 1e-3) / 1e9
         stages[name] = {"ms": round(float(ms), 4), "bytes": int(b), "GB/s": round(gbs, 1),
-                        "frac": round(gbs / HBM_PEAK_GBS, 3)}
+                        "frac": round(gbs / HBM_PEAK_GBS, 3), "frac_of_copy": round(gbs / copy_gbs, 3)}
+    # the dominant kernel: S7 is one launch (k_oct_nodes), the longest of the build
+    s7 = stages["S7_octree"]
+    out_bytes_true = 17 * nb + 60 * no  # every slot's row, corner, cell and both masks
     sort = stages["S2_sort"]
     out = {
         "metric": METRIC,
@@ -325,21 +457,43 @@ def main():
         "config": {"workload": f"plain single octree build, 2^{args.log2n} uniform float4 points "
                                f"in [0,1024)^3, seed 114514, 7 stages, inputs resident in HBM",
                    "n_points": n, "n_unique": nu, "n_octree_nodes": no},
-        "roofline": {"bound": "hbm", "kernel": "S2 onesweep radix sort (histogram + 4 digit passes)",
-                     "achieved": sort["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sort["frac"], "traffic": measured_sort_traffic(args.log2n)},
+        "verified": verified,
+        "roofline": {"bound": "hbm", "kernel": "k_oct_nodes (S7 octree nodes, one launch)",
+                     "achieved": s7["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": s7["frac"], "traffic": profile_figure(args.log2n, "k_oct_nodes_hbm_bytes_per_call"),
+                     "algorithmic_bytes": s7["bytes"], "ms": s7["ms"],
+                     "bytes_with_all_outputs": out_bytes_true,
+                     "frac_with_all_outputs": round(out_bytes_true / (s7["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)},
+        "sort_roofline": {"kernels": "S2 onesweep radix sort (histogram + 4 digit passes)",
+                          "achieved": sort["GB/s"], "peak": HBM_PEAK_GBS, "frac": sort["frac"],
+                          "frac_of_copy": sort["frac_of_copy"], "ms": sort["ms"],
+                          "traffic": profile_figure(args.log2n, "sort_hbm_bytes_per_call")},
+        "copy_GBs": round(copy_gbs, 1),
         "stages": stages,
         "sort_variants": {f"{b}x{k}": round(ms, 4) for (b, k), ms in zip(rw.sort_variants(), variant_ms)},
         "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
         "device": rw.device_name(),
     }
+    if mismatches:
+        out["mismatches"] = mismatches
+    if not args.no_clustered:
+        orc = orc or ge.load_oracle()
+        out["clustered"] = clustered_section(rw, orc, orc.Pool(threads), max(3, args.steps // 2), args.warmup)
     if not args.no_batched:
+        progress("batched build of 512 clouds")
         out["batched"] = batched_section(rw)
     if not args.no_serving:
         out["serving"] = serving_section(rw)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(rw, pts)
-    print(json.dumps(out))
+        progress("CPU baseline")
+        orc = orc or ge.load_oracle()
+        out["cpu_baseline"] = cpu_baseline(rw, orc, pts)
+    print(json.dumps(out), flush=True)
+    failed = verified is False or any(not c["verified"] for c in out.get("clustered", {}).values()) or \
+        out.get("batched", {}).get("equal_output") is False
+    if failed:
+        print("bench: a build differs from its reference result", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

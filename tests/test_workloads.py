@@ -32,13 +32,18 @@ def test_batch_workload_matches_oracle(gpu, orc, big_pool, variant):
     b.ws.set_sort_variant(variant)
     b.run()
     res = b.read()
+    over = 0
     for s, c in enumerate(clouds):
         (nu, nb, no, rc), o, _ = orc.build(big_pool, c)
-        assert rc == 0, s
+        # a sparse cloud can need more octree nodes than its 0.6 n slots: both
+        # sides report the count and keep the first 0.6 n slots (D11)
+        assert rc == (-1 if no > rw.octree_capacity(c.shape[0]) else 0), s
+        over += rc != 0
         r = res[s]
         g = dict(r, morton_keys=r["unique"])
         compare(rw, c.shape[0], (r["n_unique"], max(r["n_unique"] - 1, 0), r["n_oct"]), g,
                 (nu, nb, no), o, f"cloud{s}")
+    print(f"clouds above the octree capacity: {over}", flush=True)
 
 
 def test_served_workload_matches_oracle(gpu, orc, big_pool):
