@@ -176,6 +176,20 @@ int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
  * rwh_workspace_counts(ws). */
 int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
                      float range, rwh_stream_t stream);
+/* ---- the same build without allocating (no counterpart in the reference)
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The build of
+ * rwh_build_octree, with its workspace in a static device array of this
+ * library (about 19 MiB per GPU, held until the process ends), so at most
+ * RWH_NOALLOC_MAX_POINTS points (larger n: RWH_ERR_CAPACITY, nothing
+ * enqueued).  d_counts (device, 2 words) gets {n_unique, n_octree_nodes};
+ * compare n_octree_nodes with bufs->oct_capacity.  Calls on one GPU share that
+ * array: calls that overlap on one GPU (two streams, two threads) race.  The
+ * sort runs its 1024 x 16 tile shape. */
+#define RWH_NOALLOC_MAX_POINTS (1u << 22)
+int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
+                             uint32_t* d_counts, rwh_stream_t stream);
+
 /* Waits for the stream and copies {n_unique, n_brt, n_octree_nodes} of cloud
  * 0 to host; RWH_ERR_CAPACITY if n_octree_nodes exceeds the capacity given. */
 int rwh_read_counts(rwh_workspace* ws, uint32_t oct_capacity, uint32_t* counts3,

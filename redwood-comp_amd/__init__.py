@@ -32,6 +32,7 @@ RWH_ERR_HIP = 2
 RWH_ERR_CAPACITY = 3
 RWH_ERR_NO_DEVICE = 4
 RWH_ERR_INJECTED = 5
+NOALLOC_MAX_POINTS = 1 << 22  # RWH_NOALLOC_MAX_POINTS
 
 # Every function of include/rwhip.h: name -> (restype, argtypes).
 _P = C.c_void_p
@@ -80,6 +81,7 @@ ABI = {
     "rwh_octree": (_I, [_P, _P, _P, _U32, _F, _F, _P]),
     "rwh_link_leaf": (_I, [_P, _P, _P, _U32, _P]),
     "rwh_build_octree": (_I, [_P, _P, _U32, _F, _F, _P]),
+    "rwh_build_octree_noalloc": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_read_counts": (_I, [_P, _U32, C.POINTER(_U32), _P]),
     "rwh_batch_layout": (_I, [_P, _U32, _P, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
     "rwh_build_octree_batch": (_I, [_P, _P, _P, _P, _P, _P, _U32, _F, _F, _P]),

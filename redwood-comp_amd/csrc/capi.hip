@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -85,7 +86,7 @@ struct rwh_workspace {
   uint32_t max_sort_tiles = 0;
   uint32_t max_scan_tiles = 0;
   size_t bytes = 0;
-  void* block = nullptr;  // one device allocation for everything below
+  void* block = nullptr;  // one device allocation for everything below (none for a view of the scratch)
   uint32_t* keys_tmp = nullptr;
   uint32_t* hist = nullptr;
   uint32_t* digit_start = nullptr;
@@ -118,7 +119,85 @@ struct rwh_workspace {
 
 namespace {
 
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Byte offsets of a workspace's areas inside its one device block, for
+// max_points index-space points and max_segs clouds.
+struct WsLayout {
+  uint32_t max_chunks, max_sort_tiles, max_scan_tiles;
+  size_t tmp, hist, ds, lbs, tc, lbu, lbc, ctrl, cnt, tab, tl, ts;
+  size_t bytes;
+};
+
+constexpr WsLayout ws_layout(uint32_t max_points, uint32_t max_segs) {
+  WsLayout L{};
+  const uint32_t mp = max_points > 0 ? max_points : 1u;
+  L.max_chunks = (mp + kChunk - 1) / kChunk;
+  L.max_sort_tiles = (mp + kSortMinTile - 1) / kSortMinTile;
+  L.max_scan_tiles = (mp + kScanTile - 1) / kScanTile;
+  size_t off = 0;
+  const auto take = [&off](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  L.tmp = take(sizeof(uint32_t) * static_cast<size_t>(mp));
+  L.hist = take(sizeof(uint32_t) * static_cast<size_t>(max_segs) * kHistCopies * kMaxPasses * 256);
+  L.ds = take(sizeof(uint32_t) * static_cast<size_t>(max_segs) * kMaxPasses * 256);
+  L.lbs = take(sizeof(uint64_t) * static_cast<size_t>(L.max_sort_tiles) * 256);
+  L.tc = take(sizeof(uint32_t) * static_cast<size_t>(L.max_sort_tiles) * 256);
+  L.lbu = take(sizeof(uint64_t) * static_cast<size_t>(L.max_scan_tiles));
+  L.lbc = take(sizeof(uint64_t) * static_cast<size_t>(L.max_scan_tiles));
+  L.ctrl = take(sizeof(uint32_t) * 3 * kCtrlWords);
+  L.cnt = take(sizeof(uint32_t) * 2ull * max_segs);
+  L.tab = take(sizeof(uint32_t) * (4ull * max_segs + L.max_chunks));
+  L.tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
+  L.ts = take(sizeof(uint32_t) * static_cast<size_t>(tree_tiles(mp)));
+  L.bytes = off;
+  return L;
+}
+
+// Points the workspace's areas into base (a block laid out by L).
+void bind_workspace(rwh_workspace* ws, char* base, const WsLayout& L, uint32_t max_points, uint32_t max_segs) {
+  ws->max_points = max_points;
+  ws->max_segs = max_segs;
+  ws->max_chunks = L.max_chunks;
+  ws->max_sort_tiles = L.max_sort_tiles;
+  ws->max_scan_tiles = L.max_scan_tiles;
+  ws->bytes = L.bytes;
+  ws->keys_tmp = reinterpret_cast<uint32_t*>(base + L.tmp);
+  ws->hist = reinterpret_cast<uint32_t*>(base + L.hist);
+  ws->digit_start = reinterpret_cast<uint32_t*>(base + L.ds);
+  ws->lb_sort = reinterpret_cast<uint64_t*>(base + L.lbs);
+  ws->tile_counts = reinterpret_cast<uint32_t*>(base + L.tc);
+  ws->lb_unique = reinterpret_cast<uint64_t*>(base + L.lbu);
+  ws->lb_scan = reinterpret_cast<uint64_t*>(base + L.lbc);
+  ws->ctrl = reinterpret_cast<uint32_t*>(base + L.ctrl);
+  ws->counts = reinterpret_cast<uint32_t*>(base + L.cnt);
+  ws->tables = reinterpret_cast<uint32_t*>(base + L.tab);
+  ws->tree_list = reinterpret_cast<uint32_t*>(base + L.tl);
+  ws->tree_sums = reinterpret_cast<uint32_t*>(base + L.ts);
+}
+
+// The allocation-free build's workspace: a static device array of the
+// library, laid out as a one-cloud workspace of RWH_NOALLOC_MAX_POINTS
+// points (each GPU holds its own copy until the process ends).  Host code
+// never names it directly: its device address comes from
+// hipGetSymbolAddress, once per device.  Zero at load, as a fresh workspace
+// after its memset; every launch leaves it consistent for the next one
+// (self-resetting look-back counters, histogram copies cleared by the digit
+// scan).
+constexpr WsLayout kNoallocLayout = ws_layout(RWH_NOALLOC_MAX_POINTS, 1);
+// the sort tile shape of the allocation-free build (1024 x 16)
+constexpr int kNoallocSortVariant = 5;
+static_assert(kSortVariants[kNoallocSortVariant].block == 1024 && kSortVariants[kNoallocSortVariant].kpt == 16,
+              "the allocation-free build runs the 1024 x 16 sort tile");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocLayout.bytes];
+
+namespace {
 
 int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs seg, uint32_t n_segs,
                  uint32_t n_index, uint32_t key_bits, bool plain, hipStream_t st,
@@ -193,6 +272,20 @@ int check_bufs(const rwh_tree_buffers* b) {
             "tree buffers must be 16-byte aligned");
   return RWH_OK;
 }
+
+// The whole build of one cloud on a workspace (or a view of the scratch
+// array); n_unique and n_octree_nodes go to d_nu and d_no (device).
+int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
+                 uint32_t* d_nu, uint32_t* d_no, hipStream_t st);
+
+// Per-device state of the allocation-free build.
+constexpr int kNoallocDevices = 64;
+struct NoallocDevice {
+  char* base = nullptr;  // the scratch array's address on this device
+  bool dirty = false;       // the last call stopped between two launches
+};
+std::mutex g_noalloc_mu;
+NoallocDevice g_noalloc[kNoallocDevices];
 
 }  // namespace
 
@@ -343,55 +436,19 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   RWH_CHECK(max_points <= kMaxPoints, "max_points above 2^30");
   RWH_CHECK(max_segments >= 1 && max_segments <= (1u << 20), "max_segments out of range");
   auto* ws = new rwh_workspace();
-  ws->max_points = max_points;
-  ws->max_segs = max_segments;
-  const uint32_t mp = std::max<uint32_t>(max_points, 1);
-  ws->max_chunks = (mp + kChunk - 1) / kChunk;
-  ws->max_sort_tiles = (mp + kSortMinTile - 1) / kSortMinTile;
-  ws->max_scan_tiles = (mp + kScanTile - 1) / kScanTile;
-  // carve one allocation
-  size_t off = 0;
-  const auto take = [&](size_t bytes) {
-    const size_t o = off;
-    off = align_up(off + bytes, 256);
-    return o;
-  };
-  const size_t o_tmp = take(sizeof(uint32_t) * static_cast<size_t>(mp));
-  const size_t o_hist = take(sizeof(uint32_t) * static_cast<size_t>(max_segments) * kHistCopies * kMaxPasses * 256);
-  const size_t o_ds = take(sizeof(uint32_t) * static_cast<size_t>(max_segments) * kMaxPasses * 256);
-  const size_t o_lbs = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_sort_tiles) * 256);
-  const size_t o_tc = take(sizeof(uint32_t) * static_cast<size_t>(ws->max_sort_tiles) * 256);
-  const size_t o_lbu = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_scan_tiles));
-  const size_t o_lbc = take(sizeof(uint64_t) * static_cast<size_t>(ws->max_scan_tiles));
-  const size_t o_ctrl = take(sizeof(uint32_t) * 3 * kCtrlWords);
-  const size_t o_cnt = take(sizeof(uint32_t) * 2ull * max_segments);
-  const size_t o_tab = take(sizeof(uint32_t) * ws->table_words());
-  const size_t o_tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
-  const size_t o_ts = take(sizeof(uint32_t) * static_cast<size_t>(tree_tiles(mp)));
-  ws->bytes = off;
+  const WsLayout L = ws_layout(max_points, max_segments);
   char* base = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), off);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), L.bytes);
   if (e != hipSuccess) {
     delete ws;
     return fail(e == hipErrorNoDevice ? RWH_ERR_NO_DEVICE : RWH_ERR_HIP,
                 std::string("hipMalloc(workspace): ") + hipGetErrorString(e));
   }
   ws->block = base;
-  ws->keys_tmp = reinterpret_cast<uint32_t*>(base + o_tmp);
-  ws->hist = reinterpret_cast<uint32_t*>(base + o_hist);
-  ws->digit_start = reinterpret_cast<uint32_t*>(base + o_ds);
-  ws->lb_sort = reinterpret_cast<uint64_t*>(base + o_lbs);
-  ws->tile_counts = reinterpret_cast<uint32_t*>(base + o_tc);
-  ws->lb_unique = reinterpret_cast<uint64_t*>(base + o_lbu);
-  ws->lb_scan = reinterpret_cast<uint64_t*>(base + o_lbc);
-  ws->ctrl = reinterpret_cast<uint32_t*>(base + o_ctrl);
-  ws->counts = reinterpret_cast<uint32_t*>(base + o_cnt);
-  ws->tables = reinterpret_cast<uint32_t*>(base + o_tab);
-  ws->tree_list = reinterpret_cast<uint32_t*>(base + o_tl);
-  ws->tree_sums = reinterpret_cast<uint32_t*>(base + o_ts);
+  bind_workspace(ws, base, L, max_points, max_segments);
   // Look-back words, histograms and counters must start at zero; later
   // launches keep them consistent by themselves.
-  e = hipMemset(base, 0, off);
+  e = hipMemset(base, 0, L.bytes);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ws->h_stage),
                                          sizeof(uint32_t) * ws->table_words(), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ws->h_counts),
@@ -471,8 +528,8 @@ int rwh_unique_u32(rwh_workspace* ws, const uint32_t* d_sorted, uint32_t n, uint
   RWH_CHECK(d_n_unique, "null count pointer");
   const hipStream_t st = static_cast<hipStream_t>(stream);
   if (n == 0) {
-    RWH_HIP(hipMemsetAsync(d_n_unique, 0, sizeof(uint32_t), st));
-    return RWH_OK;
+    launch_fill_u32(d_n_unique, 0, 1, st);
+    return launched();
   }
   RWH_CHECK(d_sorted && d_unique, "null buffer");
   RWH_CHECK(aligned16(d_sorted), "keys must be 16-byte aligned");
@@ -519,8 +576,8 @@ int rwh_prefix_sum(rwh_workspace* ws, const int32_t* d_edge_count, const uint32_
   RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
   const hipStream_t st = static_cast<hipStream_t>(stream);
   if (n_max == 0) {
-    RWH_HIP(hipMemsetAsync(d_n_octree_nodes, 0, sizeof(uint32_t), st));
-    return RWH_OK;
+    launch_fill_u32(d_n_octree_nodes, 0, 1, st);
+    return launched();
   }
   RWH_CHECK(d_edge_count && d_edge_offset, "null buffer");
   RWH_CHECK(aligned16(d_edge_count) && aligned16(d_edge_offset), "buffers must be 16-byte aligned");
@@ -558,38 +615,83 @@ int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
 int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
                      float range, rwh_stream_t stream) {
   RWH_TRY(check_ws(ws, n));
+  return build_single(ws, bufs, n, min_coord, range, ws->n_unique(), ws->n_oct(), static_cast<hipStream_t>(stream));
+}
+
+int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
+                             uint32_t* d_counts, rwh_stream_t stream) {
+  RWH_CHECK(d_counts, "null counts pointer");
+  if (n > RWH_NOALLOC_MAX_POINTS)
+    return fail(RWH_ERR_CAPACITY, "n = " + std::to_string(n) + " exceeds RWH_NOALLOC_MAX_POINTS (" +
+                                      std::to_string(RWH_NOALLOC_MAX_POINTS) + ")");
   RWH_TRY(check_bufs(bufs));
-  uint32_t* d_nu = ws->n_unique();
-  uint32_t* d_no = ws->n_oct();
+  int dev = 0;
+  RWH_HIP(hipGetDevice(&dev));
+  RWH_CHECK(dev >= 0 && dev < kNoallocDevices, "device index above the library's table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  NoallocDevice& d = g_noalloc[dev];
+  if (!d.base) {
+    // set-up, once per device with that device current: the scratch array's
+    // address on this device
+    void* p = nullptr;
+    RWH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    d.base = static_cast<char*>(p);
+  }
+  rwh_workspace ws;  // a view of the scratch array: nothing allocated
+  bind_workspace(&ws, d.base, kNoallocLayout, RWH_NOALLOC_MAX_POINTS, 1);
+  ws.sort_variant = kNoallocSortVariant;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (d.dirty) {
+    // an earlier call stopped between launches: its histogram copies may
+    // hold counts the digit scan never cleared
+    launch_fill_u32(ws.hist, 0, kHistCopies * kMaxPasses * 256, st);
+    RWH_TRY(launched());
+    d.dirty = false;
+  }
+  const int rc = build_single(&ws, bufs, n, min_coord, range, d_counts, d_counts + 1, st);
+  if (rc != RWH_OK) d.dirty = true;
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
+                 uint32_t* d_nu, uint32_t* d_no, hipStream_t st) {
+  RWH_TRY(check_ws(ws, n));
+  RWH_TRY(check_bufs(bufs));
   // S1 fused with the sort's histogram sweep, then the digit passes
   if (injected(1) || injected(2)) return RWH_ERR_INJECTED;
   RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
   if (n > 0) {
-    const hipStream_t st = static_cast<hipStream_t>(stream);
     launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n,
                        min_coord, range, 30, ws->hist, ws->sort_variant, ws->tile_counts, st);
     RWH_TRY(launched());
     RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
                          st, true));
   }
-  RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, stream));
+  RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, st));
   if (injected(4) || injected(5)) return RWH_ERR_INJECTED;
   if (n == 0) {
-    RWH_HIP(hipMemsetAsync(d_no, 0, sizeof(uint32_t), static_cast<hipStream_t>(stream)));
+    launch_fill_u32(d_no, 0, 1, st);
+    RWH_TRY(launched());
   } else {
     // S4 with S5: the tree kernels write the edge counts beside the tree
     // (Morton codes are below 2^31) and their sums per tree tile; S6 scans
     // the counts, each tile's prefix from those sums
     TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
                 bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums};
-    launch_radix_tree(ta, single_seg_dev(d_nu), n, ws->tree_list, static_cast<hipStream_t>(stream));
-    launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->tree_sums,
-                            static_cast<hipStream_t>(stream));
+    launch_radix_tree(ta, single_seg_dev(d_nu), n, ws->tree_list, st);
+    launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->tree_sums, st);
     RWH_TRY(launched());
   }
-  RWH_TRY(rwh_octree(bufs, d_nu, d_no, n, min_coord, range, stream));
-  return RWH_OK;
+  return rwh_octree(bufs, d_nu, d_no, n, min_coord, range, st);
 }
+
+}  // namespace
+
+extern "C" {
 
 int rwh_read_counts(rwh_workspace* ws, uint32_t oct_capacity, uint32_t* counts3,
                     rwh_stream_t stream) {

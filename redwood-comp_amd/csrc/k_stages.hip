@@ -38,6 +38,20 @@ void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_i
                      make_quant(min_coord, range));
 }
 
+// dst[i] = value for i < count (counts of empty builds, state cleared by
+// kernels rather than memset nodes).
+__global__ __launch_bounds__(kElemBlock) void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value,
+                                                         uint32_t count) {
+  const uint32_t i = blockIdx.x * kElemBlock + threadIdx.x;
+  if (i < count) dst[i] = value;
+}
+
+void launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t stream) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((count + kElemBlock - 1) / kElemBlock), dim3(kElemBlock), 0, stream,
+                     dst, value, count);
+}
+
 // ---------------------------------------------------------------------------
 // Tile-wide exclusive scan for BLOCK threads.
 // ---------------------------------------------------------------------------

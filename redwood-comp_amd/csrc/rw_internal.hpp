@@ -82,7 +82,7 @@ constexpr int kCrossBlock = 256;  // four waves: 32 nodes at a time
 static_assert(kChunk % kTreeTile == 0, "a tree tile never straddles two clouds");
 static_assert(kTreeTile < 0xffff, "LDS node offsets are 16-bit with 0xffff reserved");
 static_assert(kWave % kCrossGroup == 0 && kCrossBracket * (kCrossGroup - 1) <= 30, "bracket probes below 2^31");
-inline uint32_t tree_list_words(uint32_t n_index) {
+constexpr uint32_t tree_list_words(uint32_t n_index) {
   return ((n_index + kTreeTile - 1) / kTreeTile) * static_cast<uint32_t>(kTreeListStride);
 }
 
@@ -172,7 +172,7 @@ void launch_prefix_sum(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_
 // known: each tile adds up the sums of the segment's earlier tree tiles.
 void launch_prefix_sum_tiles(const int32_t* in, int32_t* out, SegArgs seg, uint32_t n_index,
                              uint32_t* total, const uint32_t* tile_sum, hipStream_t stream);
-inline uint32_t tree_tiles(uint32_t n_index) { return (n_index + kTreeTile - 1) / kTreeTile; }
+constexpr uint32_t tree_tiles(uint32_t n_index) { return (n_index + kTreeTile - 1) / kTreeTile; }
 
 struct OctArgs {
   const int32_t* edge_offset;
@@ -195,6 +195,9 @@ struct OctArgs {
   int32_t* child_leaf_mask;
 };
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
+// dst[0, count) = value with a kernel (no memset node, so a captured graph
+// replays it as it ran)
+void launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t stream);
 void launch_link_leaf(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
 
 }  // namespace rw
