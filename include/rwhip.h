@@ -35,6 +35,18 @@ extern "C" {
 #define RWH_ERR_CAPACITY 3    /* size above the workspace, or octree above capacity */
 #define RWH_ERR_NO_DEVICE 4   /* no usable GPU */
 #define RWH_ERR_INJECTED 5    /* failure injected by rwh_debug_fail_next */
+#define RWH_ERR_OVERFLOW 6    /* a kernel stopped at one of its bounds: malformed input (status bits below) */
+
+/* Status word of a build (one per cloud, next to its counts): a kernel ORs a
+ * bit in when it stops at a bound instead of finishing its work.  Each bound
+ * holds for every tree built from sorted unique codes; a set bit means the
+ * outputs of that cloud are not valid.  RWH_ERR_OVERFLOW reports any of them. */
+#define RWH_STATUS_PAIR_CAP 0x1u   /* S4: a wave's queue of leaf pairs overflowed */
+#define RWH_STATUS_CLIMB 0x2u      /* S4: a climb stopped at its iteration bound */
+#define RWH_STATUS_CROSS_CAP 0x4u  /* S4: a tile had more edge-crossing nodes than its list holds */
+#define RWH_STATUS_KEY_ORDER 0x8u  /* S4: keys not strictly increasing (the tile was skipped) */
+#define RWH_STATUS_PULL 0x10u      /* S7: a link walk ended at its step or stack bound, or left [0, n_brt) */
+#define RWH_STATUS_LEAF 0x20u      /* leaf linking: a leaf or parent index outside the tree */
 
 typedef void* rwh_stream_t; /* a hipStream_t */
 typedef void* rwh_event_t;  /* a hipEvent_t */
@@ -76,7 +88,8 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
 int rwh_workspace_destroy(rwh_workspace* ws);
 size_t rwh_workspace_bytes(const rwh_workspace* ws);
 /* Device array of per-cloud counts written by S3 and S6: n_unique of cloud s
- * at [s], n_octree_nodes at [max_segments + s] (a single build is cloud 0). */
+ * at [s], n_octree_nodes at [max_segments + s], the status word (RWH_STATUS_*)
+ * at [2 * max_segments + s] (a single build is cloud 0). */
 uint32_t* rwh_workspace_counts(rwh_workspace* ws);
 uint32_t rwh_workspace_max_segments(const rwh_workspace* ws);
 
@@ -117,7 +130,9 @@ int rwh_unique_u32(rwh_workspace* ws, const uint32_t* d_sorted, uint32_t n, uint
  * read from device memory; n_max bounds it (buffers hold n_max entries, and
  * n_max is at most the workspace's max_points).  The workspace holds the
  * per-tile lists of nodes that cross a tile edge.  The five node buffers are
- * 16-byte aligned. */
+ * 16-byte aligned.  The codes must be sorted and unique: cloud 0's status
+ * word (cleared here) gets RWH_STATUS_KEY_ORDER otherwise, and
+ * rwh_read_counts reports it. */
 int rwh_radix_tree(rwh_workspace* ws, const uint32_t* d_codes, const uint32_t* d_n_unique, uint32_t n_max,
                    uint8_t* d_prefix_n, uint8_t* d_has_leaf_left, uint8_t* d_has_leaf_right,
                    int32_t* d_left_child, int32_t* d_parents, rwh_stream_t stream);
@@ -161,15 +176,19 @@ typedef struct rwh_tree_buffers {
 /* ---- S7  (reference: cpu::run_stage7 host_dispatchers.cpp:303-336,
  *          cuda::run_stage7 cu_dispatcher.cu:692-752)
  * Octree nodes for brt nodes [1, n_unique-1); rows [0, min(n_oct, cap)) of
- * children / masks are rewritten, corner / cell_size of those slots are set. */
+ * children / masks are rewritten, corner / cell_size of those slots are set.
+ * d_status: a device word that gets RWH_STATUS_PULL OR-ed in when the radix
+ * tree is malformed (NULL: not reported; the caller clears it). */
 int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
                const uint32_t* d_n_octree_nodes, uint32_t n_max, float min_coord, float range,
-               rwh_stream_t stream);
+               uint32_t* d_status, rwh_stream_t stream);
 
 /* Leaf linking: the reference's k_LinkLeafNodes (07_octree.cu:228-258), which
- * it never launches.  Opt-in; runs after rwh_octree. */
+ * it never launches.  Opt-in; runs after rwh_octree.  d_status as for
+ * rwh_octree (RWH_STATUS_LEAF). */
 int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
-                  const uint32_t* d_n_octree_nodes, uint32_t n_max, rwh_stream_t stream);
+                  const uint32_t* d_n_octree_nodes, uint32_t n_max, uint32_t* d_status,
+                  rwh_stream_t stream);
 
 /* ---- whole build (the reference's stage loop, pipe_main_baseline.cpp:23-31)
  * Enqueues S1..S7 for n points on the stream; the counts of cloud 0 go to
@@ -182,8 +201,8 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
  * rwh_build_octree, with its workspace in a static device array of this
  * library (about 19 MiB per GPU, held until the process ends), so at most
  * RWH_NOALLOC_MAX_POINTS points (larger n: RWH_ERR_CAPACITY, nothing
- * enqueued).  d_counts (device, 2 words) gets {n_unique, n_octree_nodes};
- * compare n_octree_nodes with bufs->oct_capacity.  Calls on one GPU share that
+ * enqueued).  d_counts (device, 3 words) gets {n_unique, n_octree_nodes,
+ * status word (RWH_STATUS_*)}; compare n_octree_nodes with bufs->oct_capacity.  Calls on one GPU share that
  * array: calls that overlap on one GPU (two streams, two threads) race.  The
  * sort runs its 1024 x 16 tile shape. */
 #define RWH_NOALLOC_MAX_POINTS (1u << 22)
@@ -191,7 +210,9 @@ int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min
                              uint32_t* d_counts, rwh_stream_t stream);
 
 /* Waits for the stream and copies {n_unique, n_brt, n_octree_nodes} of cloud
- * 0 to host; RWH_ERR_CAPACITY if n_octree_nodes exceeds the capacity given. */
+ * 0 to host; RWH_ERR_OVERFLOW if cloud 0's status word is set (the message
+ * names the bounds), else RWH_ERR_CAPACITY if n_octree_nodes exceeds the
+ * capacity given. */
 int rwh_read_counts(rwh_workspace* ws, uint32_t oct_capacity, uint32_t* counts3,
                     rwh_stream_t stream);
 
@@ -231,7 +252,8 @@ int rwh_graph_launch(rwh_graph_t graph, rwh_stream_t stream);
 int rwh_graph_destroy(rwh_graph_t graph);
 
 /* Waits and copies the counts of every cloud: n_unique to counts[s],
- * n_octree_nodes to counts[n_clouds + s]. */
+ * n_octree_nodes to counts[n_clouds + s]; RWH_ERR_OVERFLOW if a cloud's
+ * status word is set (the counts are copied either way). */
 int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts,
                           rwh_stream_t stream);
 

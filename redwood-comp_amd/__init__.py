@@ -32,6 +32,10 @@ RWH_ERR_HIP = 2
 RWH_ERR_CAPACITY = 3
 RWH_ERR_NO_DEVICE = 4
 RWH_ERR_INJECTED = 5
+RWH_ERR_OVERFLOW = 6
+# status bits of a build (RWH_STATUS_* of include/rwhip.h)
+STATUS_PAIR_CAP, STATUS_CLIMB, STATUS_CROSS_CAP = 0x1, 0x2, 0x4
+STATUS_KEY_ORDER, STATUS_PULL, STATUS_LEAF = 0x8, 0x10, 0x20
 NOALLOC_MAX_POINTS = 1 << 22  # RWH_NOALLOC_MAX_POINTS
 
 # Every function of include/rwhip.h: name -> (restype, argtypes).
@@ -78,8 +82,8 @@ ABI = {
     "rwh_radix_tree": (_I, [_P, _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
     "rwh_edge_count": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rwh_prefix_sum": (_I, [_P, _P, _P, _U32, _P, _P, _P]),
-    "rwh_octree": (_I, [_P, _P, _P, _U32, _F, _F, _P]),
-    "rwh_link_leaf": (_I, [_P, _P, _P, _U32, _P]),
+    "rwh_octree": (_I, [_P, _P, _P, _U32, _F, _F, _P, _P]),
+    "rwh_link_leaf": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rwh_build_octree": (_I, [_P, _P, _U32, _F, _F, _P]),
     "rwh_build_octree_noalloc": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_read_counts": (_I, [_P, _U32, C.POINTER(_U32), _P]),
@@ -299,8 +303,12 @@ class Workspace:
     def n_oct_ptr(self, s: int = 0) -> int:
         return self.counts_ptr + 4 * (self.max_segments + s)
 
+    def status_ptr(self, s: int = 0) -> int:
+        return self.counts_ptr + 4 * (2 * self.max_segments + s)
+
     def counts(self, n_clouds: int = 1) -> tuple[np.ndarray, np.ndarray]:
-        out = np.empty(2 * self.max_segments, dtype=np.uint32)
+        """(n_unique, n_octree_nodes) of the first n_clouds clouds."""
+        out = np.empty(3 * self.max_segments, dtype=np.uint32)
         check(lib().rwh_memcpy_d2h(out.ctypes.data, self.counts_ptr, out.nbytes, None), "rwh_memcpy_d2h")
         check(lib().rwh_stream_sync(None), "rwh_stream_sync")
         return out[:n_clouds], out[self.max_segments:self.max_segments + n_clouds]
@@ -449,7 +457,8 @@ def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
     tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
     check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
     if link_leaf:
-        check(lib().rwh_link_leaf(tb.ptr(), ws.n_unique_ptr(), ws.n_oct_ptr(), n, stream), "rwh_link_leaf")
+        check(lib().rwh_link_leaf(tb.ptr(), ws.n_unique_ptr(), ws.n_oct_ptr(), n, ws.status_ptr(), stream),
+              "rwh_link_leaf")
     counts = (_U32 * 3)()
     st = lib().rwh_read_counts(ws.ptr, cap, counts, stream)
     if strict or st != RWH_ERR_CAPACITY:

@@ -43,6 +43,24 @@ int fail(int status, const std::string& msg) {
     if (s_ != RWH_OK) return s_;  \
   } while (0)
 
+// The names of the bounds set in a status word.
+std::string status_text(uint32_t bits) {
+  static const struct {
+    uint32_t bit;
+    const char* what;
+  } kNames[] = {{RWH_STATUS_PAIR_CAP, "radix tree: leaf-pair queue full"},
+                {RWH_STATUS_CLIMB, "radix tree: climb bound reached"},
+                {RWH_STATUS_CROSS_CAP, "radix tree: edge-crossing list full"},
+                {RWH_STATUS_KEY_ORDER, "radix tree: keys not sorted and unique"},
+                {RWH_STATUS_PULL, "octree: link walk bound reached or index outside the tree"},
+                {RWH_STATUS_LEAF, "leaf linking: index outside the tree"}};
+  std::string out;
+  for (const auto& k : kNames)
+    if (bits & k.bit) out += (out.empty() ? "" : "; ") + std::string(k.what);
+  if (bits & ~0x3fu) out += (out.empty() ? "" : "; ") + std::string("unknown bits");
+  return out;
+}
+
 int injected(int stage) {
   if (g_fail_stage < 0) return 0;
   if (g_fail_stage == 0 || g_fail_stage == stage) {
@@ -95,7 +113,7 @@ struct rwh_workspace {
   uint64_t* lb_unique = nullptr;
   uint64_t* lb_scan = nullptr;
   uint32_t* ctrl = nullptr;    // 3 users x kCtrlWords
-  uint32_t* counts = nullptr;  // n_unique[max_segs], n_oct[max_segs]
+  uint32_t* counts = nullptr;  // n_unique[max_segs], n_oct[max_segs], status[max_segs]
   uint32_t* tables = nullptr;  // seg_base, seg_n, oct_base, oct_cap [max_segs each], chunk_seg
   uint32_t* tree_list = nullptr;  // radix tree: edge-crossing nodes per tile
   uint32_t* tree_sums = nullptr;  // radix tree: edge counts summed per tile (build path)
@@ -115,6 +133,7 @@ struct rwh_workspace {
   size_t table_words() const { return 4ull * max_segs + max_chunks; }
   uint32_t* n_unique() { return counts; }
   uint32_t* n_oct() { return counts + max_segs; }
+  uint32_t* status() { return counts + 2 * max_segs; }
 };
 
 namespace {
@@ -149,7 +168,7 @@ constexpr WsLayout ws_layout(uint32_t max_points, uint32_t max_segs) {
   L.lbu = take(sizeof(uint64_t) * static_cast<size_t>(L.max_scan_tiles));
   L.lbc = take(sizeof(uint64_t) * static_cast<size_t>(L.max_scan_tiles));
   L.ctrl = take(sizeof(uint32_t) * 3 * kCtrlWords);
-  L.cnt = take(sizeof(uint32_t) * 2ull * max_segs);
+  L.cnt = take(sizeof(uint32_t) * 3ull * max_segs);
   L.tab = take(sizeof(uint32_t) * (4ull * max_segs + L.max_chunks));
   L.tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
   L.ts = take(sizeof(uint32_t) * static_cast<size_t>(tree_tiles(mp)));
@@ -274,9 +293,10 @@ int check_bufs(const rwh_tree_buffers* b) {
 }
 
 // The whole build of one cloud on a workspace (or a view of the scratch
-// array); n_unique and n_octree_nodes go to d_nu and d_no (device).
+// array); n_unique, n_octree_nodes and the status word go to d_nu, d_no and
+// d_status (device).
 int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
-                 uint32_t* d_nu, uint32_t* d_no, hipStream_t st);
+                 uint32_t* d_nu, uint32_t* d_no, uint32_t* d_status, hipStream_t st);
 
 // Per-device state of the allocation-free build.
 constexpr int kNoallocDevices = 64;
@@ -299,6 +319,7 @@ const char* rwh_error_string(int status) {
     case RWH_ERR_CAPACITY: return "capacity exceeded";
     case RWH_ERR_NO_DEVICE: return "no GPU device";
     case RWH_ERR_INJECTED: return "injected failure";
+    case RWH_ERR_OVERFLOW: return "a kernel bound was exceeded (malformed input)";
     default: return "unknown status";
   }
 }
@@ -452,7 +473,7 @@ int rwh_workspace_create(uint32_t max_points, uint32_t max_segments, rwh_workspa
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ws->h_stage),
                                          sizeof(uint32_t) * ws->table_words(), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&ws->h_counts),
-                                         sizeof(uint32_t) * 2ull * max_segments, hipHostMallocDefault);
+                                         sizeof(uint32_t) * 3ull * max_segments, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ws->upload_done, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -504,7 +525,7 @@ int rwh_morton(const float* d_xyzw, uint32_t n, float min_coord, float range, ui
   RWH_CHECK(d_xyzw && d_codes, "null buffer");
   RWH_CHECK(aligned16(d_xyzw), "points must be 16-byte aligned");
   launch_morton(reinterpret_cast<const float4*>(d_xyzw), d_codes, single_seg(n), n, min_coord,
-                range, static_cast<hipStream_t>(stream));
+                range, nullptr, static_cast<hipStream_t>(stream));
   return launched();
 }
 
@@ -553,7 +574,10 @@ int rwh_radix_tree(rwh_workspace* ws, const uint32_t* d_codes, const uint32_t* d
                 aligned16(d_left_child) && aligned16(d_parents),
             "node buffers must be 16-byte aligned");
   TreeArgs a{d_codes, d_prefix_n, d_has_leaf_left, d_has_leaf_right, d_left_child, d_parents};
-  launch_radix_tree(a, single_seg_dev(d_n_unique), n_max, ws->tree_list, static_cast<hipStream_t>(stream));
+  a.status = ws->status();
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  launch_fill_u32(a.status, 0, 1, st);
+  launch_radix_tree(a, single_seg_dev(d_n_unique), n_max, ws->tree_list, st);
   return launched();
 }
 
@@ -588,7 +612,7 @@ int rwh_prefix_sum(rwh_workspace* ws, const int32_t* d_edge_count, const uint32_
 
 int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
                const uint32_t* d_n_octree_nodes, uint32_t n_max, float min_coord, float range,
-               rwh_stream_t stream) {
+               uint32_t* d_status, rwh_stream_t stream) {
   if (injected(7)) return RWH_ERR_INJECTED;
   RWH_TRY(check_bufs(bufs));
   RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
@@ -596,18 +620,21 @@ int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
   if (n_max == 0) return RWH_OK;
   OctArgs a = oct_args(bufs, min_coord, range);
   a.n_oct = d_n_octree_nodes;
+  a.status = d_status;
   launch_octree(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
   return launched();
 }
 
 int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
-                  const uint32_t* d_n_octree_nodes, uint32_t n_max, rwh_stream_t stream) {
+                  const uint32_t* d_n_octree_nodes, uint32_t n_max, uint32_t* d_status,
+                  rwh_stream_t stream) {
   RWH_TRY(check_bufs(bufs));
   RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
   RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
   if (n_max == 0) return RWH_OK;
   OctArgs a = oct_args(bufs, 0.0f, 1.0f);
   a.n_oct = d_n_octree_nodes;
+  a.status = d_status;
   launch_link_leaf(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
   return launched();
 }
@@ -615,7 +642,8 @@ int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
 int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
                      float range, rwh_stream_t stream) {
   RWH_TRY(check_ws(ws, n));
-  return build_single(ws, bufs, n, min_coord, range, ws->n_unique(), ws->n_oct(), static_cast<hipStream_t>(stream));
+  return build_single(ws, bufs, n, min_coord, range, ws->n_unique(), ws->n_oct(), ws->status(),
+                      static_cast<hipStream_t>(stream));
 }
 
 int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
@@ -648,7 +676,7 @@ int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min
     RWH_TRY(launched());
     d.dirty = false;
   }
-  const int rc = build_single(&ws, bufs, n, min_coord, range, d_counts, d_counts + 1, st);
+  const int rc = build_single(&ws, bufs, n, min_coord, range, d_counts, d_counts + 1, d_counts + 2, st);
   if (rc != RWH_OK) d.dirty = true;
   return rc;
 }
@@ -658,7 +686,7 @@ int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min
 namespace {
 
 int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
-                 uint32_t* d_nu, uint32_t* d_no, hipStream_t st) {
+                 uint32_t* d_nu, uint32_t* d_no, uint32_t* d_status, hipStream_t st) {
   RWH_TRY(check_ws(ws, n));
   RWH_TRY(check_bufs(bufs));
   // S1 fused with the sort's histogram sweep, then the digit passes
@@ -666,7 +694,7 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
   if (n > 0) {
     launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n,
-                       min_coord, range, 30, ws->hist, ws->sort_variant, ws->tile_counts, st);
+                       min_coord, range, 30, ws->hist, ws->sort_variant, ws->tile_counts, d_status, st);
     RWH_TRY(launched());
     RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
                          st, true));
@@ -675,18 +703,19 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   if (injected(4) || injected(5)) return RWH_ERR_INJECTED;
   if (n == 0) {
     launch_fill_u32(d_no, 0, 1, st);
+    launch_fill_u32(d_status, 0, 1, st);
     RWH_TRY(launched());
   } else {
     // S4 with S5: the tree kernels write the edge counts beside the tree
     // (Morton codes are below 2^31) and their sums per tree tile; S6 scans
     // the counts, each tile's prefix from those sums
     TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
-                bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums};
+                bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums, d_status};
     launch_radix_tree(ta, single_seg_dev(d_nu), n, ws->tree_list, st);
     launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->tree_sums, st);
     RWH_TRY(launched());
   }
-  return rwh_octree(bufs, d_nu, d_no, n, min_coord, range, st);
+  return rwh_octree(bufs, d_nu, d_no, n, min_coord, range, d_status, st);
 }
 
 }  // namespace
@@ -699,10 +728,14 @@ int rwh_read_counts(rwh_workspace* ws, uint32_t oct_capacity, uint32_t* counts3,
   const hipStream_t st = static_cast<hipStream_t>(stream);
   RWH_HIP(hipMemcpyAsync(ws->h_counts, ws->n_unique(), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   RWH_HIP(hipMemcpyAsync(ws->h_counts + 1, ws->n_oct(), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipMemcpyAsync(ws->h_counts + 2, ws->status(), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   RWH_HIP(hipStreamSynchronize(st));
   counts3[0] = ws->h_counts[0];
   counts3[1] = ws->h_counts[0] > 0 ? ws->h_counts[0] - 1 : 0;
   counts3[2] = ws->h_counts[1];
+  if (ws->h_counts[2] != 0)
+    return fail(RWH_ERR_OVERFLOW, "build status " + std::to_string(ws->h_counts[2]) + ": " +
+                                      status_text(ws->h_counts[2]));
   if (counts3[2] > oct_capacity)
     return fail(RWH_ERR_CAPACITY, "n_octree_nodes = " + std::to_string(counts3[2]) +
                                       " exceeds the octree capacity " + std::to_string(oct_capacity));
@@ -745,7 +778,7 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
   uni.count = ws->n_unique();
 
   launch_morton(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, pts, n_index,
-                min_coord, range, st);
+                min_coord, range, ws->status(), st);
   RWH_TRY(launched());
   RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, pts, n_clouds, n_index, 30,
                        false, st));
@@ -753,13 +786,14 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
                 ws->lb_unique, ws->ctrl + 1 * kCtrlWords, st);
   RWH_TRY(launched());
   TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
-              bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums};
+              bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums, ws->status()};
   launch_radix_tree(ta, uni, n_index, ws->tree_list, st);
   launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, uni, n_index, ws->n_oct(), ws->tree_sums, st);
   OctArgs oa = oct_args(bufs, min_coord, range);
   oa.n_oct = ws->n_oct();
   oa.oct_base = ws->oct_base();
   oa.oct_cap = ws->oct_cap();
+  oa.status = ws->status();
   launch_octree(oa, uni, n_index, st);
   return launched();
 }
@@ -887,9 +921,17 @@ int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts
                          hipMemcpyDeviceToHost, st));
   RWH_HIP(hipMemcpyAsync(ws->h_counts + ws->max_segs, ws->n_oct(), sizeof(uint32_t) * n_clouds,
                          hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipMemcpyAsync(ws->h_counts + 2 * ws->max_segs, ws->status(), sizeof(uint32_t) * n_clouds,
+                         hipMemcpyDeviceToHost, st));
   RWH_HIP(hipStreamSynchronize(st));
   std::memcpy(counts, ws->h_counts, sizeof(uint32_t) * n_clouds);
   std::memcpy(counts + n_clouds, ws->h_counts + ws->max_segs, sizeof(uint32_t) * n_clouds);
+  for (uint32_t s = 0; s < n_clouds; ++s) {
+    const uint32_t bits = ws->h_counts[2 * ws->max_segs + s];
+    if (bits != 0)
+      return fail(RWH_ERR_OVERFLOW, "cloud " + std::to_string(s) + ": build status " + std::to_string(bits) +
+                                        ": " + status_text(bits));
+  }
   return RWH_OK;
 }
 

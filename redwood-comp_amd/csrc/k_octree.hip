@@ -72,27 +72,38 @@ __device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c,
             a.prefix_n[base + c], row, mask);
 }
 
-// Steps a pull walk may take.  On a tree built from sorted unique codes a
-// walk visits at most two levels of count-0 nodes (a handful of steps); the
-// budget only ends walks over malformed trees handed in through the C ABI
-// (a cycle of left_child links), which would otherwise never end.
+// Steps the pull of one slot may take, all its walks together.  On a tree
+// built from sorted unique codes a walk visits at most two levels of count-0
+// nodes (a handful of steps); the budget only ends walks over malformed trees
+// handed in through the C ABI (a cycle of left_child links), which would
+// otherwise never end, and such a walk reports RWH_STATUS_PULL.
 constexpr int kPullBudget = 1024;
 
 // Pulls every chain reached from the given roots through nodes of count 0
-// (depth-first, one node per step).
-__device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int* stack, int sp,
-                                           int32_t (&row)[8], int32_t& mask) {
+// (depth-first, one node per step, each step taken from budget); nodes
+// outside [0, n), a full stack or an exhausted budget set RWH_STATUS_PULL in
+// flags.
+__device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int n, int* stack, int sp,
+                                           int32_t (&row)[8], int32_t& mask, uint32_t& flags, int& budget) {
   constexpr int kStack = 16;
-  for (int steps = 0; sp > 0 && steps < kPullBudget; ++steps) {
+  for (; sp > 0 && budget > 0; --budget) {
     const int q = stack[--sp];
+    if (q < 0 || q >= n) {
+      flags |= RWH_STATUS_PULL;
+      continue;
+    }
     if (a.edge_count[base + q] > 0) {
       add_link(a, base, q, row, mask);
       continue;
     }
     const int lc = a.left_child[base + q];
-    if (!a.has_leaf_left[base + q] && sp < kStack) stack[sp++] = lc;
-    if (!a.has_leaf_right[base + q] && sp < kStack) stack[sp++] = lc + 1;
+    for (int side = 0; side < 2; ++side) {
+      if (side == 0 ? a.has_leaf_left[base + q] : a.has_leaf_right[base + q]) continue;
+      if (sp < kStack) stack[sp++] = lc + side;
+      else flags |= RWH_STATUS_PULL;
+    }
   }
+  if (sp > 0) flags |= RWH_STATUS_PULL;  // the budget ended the walk
 }
 
 __device__ __forceinline__ int delta8o(uint32_t x, uint32_t y) {
@@ -225,18 +236,20 @@ __device__ __forceinline__ int32_t merge_row(const int32_t (&r)[8], int32_t m, i
 
 // Depth-first pull from global memory below node q (count 0), for a walk
 // deeper than the pull stack (not reached below a node of count > 0).
-__device__ __forceinline__ int32_t pull_overflow(const OctArgs& a, uint32_t base, int q, int32_t* row) {
+__device__ __forceinline__ int32_t pull_overflow(const OctArgs& a, uint32_t base, int n, int q, int32_t* row,
+                                                 uint32_t& flags, int& budget) {
   int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t m = 0;
   int stack[16];
   stack[0] = q;
-  pull_links(a, base, stack, 1, r, m);
+  pull_links(a, base, n, stack, 1, r, m, flags, budget);
   return merge_row(r, m, row);
 }
 
 // Node 0's links (slot 0 is also offset[0]): every chain reached from the
 // children of the complete-range root through nodes of count 0.
-__device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, int n, int32_t* row) {
+__device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, int n, int32_t* row,
+                                             uint32_t& flags) {
   int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t m = 0;
   const int g = full_root_split(a.codes + base, n);
@@ -244,7 +257,8 @@ __device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, in
   int sp = 0;
   if (g != 0) stack[sp++] = g;
   if (g + 1 < n) stack[sp++] = g + 1;
-  pull_links(a, base, stack, sp, r, m);
+  int budget = kPullBudget;
+  pull_links(a, base, n, stack, sp, r, m, flags, budget);
   return merge_row(r, m, row);
 }
 
@@ -260,15 +274,25 @@ constexpr int kPullStack = 4;
 // level), so at most three pairs wait on the stack; a deeper walk is still
 // handled, from global memory.
 __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, const Window& w,
-                                             int n, uint32_t pair, int32_t* row, uint32_t* stk) {
+                                             int n, uint32_t pair, int32_t* row, uint32_t* stk,
+                                             uint32_t& flags) {
   int32_t m = 0;
   int sp = 0;
-  for (int steps = 0; steps < kPullBudget; ++steps) {
+  int budget = kPullBudget;
+  for (;; --budget) {
+    if (budget <= 0) {  // a malformed tree: the walk is cut short
+      flags |= RWH_STATUS_PULL;
+      break;
+    }
     const int c = static_cast<int>(pair & 0x3fffffffu);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int q = c + side;
-      if (((pair >> (30 + side)) & 1u) || q >= n) continue;  // a leaf (or no such node)
+      if ((pair >> (30 + side)) & 1u) continue;  // a leaf
+      if (q >= n) {  // a child index outside the tree
+        flags |= RWH_STATUS_PULL;
+        continue;
+      }
       uint2 d;
       if (q >= w.w0 && q < w.w1) d = w.desc[q - w.w0];
       else d = desc_global(a, base + static_cast<uint32_t>(q));
@@ -278,7 +302,7 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
       } else if (sp < kPullStack) {
         stk[(sp++) * kOctBlock] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
       } else {
-        m |= pull_overflow(a, base, q, row);
+        m |= pull_overflow(a, base, n, q, row, flags, budget);
       }
     }
     if (sp == 0) break;
@@ -328,6 +352,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   int32_t* mask = a.child_node_mask + os.base;
   int32_t* leaf_mask = a.child_leaf_mask + os.base;
   const int nv = min(kOctSpan, n - b);  // valid nodes of the tile
+  uint32_t flags = 0;  // RWH_STATUS_* bits of this thread's walks
 
   Window w;
   w.desc = s_desc;
@@ -407,8 +432,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     } else {  // the bottom: pull the chains that link here
 #pragma unroll
       for (int e = 0; e < 8; ++e) row[e * kOctBlock] = 0;
-      m = pull_pair(a, sg.base, w, n, s_lcf[k], row, stk);
-      if (slot == 0) m |= pull_root(a, sg.base, n, row);
+      m = pull_pair(a, sg.base, w, n, s_lcf[k], row, stk, flags);
+      if (slot == 0) m |= pull_root(a, sg.base, n, row, flags);
       r0 = make_int4(row[0], row[kOctBlock], row[2 * kOctBlock], row[3 * kOctBlock]);
       r1 = make_int4(row[4 * kOctBlock], row[5 * kOctBlock], row[6 * kOctBlock], row[7 * kOctBlock]);
     }
@@ -417,6 +442,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     mask[slot] = m;
     leaf_mask[slot] = 0;
   }
+  if (flags && a.status) atomicOr(&a.status[sg.id], flags);
 }
 
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
@@ -439,24 +465,43 @@ __global__ __launch_bounds__(kElemBlock) void k_link_leaf(OctArgs a, SegArgs seg
   const int32_t* counts = a.edge_count + sg.base;
   const int32_t* parent = a.parent + sg.base;
   int32_t* children = a.children + static_cast<size_t>(os.base) * 8;
+  const int n = static_cast<int>(sg.count);
   const int lvl = a.prefix_n[sg.base + i] / 3 + 1;
   const uint32_t sh = level_shift(lvl);
+  uint32_t flags = 0;
   for (int side = 0; side < 2; ++side) {
     const bool is_leaf = side == 0 ? a.has_leaf_left[sg.base + i] : a.has_leaf_right[sg.base + i];
     if (!is_leaf) continue;
     const int leaf = a.left_child[sg.base + i] + side;
+    if (leaf < 0 || leaf > n) {  // keys are [0, n]: a malformed tree
+      flags |= RWH_STATUS_LEAF;
+      continue;
+    }
     const uint32_t which = (a.codes[sg.base + leaf] >> sh) & 7u;
+    // the walk stops after 31 steps (func_octree.hpp:83-92, the left walk of
+    // 07_octree.cu:135-146; the right one there is unbounded and would not
+    // end at node 0, whose parent is itself)
     int rt = i;
     int steps = 0;
+    bool inside = true;
     while (counts[rt] == 0) {
       rt = parent[rt];
+      if (rt < 0 || rt >= n) {
+        inside = false;
+        break;
+      }
       if (++steps > 30) break;
+    }
+    if (!inside) {
+      flags |= RWH_STATUS_LEAF;
+      continue;
     }
     const int64_t bottom = a.edge_offset[sg.base + rt];
     if (bottom < 0 || bottom >= static_cast<int64_t>(os.rows)) continue;
     atomicMax(&children[bottom * 8 + which], leaf);
     atomicAnd(&a.child_leaf_mask[os.base + bottom], ~static_cast<int32_t>(1u << which));
   }
+  if (flags && a.status) atomicOr(&a.status[sg.id], flags);
 }
 
 void launch_link_leaf(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {

@@ -188,7 +188,8 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
                                                             uint32_t last_bits,
                                                             uint32_t* __restrict__ hist,
                                                             uint32_t tile_shift,
-                                                            uint32_t* __restrict__ tile_counts) {
+                                                            uint32_t* __restrict__ tile_counts,
+                                                            uint32_t* __restrict__ status) {
   uint32_t mask[kMaxPasses];
 #pragma unroll
   for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
@@ -200,8 +201,9 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
   for (uint32_t i = threadIdx.x; i < kMaxPasses * 256; i += kHistBlock) s_h[i] = 0;
   if (per_tile)
     for (uint32_t i = threadIdx.x; i < kTilesPerHistUnit * 256; i += kHistBlock) s_tc[i] = 0;
-  __syncthreads();
   const uint32_t u = blockIdx.x;
+  if (u == 0 && threadIdx.x == 0 && status) *status = 0;  // the build's first kernel
+  __syncthreads();
   const uint32_t first = u * unit_keys;
   const uint32_t end = min(first + unit_keys, n);
   constexpr uint32_t kDepth = 4;
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
 
 void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
                         uint32_t key_bits, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
-                        hipStream_t stream) {
+                        uint32_t* status, hipStream_t stream) {
   if (n == 0) return;
   const uint32_t n_passes = (key_bits + 7) / 8;
   const uint32_t last_bits = key_bits - 8 * (n_passes - 1);
@@ -248,7 +250,7 @@ void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float mi
   if (n_passes < 2) tile_counts = nullptr;
   hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, n,
                      make_quant(min_coord, range), kPlainHistUnit, n_passes, last_bits, hist,
-                     sort_tile_shift(sort_variant), tile_counts);
+                     sort_tile_shift(sort_variant), tile_counts, status);
 }
 
 // Pass 0's exclusive digit prefix of every tile of a single-cloud sort, in

@@ -16,10 +16,12 @@ constexpr uint32_t kMortonTile = kElemBlock * kMortonPerThread;  // 1024
 
 __global__ __launch_bounds__(kElemBlock) void k_morton(const float4* __restrict__ pts,
                                                        uint32_t* __restrict__ codes, SegArgs seg,
-                                                       Quant q) {
+                                                       Quant q, uint32_t* __restrict__ status) {
   const uint32_t first = blockIdx.x * kMortonTile;
   const Seg sg = seg_of(seg, first);
   const uint32_t lim = sg.base + sg.count;
+  // a batched build's first kernel: the segment's first block clears its status
+  if (status && threadIdx.x == 0 && first == sg.base) status[sg.id] = 0;
 #pragma unroll
   for (uint32_t j = 0; j < kMortonPerThread; ++j) {
     const uint32_t i = first + j * kElemBlock + threadIdx.x;
@@ -31,11 +33,11 @@ __global__ __launch_bounds__(kElemBlock) void k_morton(const float4* __restrict_
 }
 
 void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index,
-                   float min_coord, float range, hipStream_t stream) {
+                   float min_coord, float range, uint32_t* status, hipStream_t stream) {
   const uint32_t grid = (n_index + kMortonTile - 1) / kMortonTile;
   if (grid == 0) return;
   hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kElemBlock), 0, stream, pts, codes, seg,
-                     make_quant(min_coord, range));
+                     make_quant(min_coord, range), status);
 }
 
 // dst[i] = value for i < count (counts of empty builds, state cleared by
@@ -357,6 +359,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   __shared__ uint16_t s_pairs[kTreeBlock / kWave][kTreePairCap];  // per wave: left leaves of leaf pairs
   __shared__ uint32_t s_n_cross;
   __shared__ uint32_t s_count_sum;  // the tile's edge counts, summed (with a.tile_sum)
+  __shared__ uint32_t s_flags;      // RWH_STATUS_* bits of the bounds this tile hit
   const int tid = static_cast<int>(threadIdx.x);
   const uint32_t first = blockIdx.x * static_cast<uint32_t>(T);
   const Seg sg = seg_of(seg, first);
@@ -373,6 +376,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   // leaves [lo, hi): the segment's last tile also takes leaf n
   const int hi = (lo + T < n) ? lo + T : n + 1;
   const int k_end = min(hi, n);
+  if (tid == 0) s_flags = 0;
   {
     // every load in flight before the first LDS store
     constexpr int kLoads = (T + 2 + kTreeBlock - 1) / kTreeBlock;
@@ -388,11 +392,23 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
 #pragma unroll
   for (int j = 0; j < (T + 2 + kTreeBlock - 1) / kTreeBlock; ++j) {
     const int k = lo - 1 + tid + j * kTreeBlock;
-    if (k < hi)
-      s_e[k - lo + 1] = (k < 0 || k >= n) ? uint8_t{0}
-                                           : static_cast<uint8_t>(__clz(s_ks[k - lo + 1] ^ s_ks[k - lo + 2]) + 1);
+    if (k < hi) {
+      const bool pair = k >= 0 && k < n;
+      s_e[k - lo + 1] = pair ? static_cast<uint8_t>(__clz(s_ks[k - lo + 1] ^ s_ks[k - lo + 2]) + 1) : uint8_t{0};
+      // the growth below needs strictly increasing keys (no two adjacent
+      // deltas then tie); other input is reported and the tile skipped
+      if (pair && s_ks[k - lo + 1] >= s_ks[k - lo + 2]) s_flags = RWH_STATUS_KEY_ORDER;
+    }
   }
   __syncthreads();
+  if (s_flags & RWH_STATUS_KEY_ORDER) {
+    if (tid == 0) {
+      my_list[0] = 0;
+      if (a.tile_sum) a.tile_sum[blockIdx.x] = 0;
+      if (a.status) atomicOr(&a.status[sg.id], RWH_STATUS_KEY_ORDER);
+    }
+    return;
+  }
   uint32_t* s_slot = s_ks;  // split g in [lo, hi-2] at [g - lo]
   for (int x = tid; x < T; x += kTreeBlock) {
     s_slot[x] = 0;
@@ -457,7 +473,10 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
     }
     const uint64_t qm = ballot(queued);
     const int at = n_pairs + static_cast<int>(popc_below(qm));
-    if (queued && at < kTreePairCap) pairs[at] = static_cast<uint16_t>(left);
+    if (queued) {
+      if (at < kTreePairCap) pairs[at] = static_cast<uint16_t>(left);
+      else atomicOr(&s_flags, RWH_STATUS_PAIR_CAP);
+    }
     n_pairs += static_cast<int>(__popcll(qm));
   }
   __syncthreads();  // every leaf has arrived: from here on only internal nodes do
@@ -468,7 +487,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   int next = 0;
   uint32_t pa = 0, pb = 0;
   bool have = false;
-  for (int guard = 0; guard < kTreeClimbGuard; ++guard) {
+  int guard = 0;
+  for (; guard < kTreeClimbGuard; ++guard) {
     const uint64_t need = ballot(!have);
     if (!have) {
       const int q = next + static_cast<int>(popc_below(need));
@@ -514,6 +534,9 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
     if (!ll) s_par[g] = static_cast<uint16_t>(i);
     if (!lr) s_par[g + 1] = static_cast<uint16_t>(i);
   }
+  // the bound ended the loop with climbs or queued pairs left
+  if (guard == kTreeClimbGuard && (ballot(have) != 0 || next < n_pairs) && tid % kWave == 0)
+    atomicOr(&s_flags, RWH_STATUS_CLIMB);
   __syncthreads();
 
   // The tile's nodes, four per thread with 16- and 4-byte stores (the
@@ -529,6 +552,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   const auto list_node = [&](int x) {
     const uint32_t at = atomicAdd(&s_n_cross, 1u);
     if (at < static_cast<uint32_t>(kTreeListCap)) my_list[1 + at] = static_cast<uint32_t>(lo + x);
+    else atomicOr(&s_flags, RWH_STATUS_CROSS_CAP);
   };
   const auto parent_of = [&](uint32_t p) { return p == 0xffffu ? -1 : lo + static_cast<int>(p); };
   // S5's count prefix/3 - parent's prefix/3 when the parent was formed here
@@ -595,6 +619,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
     my_list[0] = min(s_n_cross, static_cast<uint32_t>(kTreeListCap));
     // k_tree_cross adds the counts it writes over this tile's placeholders
     if (a.tile_sum) a.tile_sum[blockIdx.x] = s_count_sum;
+    if (s_flags && a.status) atomicOr(&a.status[sg.id], s_flags);
   }
 }
 

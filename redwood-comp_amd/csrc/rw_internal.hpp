@@ -9,6 +9,7 @@
 #include <atomic>
 
 #include "rw_device.hpp"
+#include "rwhip.h"
 
 namespace rw {
 
@@ -137,12 +138,16 @@ void launch_sort(const SortLaunch& L, hipStream_t stream);
 // Stage 1 for one cloud of n points fused with the sort's histogram sweep
 // (the digit histograms of key_bits-bit keys go to hist's copies, and pass
 // 0's counts per tile of sort_variant to tile_counts unless it is nullptr).
+// status (device word or nullptr) is cleared by the launch: the build's
+// first kernel.
 void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
                         uint32_t key_bits, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
-                        hipStream_t stream);
+                        uint32_t* status, hipStream_t stream);
 
+// status (per segment, or nullptr) is cleared by the launch: the first kernel
+// of a batched build.
 void launch_morton(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index,
-                   float min_coord, float range, hipStream_t stream);
+                   float min_coord, float range, uint32_t* status, hipStream_t stream);
 
 // Compaction of adjacent-distinct keys; n_unique[s] per segment.
 void launch_unique(const uint32_t* sorted, uint32_t* unique_out, SegArgs seg, uint32_t n_index,
@@ -160,6 +165,8 @@ struct TreeArgs {
   // optional with edge_count: per tree tile (kTreeTile nodes of index space)
   // the sum of its nodes' edge counts, for a scan without look-back
   uint32_t* tile_sum = nullptr;
+  // optional: per segment, RWH_STATUS_* bits OR-ed in where a bound stops the work
+  uint32_t* status = nullptr;
 };
 // seg.count = n_unique per segment with count_sub = 1, i.e. the live count is n_brt
 void launch_radix_tree(const TreeArgs& a, SegArgs seg, uint32_t n_index, uint32_t* list, hipStream_t stream);
@@ -193,6 +200,7 @@ struct OctArgs {
   float* cell_size;
   int32_t* child_node_mask;
   int32_t* child_leaf_mask;
+  uint32_t* status;  // per segment, or nullptr: RWH_STATUS_* bits OR-ed in
 };
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
 // dst[0, count) = value with a kernel (no memset node, so a captured graph

@@ -42,7 +42,7 @@ struct BuildServer::Slot {
   rwh_event_t done_ev = nullptr;
   std::vector<void*> allocs;
   rwh_tree_buffers bufs{};
-  uint32_t* h_counts = nullptr;  // pinned: n_unique[max_clouds], n_oct[max_clouds]
+  uint32_t* h_counts = nullptr;  // pinned: n_unique[max_clouds], n_oct[max_clouds], status[max_clouds]
   uint32_t max_clouds = 0;
   // the batch in flight
   std::vector<std::unique_ptr<Request>> batch;
@@ -95,7 +95,7 @@ BuildServer::BuildServer(const ServerConfig& cfg) : cfg_(cfg), t0_(std::chrono::
     check(rwh_workspace_set_sort_variant(s->ws, variant), "rwh_workspace_set_sort_variant");
     check(rwh_event_create(&s->done_ev), "rwh_event_create");
     void* h = nullptr;
-    check(rwh_host_alloc(&h, 2 * sizeof(uint32_t) * cfg_.max_clouds), "rwh_host_alloc");
+    check(rwh_host_alloc(&h, 3 * sizeof(uint32_t) * cfg_.max_clouds), "rwh_host_alloc");
     s->h_counts = static_cast<uint32_t*>(h);
     auto& a = s->allocs;
     rwh_tree_buffers& b = s->bufs;
@@ -239,6 +239,9 @@ void BuildServer::launch(Slot* s, std::vector<std::unique_ptr<Request>> batch) {
   if (st == RWH_OK)
     st = rwh_memcpy_d2h(s->h_counts + s->max_clouds, d_counts + s->max_clouds,
                         sizeof(uint32_t) * k, s->stream);
+  if (st == RWH_OK)
+    st = rwh_memcpy_d2h(s->h_counts + 2 * s->max_clouds, d_counts + 2 * s->max_clouds,
+                        sizeof(uint32_t) * k, s->stream);
   if (st == RWH_OK) st = rwh_event_record(s->done_ev, s->stream);
   if (st != RWH_OK) {
     const std::string msg = std::string("BuildServer launch: ") + rwh_error_string(st) + " (" +
@@ -284,6 +287,9 @@ void BuildServer::completion_loop(Slot* s) {
         res.t_arrival = r.t_arrival;
         res.t_done = t_done;
         res.batch_size = k;
+        if (s->h_counts[2 * s->max_clouds + i] != 0)
+          throw Error(RWH_ERR_OVERFLOW, "BuildServer: a kernel bound was exceeded (status " +
+                                            std::to_string(s->h_counts[2 * s->max_clouds + i]) + ")");
         if (res.n_octree_nodes > s->oc[i])
           throw Error(RWH_ERR_CAPACITY, "BuildServer: octree above the cloud's 0.6 n capacity");
         if (r.keep) {

@@ -148,7 +148,7 @@ namespace {
 
 struct DeviceState {
   rwh_workspace* ws = nullptr;
-  uint32_t* h_counts = nullptr;  // pinned: n_unique, n_octree_nodes
+  uint32_t* h_counts = nullptr;  // pinned: n_unique, n_octree_nodes, status word
   ~DeviceState() {
     if (ws) (void)rwh_workspace_destroy(ws);
     if (h_counts) (void)rwh_host_free(h_counts);
@@ -169,7 +169,7 @@ DeviceState& state_of(AppData& a) {
     check(rwh_workspace_set_sort_variant(s->ws, redwood::tune::best_sort_variant(a.n_input)),
           "rwh_workspace_set_sort_variant");
     void* h = nullptr;
-    check(rwh_host_alloc(&h, 2 * sizeof(uint32_t)), "rwh_host_alloc");
+    check(rwh_host_alloc(&h, 3 * sizeof(uint32_t)), "rwh_host_alloc");
     s->h_counts = static_cast<uint32_t*>(h);
     a.backend_state = s;
   }
@@ -199,6 +199,18 @@ rwh_tree_buffers buffers_of(AppData& a) {
 
 uint32_t* d_n_unique(DeviceState& s) { return rwh_workspace_counts(s.ws); }
 uint32_t* d_n_oct(DeviceState& s) { return rwh_workspace_counts(s.ws) + 1; }  // max_segments = 1
+uint32_t* d_status(DeviceState& s) { return rwh_workspace_counts(s.ws) + 2; }
+
+// Enqueues the copy of the status word; the stage's completion raises on it.
+int fetch_status(Engine& e, DeviceState& s) {
+  return rwh_memcpy_d2h(s.h_counts + 2, d_status(s), sizeof(uint32_t), e.stream());
+}
+
+void raise_status(const DeviceState& s, const char* stage) {
+  if (s.h_counts[2] != 0)
+    throw Error(RWH_ERR_OVERFLOW, std::string(stage) + ": a kernel bound was exceeded (status " +
+                                      std::to_string(s.h_counts[2]) + ", RWH_STATUS_* bits)");
+}
 
 // Runs enqueue(); a failure becomes the future's exception.  then(state)
 // runs on the completion thread once the stage's work is done; the AppData
@@ -257,12 +269,17 @@ std::future<void> run_stage3(AppData& a) {
 }
 
 std::future<void> run_stage4(AppData& a) {
-  return stage(a, "run_stage4", [&](Engine& e, DeviceState& s) {
-    return rwh_radix_tree(s.ws, a.get_unique_morton_keys(), d_n_unique(s), a.n_input,
-                          a.brt.u_prefix_n.data(), a.brt.u_has_leaf_left.data(),
-                          a.brt.u_has_leaf_right.data(), a.brt.u_left_child.data(),
-                          a.brt.u_parents.data(), e.stream());
-  });
+  return stage(
+      a, "run_stage4",
+      [&](Engine& e, DeviceState& s) {
+        int st = rwh_radix_tree(s.ws, a.get_unique_morton_keys(), d_n_unique(s), a.n_input,
+                                a.brt.u_prefix_n.data(), a.brt.u_has_leaf_left.data(),
+                                a.brt.u_has_leaf_right.data(), a.brt.u_left_child.data(),
+                                a.brt.u_parents.data(), e.stream());
+        if (st == RWH_OK) st = fetch_status(e, s);
+        return st;
+      },
+      [](DeviceState& s) { raise_status(s, "run_stage4"); });
 }
 
 std::future<void> run_stage5(AppData& a) {
@@ -292,18 +309,28 @@ std::future<void> run_stage6(AppData& a) {
 }
 
 std::future<void> run_stage7(AppData& a) {
-  return stage(a, "run_stage7", [&](Engine& e, DeviceState& s) {
-    const rwh_tree_buffers b = buffers_of(a);
-    return rwh_octree(&b, d_n_unique(s), d_n_oct(s), a.n_input, AppData::min_coord, AppData::range,
-                      e.stream());
-  });
+  return stage(
+      a, "run_stage7",
+      [&](Engine& e, DeviceState& s) {
+        const rwh_tree_buffers b = buffers_of(a);
+        int st = rwh_octree(&b, d_n_unique(s), d_n_oct(s), a.n_input, AppData::min_coord, AppData::range,
+                            d_status(s), e.stream());
+        if (st == RWH_OK) st = fetch_status(e, s);
+        return st;
+      },
+      [](DeviceState& s) { raise_status(s, "run_stage7"); });
 }
 
 std::future<void> run_link_leaf(AppData& a) {
-  return stage(a, "run_link_leaf", [&](Engine& e, DeviceState& s) {
-    const rwh_tree_buffers b = buffers_of(a);
-    return rwh_link_leaf(&b, d_n_unique(s), d_n_oct(s), a.n_input, e.stream());
-  });
+  return stage(
+      a, "run_link_leaf",
+      [&](Engine& e, DeviceState& s) {
+        const rwh_tree_buffers b = buffers_of(a);
+        int st = rwh_link_leaf(&b, d_n_unique(s), d_n_oct(s), a.n_input, d_status(s), e.stream());
+        if (st == RWH_OK) st = fetch_status(e, s);
+        return st;
+      },
+      [](DeviceState& s) { raise_status(s, "run_link_leaf"); });
 }
 
 }  // namespace redwood::hip

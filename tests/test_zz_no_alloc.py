@@ -14,10 +14,11 @@ def noalloc_build(rw, pts):
     n = pts.shape[0]
     cap = rw.octree_capacity(n)
     tb = rw.TreeBuild(n, cap, rw.DeviceArray.from_numpy(pts.reshape(-1)))
-    d_counts = rw.DeviceArray(2, np.uint32, zero=True)
+    d_counts = rw.DeviceArray(3, np.uint32, zero=True)
     rw.check(rw.lib().rwh_build_octree_noalloc(tb.ptr(), n, 0.0, 1024.0, d_counts.ptr, None),
              "rwh_build_octree_noalloc")
-    nu, no = (int(x) for x in d_counts.numpy())  # waits for the null stream
+    nu, no, status = (int(x) for x in d_counts.numpy())  # waits for the null stream
+    assert status == 0
     return (nu, max(nu - 1, 0), no), tb.read(n, min(max(no, 1), cap))
 
 
@@ -40,7 +41,7 @@ def test_noalloc_equals_workspace_build(gpu):
 def test_noalloc_rejects_sizes_above_its_maximum(gpu):
     rw = gpu
     L = rw.lib()
-    d_counts = rw.DeviceArray(2, np.uint32, zero=True)
+    d_counts = rw.DeviceArray(3, np.uint32, zero=True)
     tb = rw.TreeBuild(16, 9)
     st = L.rwh_build_octree_noalloc(tb.ptr(), rw.NOALLOC_MAX_POINTS + 1, 0.0, 1024.0, d_counts.ptr, None)
     assert st == rw.RWH_ERR_CAPACITY
