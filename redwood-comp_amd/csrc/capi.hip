@@ -621,7 +621,7 @@ int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
   OctArgs a = oct_args(bufs, min_coord, range);
   a.n_oct = d_n_octree_nodes;
   a.status = d_status;
-  launch_octree(a, single_seg_dev(d_n_unique), n_max, static_cast<hipStream_t>(stream));
+  launch_octree(a, single_seg_dev(d_n_unique), n_max, true, static_cast<hipStream_t>(stream));
   return launched();
 }
 
@@ -715,7 +715,13 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
     launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->tree_sums, st);
     RWH_TRY(launched());
   }
-  return rwh_octree(bufs, d_nu, d_no, n, min_coord, range, d_status, st);
+  // S7 on the tree just built (its kernels checked their own bounds)
+  OctArgs oa = oct_args(bufs, min_coord, range);
+  oa.n_oct = d_no;
+  oa.status = d_status;
+  if (injected(7)) return RWH_ERR_INJECTED;
+  if (n > 0) launch_octree(oa, single_seg_dev(d_nu), n, false, st);
+  return launched();
 }
 
 }  // namespace
@@ -794,7 +800,7 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
   oa.oct_base = ws->oct_base();
   oa.oct_cap = ws->oct_cap();
   oa.status = ws->status();
-  launch_octree(oa, uni, n_index, st);
+  launch_octree(oa, uni, n_index, false, st);
   return launched();
 }
 

@@ -72,23 +72,25 @@ __device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c,
             a.prefix_n[base + c], row, mask);
 }
 
-// Steps the pull of one slot may take, all its walks together.  On a tree
-// built from sorted unique codes a walk visits at most two levels of count-0
-// nodes (a handful of steps); the budget only ends walks over malformed trees
-// handed in through the C ABI (a cycle of left_child links), which would
-// otherwise never end, and such a walk reports RWH_STATUS_PULL.
+// Steps a pull walk may take.  On a tree built from sorted unique codes a
+// walk visits at most two levels of count-0 nodes (a handful of steps); the
+// budget only ends walks over malformed trees handed in through the C ABI
+// (a cycle of left_child links), which would otherwise never end.
 constexpr int kPullBudget = 1024;
 
 // Pulls every chain reached from the given roots through nodes of count 0
-// (depth-first, one node per step, each step taken from budget); nodes
-// outside [0, n), a full stack or an exhausted budget set RWH_STATUS_PULL in
-// flags.
+// (depth-first, one node per step).  kCheck: nodes outside [0, n), a full
+// stack or the step budget set RWH_STATUS_PULL in flags (a tree handed in
+// through the C ABI); without it the walk trusts the tree (one built by the
+// radix-tree kernels, whose bounds it provably keeps).
+template <bool kCheck>
 __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int n, int* stack, int sp,
-                                           int32_t (&row)[8], int32_t& mask, uint32_t& flags, int& budget) {
+                                           int32_t (&row)[8], int32_t& mask, uint32_t& flags) {
   constexpr int kStack = 16;
-  for (; sp > 0 && budget > 0; --budget) {
+  int steps = 0;
+  for (; sp > 0 && steps < kPullBudget; ++steps) {
     const int q = stack[--sp];
-    if (q < 0 || q >= n) {
+    if (kCheck && (q < 0 || q >= n)) {
       flags |= RWH_STATUS_PULL;
       continue;
     }
@@ -97,13 +99,18 @@ __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int 
       continue;
     }
     const int lc = a.left_child[base + q];
-    for (int side = 0; side < 2; ++side) {
-      if (side == 0 ? a.has_leaf_left[base + q] : a.has_leaf_right[base + q]) continue;
-      if (sp < kStack) stack[sp++] = lc + side;
-      else flags |= RWH_STATUS_PULL;
+    if (kCheck) {
+      for (int side = 0; side < 2; ++side) {
+        if (side == 0 ? a.has_leaf_left[base + q] : a.has_leaf_right[base + q]) continue;
+        if (sp < kStack) stack[sp++] = lc + side;
+        else flags |= RWH_STATUS_PULL;
+      }
+    } else {
+      if (!a.has_leaf_left[base + q] && sp < kStack) stack[sp++] = lc;
+      if (!a.has_leaf_right[base + q] && sp < kStack) stack[sp++] = lc + 1;
     }
   }
-  if (sp > 0) flags |= RWH_STATUS_PULL;  // the budget ended the walk
+  if (sp > 0) flags |= RWH_STATUS_PULL;
 }
 
 __device__ __forceinline__ int delta8o(uint32_t x, uint32_t y) {
@@ -236,18 +243,20 @@ __device__ __forceinline__ int32_t merge_row(const int32_t (&r)[8], int32_t m, i
 
 // Depth-first pull from global memory below node q (count 0), for a walk
 // deeper than the pull stack (not reached below a node of count > 0).
+template <bool kCheck>
 __device__ __forceinline__ int32_t pull_overflow(const OctArgs& a, uint32_t base, int n, int q, int32_t* row,
-                                                 uint32_t& flags, int& budget) {
+                                                 uint32_t& flags) {
   int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t m = 0;
   int stack[16];
   stack[0] = q;
-  pull_links(a, base, n, stack, 1, r, m, flags, budget);
+  pull_links<kCheck>(a, base, n, stack, 1, r, m, flags);
   return merge_row(r, m, row);
 }
 
 // Node 0's links (slot 0 is also offset[0]): every chain reached from the
 // children of the complete-range root through nodes of count 0.
+template <bool kCheck>
 __device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, int n, int32_t* row,
                                              uint32_t& flags) {
   int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -257,8 +266,7 @@ __device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, in
   int sp = 0;
   if (g != 0) stack[sp++] = g;
   if (g + 1 < n) stack[sp++] = g + 1;
-  int budget = kPullBudget;
-  pull_links(a, base, n, stack, sp, r, m, flags, budget);
+  pull_links<kCheck>(a, base, n, stack, sp, r, m, flags);
   return merge_row(r, m, row);
 }
 
@@ -273,26 +281,19 @@ constexpr int kPullStack = 4;
 // (prefix lengths increase strictly and a count-0 node has its parent's
 // level), so at most three pairs wait on the stack; a deeper walk is still
 // handled, from global memory.
+template <bool kCheck>
 __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, const Window& w,
                                              int n, uint32_t pair, int32_t* row, uint32_t* stk,
                                              uint32_t& flags) {
   int32_t m = 0;
   int sp = 0;
-  int budget = kPullBudget;
-  for (;; --budget) {
-    if (budget <= 0) {  // a malformed tree: the walk is cut short
-      flags |= RWH_STATUS_PULL;
-      break;
-    }
+  int steps = 0;
+  for (; steps < kPullBudget; ++steps) {
     const int c = static_cast<int>(pair & 0x3fffffffu);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int q = c + side;
-      if ((pair >> (30 + side)) & 1u) continue;  // a leaf
-      if (q >= n) {  // a child index outside the tree
-        flags |= RWH_STATUS_PULL;
-        continue;
-      }
+      if (((pair >> (30 + side)) & 1u) || q >= n) continue;  // a leaf (or no such node)
       uint2 d;
       if (q >= w.w0 && q < w.w1) d = w.desc[q - w.w0];
       else d = desc_global(a, base + static_cast<uint32_t>(q));
@@ -302,12 +303,13 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
       } else if (sp < kPullStack) {
         stk[(sp++) * kOctBlock] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
       } else {
-        m |= pull_overflow(a, base, n, q, row, flags, budget);
+        m |= pull_overflow<kCheck>(a, base, n, q, row, flags);
       }
     }
     if (sp == 0) break;
     pair = stk[(--sp) * kOctBlock];
   }
+  if (steps == kPullBudget) flags |= RWH_STATUS_PULL;  // a malformed tree: the walk was cut short
   return m;
 }
 
@@ -320,6 +322,11 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
 // counts), found by a binary search in LDS.  Slot offset + j is the node j
 // levels above the chain's deepest one; j = 0 is the bottom, whose row is
 // pulled.  Node 0 writes the root cell when no node contributes (D8).
+// kCheck: the tree comes through the C ABI and may be malformed: child
+// indices are checked and walks that stop at their bounds are reported in
+// a.status (RWH_STATUS_PULL).  The whole builds run the kernel without the
+// checks on the tree their radix-tree kernels built.
+template <bool kCheck>
 __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
   __shared__ uint2 s_desc[kOctWin];
   __shared__ __align__(16) int32_t s_off[kOctSpan];
@@ -380,7 +387,14 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     uint32_t lcf[4], lv = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      if (q0 + e < n) s_desc[q0 + e - w.w0] = g.desc(e);
+      if (q0 + e < n) {
+        s_desc[q0 + e - w.w0] = g.desc(e);
+        const int lc = g.lc[e];
+        // a child index outside the tree: the pull skips it, reported here
+        if (kCheck && ((!g.ll(e) && static_cast<uint32_t>(lc) >= static_cast<uint32_t>(n)) ||
+                       (!g.lr(e) && static_cast<uint32_t>(lc + 1) >= static_cast<uint32_t>(n))))
+          flags |= RWH_STATUS_PULL;
+      }
       lcf[e] = (static_cast<uint32_t>(g.lc[e]) & 0x3fffffffu) | (g.ll(e) ? 1u << 30 : 0u) |
                (g.lr(e) ? 1u << 31 : 0u);
       lv |= static_cast<uint32_t>(g.prefix(e) / 3) << (8 * e);
@@ -432,8 +446,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     } else {  // the bottom: pull the chains that link here
 #pragma unroll
       for (int e = 0; e < 8; ++e) row[e * kOctBlock] = 0;
-      m = pull_pair(a, sg.base, w, n, s_lcf[k], row, stk, flags);
-      if (slot == 0) m |= pull_root(a, sg.base, n, row, flags);
+      m = pull_pair<kCheck>(a, sg.base, w, n, s_lcf[k], row, stk, flags);
+      if (slot == 0) m |= pull_root<kCheck>(a, sg.base, n, row, flags);
       r0 = make_int4(row[0], row[kOctBlock], row[2 * kOctBlock], row[3 * kOctBlock]);
       r1 = make_int4(row[4 * kOctBlock], row[5 * kOctBlock], row[6 * kOctBlock], row[7 * kOctBlock]);
     }
@@ -442,15 +456,17 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     mask[slot] = m;
     leaf_mask[slot] = 0;
   }
-  if (flags && a.status) atomicOr(&a.status[sg.id], flags);
+  if (kCheck && flags && a.status) atomicOr(&a.status[sg.id], flags);
 }
 
-void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream) {
+void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, bool check, hipStream_t stream) {
   const uint32_t n_tiles = (n_index + kOctSpan - 1) / kOctSpan;
   if (n_tiles == 0) return;
   seg.count_sub = 1;
-  hipLaunchKernelGGL(k_oct_nodes, dim3(xcd_grid(n_tiles)), dim3(kOctBlock), 0, stream, a, seg,
-                     n_tiles);
+  if (check)
+    hipLaunchKernelGGL(k_oct_nodes<true>, dim3(xcd_grid(n_tiles)), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
+  else
+    hipLaunchKernelGGL(k_oct_nodes<false>, dim3(xcd_grid(n_tiles)), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
 }
 
 // Leaf linking (opt-in): each leaf child of brt node i goes into the bottom

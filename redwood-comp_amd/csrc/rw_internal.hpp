@@ -202,7 +202,10 @@ struct OctArgs {
   int32_t* child_leaf_mask;
   uint32_t* status;  // per segment, or nullptr: RWH_STATUS_* bits OR-ed in
 };
-void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
+// check: a tree handed in through the C ABI (child indices checked, walk
+// bounds reported in a.status); the whole builds pass the tree their own
+// radix-tree kernels built, unchecked.
+void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, bool check, hipStream_t stream);
 // dst[0, count) = value with a kernel (no memset node, so a captured graph
 // replays it as it ran)
 void launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t stream);

@@ -6,7 +6,8 @@
 // or past an LDS array aborts the run.
 //
 //   octree_emul <in.bin> <out.bin>
-// in.bin:  u32 n_unique, n_octree_nodes, capacity, link_leaf; f32 min, range;
+// in.bin:  u32 n_unique, n_octree_nodes, capacity, mode (bit 0: run leaf
+//          linking, bit 1: the kernel without checks); f32 min, range;
 //          codes (u32 x n_unique); prefix_n, has_leaf_left, has_leaf_right
 //          (u8 x n_brt); left_child, parents, edge_count, edge_offset
 //          (i32 x n_brt)
@@ -40,7 +41,8 @@ int main(int argc, char** argv) {
   uint32_t hdr[4];
   float box[2];
   if (std::fread(hdr, 4, 4, f) != 4 || std::fread(box, 4, 2, f) != 2) return 2;
-  const uint32_t nu = hdr[0], n_oct = hdr[1], cap = hdr[2], leaf = hdr[3];
+  const uint32_t nu = hdr[0], n_oct = hdr[1], cap = hdr[2], leaf = hdr[3] & 1u;
+  const bool check = (hdr[3] & 2u) == 0;
   const uint32_t nb = nu - 1;
   auto* codes = read_array<uint32_t>(f, nu);
   auto* prefix_n = read_array<uint8_t>(f, nb);
@@ -82,7 +84,7 @@ int main(int argc, char** argv) {
   rw::SegArgs seg{};
   seg.n_segs = 1;
   seg.count = d_nu;
-  rw::launch_octree(a, seg, nu, nullptr);
+  rw::launch_octree(a, seg, nu, check, nullptr);
   if (leaf) rw::launch_link_leaf(a, seg, nu, nullptr);
 
   FILE* o = std::fopen(argv[2], "wb");

@@ -140,14 +140,14 @@ NODE_FIELDS = (("prefix_n", np.uint8), ("has_leaf_left", np.uint8), ("has_leaf_r
                ("edge_offset", np.int32))
 
 
-def run_octree(o, nu, no, cap, tmp_path, link_leaf=False, min_coord=0.0, range_=1024.0):
+def run_octree(o, nu, no, cap, tmp_path, link_leaf=False, check=True, min_coord=0.0, range_=1024.0):
     if not os.path.exists(OCT_EMU):
         subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "cpp"), "all"], check=True,
                        capture_output=True)
     nb = nu - 1
     ip, op = tmp_path / "tree.bin", tmp_path / "oct.bin"
     with open(ip, "wb") as f:
-        f.write(np.array([nu, no, cap, int(link_leaf)], np.uint32).tobytes())
+        f.write(np.array([nu, no, cap, int(link_leaf) | (0 if check else 2)], np.uint32).tobytes())
         f.write(np.array([min_coord, range_], np.float32).tobytes())
         f.write(np.ascontiguousarray(o["unique"][:nu], np.uint32).tobytes())
         for name, dt in NODE_FIELDS:
@@ -179,13 +179,15 @@ def octree_cases(rw):
         yield kind, p
 
 
-@pytest.mark.parametrize("link_leaf", [False, True])
-def test_octree_kernels_on_cpu_match_oracle(rw, orc, pool, tmp_path, link_leaf):
+@pytest.mark.parametrize("link_leaf,check", [(False, True), (False, False), (True, True)])
+def test_octree_kernels_on_cpu_match_oracle(rw, orc, pool, tmp_path, link_leaf, check):
+    """check: the kernel of the stage entry point (a tree from the caller);
+    without: the one the whole builds run on the tree they built."""
     for name, pts in octree_cases(rw):
         (nu, nb, no, rc), o, _ = orc.build(pool, pts)
         cap = rw.octree_capacity(pts.shape[0])
         assert rc == 0 and no <= cap, name
-        got = run_octree(o, nu, no, cap, tmp_path, link_leaf)
+        got = run_octree(o, nu, no, cap, tmp_path, link_leaf, check)
         assert got["status"] == 0, name
         if link_leaf:
             orc.link_leaf(pool, o, (nu, nb, no))
