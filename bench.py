@@ -186,8 +186,8 @@ def clustered_section(rw, orc, pool, steps, warmup):
         n = 1 << log2n
         progress(f"clustered build 2^{log2n}")
         pts = rw.gmm_points(n, 1, 64, 20.48)
-        variant = rw.best_sort_variant(n)
-        ws = rw.Workspace(n, sort_variant=variant)
+        variant, plan = rw.best_sort_config(n)
+        ws = rw.Workspace(n, sort_variant=variant, sort_plan=plan)
         cap = rw.octree_capacity(n)
         tb = rw.TreeBuild(n, cap, rw.DeviceArray.from_numpy(pts.reshape(-1)))
         stream = rw.Stream()
@@ -196,9 +196,14 @@ def clustered_section(rw, orc, pool, steps, warmup):
         ok, bad = verify_build(rw, orc, pool, tb, pts, counts)
         out[f"2^{log2n}"] = {"points": n, "ms": round(ms, 4), "mpts_per_s": round(n / (ms * 1e-3) / 1e6, 1),
                              "n_unique": counts[0], "n_octree_nodes": counts[2],
-                             "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
+                             "sort_config": config_name(rw, variant, plan),
                              "verified": ok, "mismatches": bad}
     return out
+
+
+def config_name(rw, variant, plan):
+    """A sort configuration as '<block>x<keys per thread>/<digit widths>'."""
+    return "{}x{}/".format(*rw.sort_variants()[variant]) + "-".join(str(w) for w in rw.sort_plans()[plan])
 
 
 def batched_section(rw, reps=5):
@@ -230,15 +235,19 @@ def batched_section(rw, reps=5):
     def enqueue():
         rw.check(L.rwh_build_octree_batch_enqueue(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr), "batch")
 
-    # the batch's sort tile shape, tuned on the batch itself (shapes whose
-    # tile fits a chunk; larger ones fall back to the chunk-sized tile)
+    # the batch's sort tile shape and digit plan, tuned on the batch itself
+    # (shapes whose tile fits a chunk; larger ones fall back to the
+    # chunk-sized tile)
     shape_ms = {}
     for v, (blk, kpt) in enumerate(rw.sort_variants()):
         if blk * kpt <= 8192:
-            b.ws.set_sort_variant(v)
-            shape_ms[v] = timed(enqueue, 3)
-    variant = min(shape_ms, key=shape_ms.get)
+            for p in range(len(rw.sort_plans())):
+                b.ws.set_sort_variant(v)
+                b.ws.set_sort_plan(p)
+                shape_ms[(v, p)] = timed(enqueue, 3)
+    variant, plan = min(shape_ms, key=shape_ms.get)
     b.ws.set_sort_variant(variant)
+    b.ws.set_sort_plan(plan)
     t_enqueue = timed(enqueue, reps)
     g = C.c_void_p()
     rw.check(L.rwh_batch_graph_create(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, stream.ptr, C.byref(g)), "graph")
@@ -271,8 +280,8 @@ def batched_section(rw, reps=5):
     rw.check(L.rwh_graph_destroy(g), "graph destroy")
     return {"workload": "512 clouds, sizes log-uniform 2^10..2^20 (seed 7), half uniform, "
                         "half 16-cluster Gaussian sigma 10.24, one segmented build",
-            "points": total, "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
-            "sort_variant_ms": {"{}x{}".format(*rw.sort_variants()[v]): round(t, 3) for v, t in shape_ms.items()},
+            "points": total, "sort_config": config_name(rw, variant, plan),
+            "sort_config_ms": {config_name(rw, v, p): round(t, 3) for (v, p), t in shape_ms.items()},
             "ms_batched": round(t_enqueue, 3), "ms_batched_graph": round(t_graph, 3),
             "ms_plain": round(t_plain, 3),
             "mpts_per_s_batched": round(total / (min(t_enqueue, t_graph) * 1e-3) / 1e6, 1),
@@ -396,10 +405,10 @@ def main():
     n = 1 << args.log2n
     pts = rw.uniform_points(n, 114514)
     cap = rw.octree_capacity(n)
-    progress("timing the sort tile shapes")
-    variant_ms = rw.measure_sort_variants(n, 5)
-    variant = int(np.argmin(variant_ms)) if variant_ms else 0
-    ws = rw.Workspace(n, sort_variant=variant)
+    progress("tuning the sort (tile shape x digit plan)")
+    variant, plan = rw.best_sort_config(n)
+    config_ms = rw.measure_sort_configs(n, 3)
+    ws = rw.Workspace(n, sort_variant=variant, sort_plan=plan)
     tb = rw.TreeBuild(n, cap, rw.DeviceArray.from_numpy(pts.reshape(-1)))
     stream = rw.Stream()
     L = rw.lib()
@@ -470,8 +479,8 @@ def main():
                           "traffic": profile_figure(args.log2n, "sort_hbm_bytes_per_call")},
         "copy_GBs": round(copy_gbs, 1),
         "stages": stages,
-        "sort_variants": {f"{b}x{k}": round(ms, 4) for (b, k), ms in zip(rw.sort_variants(), variant_ms)},
-        "sort_variant": "{}x{}".format(*rw.sort_variants()[variant]),
+        "sort_config": config_name(rw, variant, plan),
+        "sort_config_ms": {config_name(rw, v, p): round(ms, 4) for v, p, ms in config_ms},
         "device": rw.device_name(),
     }
     if mismatches:

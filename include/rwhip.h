@@ -102,6 +102,16 @@ int rwh_sort_variant_count(void);
 int rwh_sort_variant_shape(int variant, uint32_t* block, uint32_t* keys_per_thread);
 int rwh_workspace_set_sort_variant(rwh_workspace* ws, int variant);
 
+/* Digit plans of the sort of 30-bit keys (the Morton codes of a build, or
+ * rwh_sort_u32 with key_bits = 30): plan p runs four passes whose digit
+ * widths, least significant first, rwh_sort_plan_widths gives (they sum to
+ * 30; plan 0 is 8, 8, 8, 6).  A workspace uses plan 0 until told otherwise;
+ * the host layer's autotuner picks one per size.  Sorts of keys of any other
+ * width run 8-bit passes with the remaining bits in the last one. */
+int rwh_sort_plan_count(void);
+int rwh_sort_plan_widths(int plan, uint32_t* widths4);
+int rwh_workspace_set_sort_plan(rwh_workspace* ws, int plan);
+
 /* ---- S1  (reference: cpu::run_stage1 host_dispatchers.cpp:17-41,
  *          cuda::run_stage1 cu_dispatcher.cu:501-522)
  * codes[i] = Morton code of float4 point i (x,y,z used), 10 bits per axis.

@@ -76,6 +76,9 @@ ABI = {
     "rwh_sort_variant_count": (_I, []),
     "rwh_sort_variant_shape": (_I, [_I, C.POINTER(_U32), C.POINTER(_U32)]),
     "rwh_workspace_set_sort_variant": (_I, [_P, _I]),
+    "rwh_sort_plan_count": (_I, []),
+    "rwh_sort_plan_widths": (_I, [_I, _P]),
+    "rwh_workspace_set_sort_plan": (_I, [_P, _I]),
     "rwh_morton": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_sort_u32": (_I, [_P, _P, _P, _U32, _U32, _P]),
     "rwh_unique_u32": (_I, [_P, _P, _U32, _P, _P, _P]),
@@ -107,7 +110,10 @@ HOST_ABI = {
     "rwt_dropin_build": (_I, [_P, _U32, _I] + [_P] * 14 + [_P]),
     "rwt_plan_batch": (_U32, [C.c_uint64, _U32, _P, _U32]),
     "rwt_best_sort_variant": (_I, [_U32]),
-    "rwt_measure_sort_variants": (_I, [_U32, _I, _P, _I]),
+    "rwt_best_sort_config": (None, [_U32, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "rwt_measure_sort_configs": (_I, [_U32, _I, _I, _P, _P, _P, _I]),
+    "rwt_tune_lookup": (_I, [_U32, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "rwt_tune_store": (None, [_U32, _I, _I, _I]),
     "rwt_server_create": (_P, [C.c_uint64, _U32, _U32, _U32, _I]),
     "rwt_server_destroy": (None, [_P]),
     "rwt_server_slots": (_U32, [_P]),
@@ -282,16 +288,22 @@ class DeviceArray:
 
 
 class Workspace:
-    def __init__(self, max_points: int, max_segments: int = 1, sort_variant: int | None = None):
+    def __init__(self, max_points: int, max_segments: int = 1, sort_variant: int | None = None,
+                 sort_plan: int | None = None):
         p = _P()
         check(lib().rwh_workspace_create(max_points, max_segments, C.byref(p)), "rwh_workspace_create")
         self.ptr = p.value
         self.max_segments = max_segments
         if sort_variant is not None:
             self.set_sort_variant(sort_variant)
+        if sort_plan is not None:
+            self.set_sort_plan(sort_plan)
 
     def set_sort_variant(self, v: int):
         check(lib().rwh_workspace_set_sort_variant(self.ptr, v), "rwh_workspace_set_sort_variant")
+
+    def set_sort_plan(self, p: int):
+        check(lib().rwh_workspace_set_sort_plan(self.ptr, p), "rwh_workspace_set_sort_plan")
 
     @property
     def counts_ptr(self) -> int:
@@ -443,7 +455,7 @@ class PinnedArray:
 
 def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
                  min_coord: float = MIN_COORD, range_: float = RANGE, link_leaf: bool = False,
-                 strict: bool = True, sort_variant: int | None = None):
+                 strict: bool = True, sort_variant: int | None = None, sort_plan: int | None = None):
     """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs).
     link_leaf also runs the opt-in leaf linking after S7.  strict=False returns the
     (capacity-clamped) outputs when n_octree_nodes exceeds the 0.6 n capacity
@@ -454,6 +466,8 @@ def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
     ws = ws or Workspace(max(n, 1))
     if sort_variant is not None:
         ws.set_sort_variant(sort_variant)
+    if sort_plan is not None:
+        ws.set_sort_plan(sort_plan)
     tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
     check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
     if link_leaf:
@@ -638,11 +652,42 @@ def sort_variants():
     return out
 
 
+def sort_plans():
+    """Digit widths (least significant pass first) of every sort plan of 30-bit keys."""
+    out = []
+    for p in range(lib().rwh_sort_plan_count()):
+        w = np.zeros(4, np.uint32)
+        check(lib().rwh_sort_plan_widths(p, w.ctypes.data), "rwh_sort_plan_widths")
+        out.append(tuple(int(x) for x in w))
+    return out
+
+
 def best_sort_variant(n: int) -> int:
     return int(host().rwt_best_sort_variant(n))
 
 
-def measure_sort_variants(n: int, reps: int = 5) -> list:
-    buf = np.zeros(16, np.float64)
-    k = host().rwt_measure_sort_variants(n, reps, buf.ctypes.data, 16)
-    return buf[:k].tolist()
+def best_sort_config(n: int, chunk: bool = False) -> tuple[int, int]:
+    """The tuned (tile shape, digit plan) for n keys (chunk: a batch's sort)."""
+    v, p = _I(0), _I(0)
+    host().rwt_best_sort_config(n, int(chunk), C.byref(v), C.byref(p))
+    return v.value, p.value
+
+
+def measure_sort_configs(n: int, reps: int = 5, chunk: bool = False) -> list:
+    """[(variant, plan, ms)] for every configuration at size n."""
+    cap = 256
+    ms = np.zeros(cap, np.float64)
+    vs = np.zeros(cap, np.int32)
+    ps = np.zeros(cap, np.int32)
+    k = host().rwt_measure_sort_configs(n, reps, int(chunk), ms.ctypes.data, vs.ctypes.data, ps.ctypes.data, cap)
+    return [(int(vs[i]), int(ps[i]), float(ms[i])) for i in range(k)]
+
+
+def tune_lookup(n: int, chunk: bool = False):
+    """The cached (variant, plan) of n's size bucket, or None (no measuring)."""
+    v, p = _I(0), _I(0)
+    return (v.value, p.value) if host().rwt_tune_lookup(n, int(chunk), C.byref(v), C.byref(p)) else None
+
+
+def tune_store(n: int, variant: int, plan: int, chunk: bool = False):
+    host().rwt_tune_store(n, int(chunk), variant, plan)

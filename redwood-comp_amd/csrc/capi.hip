@@ -124,6 +124,7 @@ struct rwh_workspace {
   uint32_t batch_clouds = 0;  // shape of the last staged batch
   uint32_t batch_index = 0;
   int sort_variant = 0;       // index into kSortVariants
+  int sort_plan = 0;          // index into kPlanWidths (sorts of 30-bit keys)
 
   uint32_t* seg_base() { return tables; }
   uint32_t* seg_n() { return tables + max_segs; }
@@ -235,7 +236,7 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
     L.hist_unit_keys = kChunk;
   }
   L.n_hist_units = (n_index + L.hist_unit_keys - 1) / L.hist_unit_keys;
-  L.key_bits = key_bits;
+  L.plan = make_plan(key_bits, ws->sort_plan);
   L.hist = ws->hist;
   L.digit_start = ws->digit_start;
   L.lookback = ws->lb_sort;
@@ -513,6 +514,22 @@ int rwh_workspace_set_sort_variant(rwh_workspace* ws, int variant) {
   ws->sort_variant = variant;
   return RWH_OK;
 }
+int rwh_sort_plan_count(void) { return kNumSortPlans; }
+
+int rwh_sort_plan_widths(int plan, uint32_t* widths) {
+  RWH_CHECK(plan >= 0 && plan < kNumSortPlans, "no such sort plan");
+  RWH_CHECK(widths, "null argument");
+  for (uint32_t p = 0; p < kMaxPasses; ++p) widths[p] = kPlanWidths[plan][p];
+  return RWH_OK;
+}
+
+int rwh_workspace_set_sort_plan(rwh_workspace* ws, int plan) {
+  RWH_CHECK(ws, "null workspace");
+  RWH_CHECK(plan >= 0 && plan < kNumSortPlans, "no such sort plan");
+  ws->sort_plan = plan;
+  return RWH_OK;
+}
+
 uint32_t* rwh_workspace_counts(rwh_workspace* ws) { return ws ? ws->counts : nullptr; }
 uint32_t rwh_workspace_max_segments(const rwh_workspace* ws) { return ws ? ws->max_segs : 0; }
 
@@ -694,7 +711,8 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
   if (n > 0) {
     launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n,
-                       min_coord, range, 30, ws->hist, ws->sort_variant, ws->tile_counts, d_status, st);
+                       min_coord, range, make_plan(30, ws->sort_plan), ws->hist, ws->sort_variant,
+                       ws->tile_counts, d_status, st);
     RWH_TRY(launched());
     RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
                          st, true));

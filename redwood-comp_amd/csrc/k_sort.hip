@@ -95,16 +95,19 @@ __device__ __forceinline__ void finish_pass0(const uint32_t* s_tc, uint32_t* s_h
 
 __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __restrict__ keys,
                                                            SegArgs seg, uint32_t unit_keys,
-                                                           uint32_t n_units, uint32_t n_passes,
-                                                           uint32_t last_bits,
+                                                           uint32_t n_units, DigitPlan plan,
                                                            uint32_t* __restrict__ hist,
                                                            uint32_t tile_shift,
                                                            uint32_t* __restrict__ tile_counts) {
-  // the digit of pass p is masked exactly as the binning pass masks it, so
+  // the digit of pass p is taken exactly as the binning pass takes it, so
   // the counts match the ranks whatever the key values
-  uint32_t mask[kMaxPasses];
+  const uint32_t n_passes = plan.n_passes;
+  uint32_t mask[kMaxPasses], shift[kMaxPasses];
 #pragma unroll
-  for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
+  for (uint32_t p = 0; p < kMaxPasses; ++p) {
+    mask[p] = p < n_passes ? (1u << plan.bits[p]) - 1u : 0u;
+    shift[p] = p < n_passes ? plan.shift[p] : 0u;
+  }
   __shared__ uint32_t s_h[kMaxPasses * 256];
   __shared__ uint32_t s_tc[kTilesPerHistUnit * 256];  // pass 0 digit counts per sort tile
   // With tile counts requested, pass 0 is counted per tile only and its unit
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
         const uint32_t tc_row = ((4 * (i0 + r * kHistBlock)) >> tile_shift) * 256;
         // keys that are ordered, or share their low bits, show as a top or
         // a low digit common to the whole wave
-        const bool shared = wave_same(k[0] >> (8 * (n_passes - 1))) || wave_same(k[0] & mask[0]);
+        const bool shared = wave_same(k[0] >> shift[n_passes - 1]) || wave_same(k[0] & mask[0]);
         // one branch per group of keys; the plain counts stay a tight loop
         const auto count4 = [&](auto check) {
 #pragma unroll
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 #pragma unroll
             for (uint32_t p = 0; p < kMaxPasses; ++p)
               if (p < n_passes && (p > 0 || !per_tile))
-                hist_add(s_h, p * 256 + ((k[c] >> (8 * p)) & mask[p]), decltype(check)::value);
+                hist_add(s_h, p * 256 + ((k[c] >> shift[p]) & mask[p]), decltype(check)::value);
             if (per_tile) hist_add(s_tc, tc_row + (k[c] & mask[0]), decltype(check)::value);
           }
         };
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
     for (uint32_t i = body + threadIdx.x; i < end; i += kHistBlock) {
       const uint32_t k = keys[i];
       for (uint32_t p = per_tile ? 1 : 0; p < n_passes; ++p)
-        hist_add(s_h, p * 256 + ((k >> (8 * p)) & mask[p]), false);
+        hist_add(s_h, p * 256 + ((k >> shift[p]) & mask[p]), false);
       if (per_tile) hist_add(s_tc, ((i - first) >> tile_shift) * 256 + (k & mask[0]), false);
     }
   }
@@ -184,15 +187,18 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __restrict__ pts,
                                                             uint32_t* __restrict__ codes, uint32_t n,
                                                             Quant quant,
-                                                            uint32_t unit_keys, uint32_t n_passes,
-                                                            uint32_t last_bits,
+                                                            uint32_t unit_keys, DigitPlan plan,
                                                             uint32_t* __restrict__ hist,
                                                             uint32_t tile_shift,
                                                             uint32_t* __restrict__ tile_counts,
                                                             uint32_t* __restrict__ status) {
-  uint32_t mask[kMaxPasses];
+  const uint32_t n_passes = plan.n_passes;
+  uint32_t mask[kMaxPasses], shift[kMaxPasses];
 #pragma unroll
-  for (uint32_t p = 0; p < kMaxPasses; ++p) mask[p] = (p + 1 == n_passes) ? ((1u << last_bits) - 1u) : 255u;
+  for (uint32_t p = 0; p < kMaxPasses; ++p) {
+    mask[p] = p < n_passes ? (1u << plan.bits[p]) - 1u : 0u;
+    shift[p] = p < n_passes ? plan.shift[p] : 0u;
+  }
   __shared__ uint32_t s_h[kMaxPasses * 256];
   __shared__ uint32_t s_tc[kTilesPerHistUnit * 256];  // pass 0 digit counts per sort tile
   // With tile counts requested, pass 0 is counted per tile only and its unit
@@ -220,11 +226,11 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
       codes[i] = k;
       // points that are ordered, or share their low bits, show as a top or a
       // low digit common to the whole wave
-      const bool shared = wave_same(k >> (8 * (n_passes - 1))) || wave_same(k & mask[0]);
+      const bool shared = wave_same(k >> shift[n_passes - 1]) || wave_same(k & mask[0]);
 #pragma unroll
       for (uint32_t p = 0; p < kMaxPasses; ++p)
         if (p < n_passes && (p > 0 || !per_tile))
-          hist_add(s_h, p * 256 + ((k >> (8 * p)) & mask[p]), shared);
+          hist_add(s_h, p * 256 + ((k >> shift[p]) & mask[p]), shared);
       if (per_tile) hist_add(s_tc, ((i - first) >> tile_shift) * 256 + (k & mask[0]), shared);
     }
   }
@@ -240,16 +246,20 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
   }
 }
 
+// Pass 0 runs from the sweep's tile counts (no look-back) for a single cloud
+// when its digit has a table kernel (6 to 8 bits) and another pass follows.
+constexpr bool table_pass0(const DigitPlan& plan) {
+  return plan.n_passes >= 2 && plan.bits[0] >= 6 && plan.bits[0] <= 8;
+}
+
 void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
-                        uint32_t key_bits, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
+                        const DigitPlan& plan, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
                         uint32_t* status, hipStream_t stream) {
   if (n == 0) return;
-  const uint32_t n_passes = (key_bits + 7) / 8;
-  const uint32_t last_bits = key_bits - 8 * (n_passes - 1);
   const uint32_t units = (n + kPlainHistUnit - 1) / kPlainHistUnit;
-  if (n_passes < 2) tile_counts = nullptr;
+  if (!table_pass0(plan)) tile_counts = nullptr;
   hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, n,
-                     make_quant(min_coord, range), kPlainHistUnit, n_passes, last_bits, hist,
+                     make_quant(min_coord, range), kPlainHistUnit, plan, hist,
                      sort_tile_shift(sort_variant), tile_counts, status);
 }
 
@@ -372,6 +382,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   static_assert(kChunk % TILE == 0 || TILE % kChunk == 0, "tile and chunk must nest");
   static_assert(!kTable || kFirst, "the tile table serves pass 0");
   constexpr uint32_t dmask = (1u << BITS) - 1u;
+  // digits of this pass: the per-digit work (counts, starts, look-back,
+  // scans) runs on threads tid < RADIX only (64 for a 6-bit pass)
+  constexpr uint32_t RADIX = 1u << BITS;
 
   __shared__ uint32_t s_stage[TILE];
   __shared__ uint32_t s_whist[WAVES][256];
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   const uint32_t n_grabs = n_tiles + gridDim.x;
   if (!kTable && tid == 0) s_grab = grab_tiles(ctrl, n_grabs);
   for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) (&s_whist[0][0])[i] = 0;
-  if (tid < 256) s_count[tid] = 0;
+  if (tid < RADIX) s_count[tid] = 0;
   __syncthreads();
   const uint32_t epoch = kTable ? 0u : s_grab.epoch;
   uint32_t tile = kTable ? blockIdx.x : s_grab.index;
@@ -415,9 +428,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   const Seg g_table = kTable ? seg_of(seg, 0) : Seg{};
   const auto load_keys = [&](uint32_t t) {
     const uint32_t f = t * TILE;
-    if (kTable && tid < 256) texcl = tile_excl[static_cast<size_t>(tid) * n_tiles + t];
+    if (kTable && tid < RADIX) texcl = tile_excl[static_cast<size_t>(tid) * n_tiles + t];
     g = kTable ? g_table : seg_of(seg, f);
-    if (tid < 256) tdig = digit_start[(static_cast<size_t>(g.id) * n_passes + pass) * 256 + tid];
+    if (tid < RADIX) tdig = digit_start[(static_cast<size_t>(g.id) * n_passes + pass) * 256 + tid];
     const uint32_t l = g.base + g.count;
     if (f >= l) return;  // a segment's padding
     const uint32_t wb = f + wave * (KPT * kWave) + lane;
@@ -439,7 +452,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   // scatter, in straight-line code: the wait for the keys then leaves the
   // scatter's stores in flight.
   const auto count_keys = [&](uint32_t t) {
-    if (tid < 256) s_pre[tid] = tdig;
+    if (tid < RADIX) s_pre[tid] = tdig;
     const uint32_t f = t * TILE;
     const uint32_t l = g.base + g.count;
     if (f >= l) {  // a segment's padding: no keys were loaded
@@ -469,7 +482,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
   // the iteration, like the counts above.  One branch per tile: the plain
   // ranks stay a tight loop.
   const auto rank_first = [&](uint32_t t) {
-    if (kTable && tid < 256) s_pre[tid] = tdig + texcl;
+    if (kTable && tid < RADIX) s_pre[tid] = tdig + texcl;
     const uint32_t f = t * TILE;
     const uint32_t l = g.base + g.count;
     if (f >= l) return;  // a segment's padding
@@ -516,7 +529,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       // and its own look-back done, before the ranking: successors waiting
       // on this tile are released early.
       uint32_t excl = 0;  // this digit's exclusive prefix over the segment's earlier tiles (look-back)
-      if (!kTable && tid < 256) {
+      if (!kTable && tid < RADIX) {
         const uint32_t d = tid;
         const uint32_t cnt_valid = s_count[d];
         s_count[d] = 0;  // for the next tile's counts (after two barriers)
@@ -575,13 +588,13 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       // digit in the staged tile (stable ranks count padding keys, staged
       // last and never written out; pass 0 does not count them).
       uint32_t cnt = 0;
-      if (tid < 256) {
+      if (tid < RADIX) {
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) cnt += s_whist[w][tid];
       }
       uint32_t total;
       const uint32_t tstart = scan256(cnt, s_tmp, &total);
-      if (tid < 256) {
+      if (tid < RADIX) {
         uint32_t run = tstart;
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
@@ -600,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
         if (!kFirst || n_valid == TILE || wbase + k * kWave < lim) s_stage[wh[d] + rank[k]] = keys[k];
       }
       // from here on only this wave uses its counters: cleared for the next tile
-      for (uint32_t i = lane; i < 256; i += kWave) wh[i] = 0;
+      for (uint32_t i = lane; i < RADIX; i += kWave) wh[i] = 0;
     }
     // The next tile: taken now, its keys loading during the scatter.
     if (!kTable && tid == 0) s_grab = grab_tiles(ctrl, n_grabs);
@@ -650,62 +663,84 @@ void with_variant(int v, F&& f) {
   }
 }
 
+// Calls f with the digit width b (1..8) as a compile-time constant.
+template <class F>
+void with_bits(uint32_t b, F&& f) {
+  switch (b) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+  }
+}
+// The same for pass 0's own kernels, compiled for 6..8-bit digits.
+template <class F>
+void with_first_bits(uint32_t b, F&& f) {
+  switch (b) {
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+  }
+}
+
 // Pass 0 of a single-cloud sort from the tile prefixes (no look-back): the
 // scan of the histogram sweep's tile counts, then the pass.
 template <int BLOCK, int KPT>
-void launch_table_pass(const SortLaunch& L, uint32_t n_passes, uint32_t* dst, hipStream_t stream) {
+void launch_table_pass(const SortLaunch& L, uint32_t* dst, hipStream_t stream) {
   constexpr uint32_t TILE = BLOCK * KPT;
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
-  constexpr auto kern = k_onesweep<BLOCK, KPT, 8, true>;
-  hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, L.keys_in,
-                     dst, L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, L.tile_counts, n_tiles);
+  with_first_bits(L.plan.bits[0], [&](auto B) {
+    constexpr auto kern = k_onesweep<BLOCK, KPT, B, true>;
+    hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, L.keys_in,
+                       dst, L.seg, L.digit_start, 0u, L.plan.n_passes, 0u, L.lookback, L.ctrl, L.tile_counts,
+                       n_tiles);
+  });
 }
 
 // Digit passes p_begin .. n_passes-1 on tiles of BLOCK x KPT.  The buffers
 // alternate in -> tmp -> out -> tmp -> out: an even number of passes ends in
 // out, an odd count starts with in -> out.
 template <int BLOCK, int KPT>
-void launch_passes(const SortLaunch& L, uint32_t n_passes, uint32_t p_begin, hipStream_t stream) {
+void launch_passes(const SortLaunch& L, uint32_t p_begin, hipStream_t stream) {
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(TILE >= kSortMinTile, "look-back table sized for kSortMinTile");
+  const uint32_t n_passes = L.plan.n_passes;
   const uint32_t n_tiles = (L.n_index + TILE - 1) / TILE;
   const auto dst_of = [&](uint32_t p) { return ((n_passes - p) % 2) == 1 ? L.keys_out : L.keys_tmp; };
   const uint32_t* src = p_begin == 0 ? L.keys_in : dst_of(p_begin - 1);
   for (uint32_t p = p_begin; p < n_passes; ++p) {
     uint32_t* dst = dst_of(p);
-    const uint32_t bits = (p + 1 == n_passes) ? (L.key_bits - 8 * p) : 8u;
-    if (p == 0 && bits == 8) {
-      constexpr auto kern = k_onesweep<BLOCK, KPT, 8, false, true>;
-      hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, src, dst,
-                         L.seg, L.digit_start, 0u, n_passes, 0u, L.lookback, L.ctrl, nullptr, n_tiles);
-      src = dst;
-      continue;
+    const uint32_t bits = L.plan.bits[p], shift = L.plan.shift[p];
+    if (p == 0 && bits >= 6) {  // no order to keep yet: atomic ranks
+      with_first_bits(bits, [&](auto B) {
+        constexpr auto kern = k_onesweep<BLOCK, KPT, B, false, true>;
+        hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, src, dst,
+                           L.seg, L.digit_start, 0u, n_passes, shift, L.lookback, L.ctrl, nullptr, n_tiles);
+      });
+    } else {
+      with_bits(bits, [&](auto B) {
+        constexpr auto kern = k_onesweep<BLOCK, KPT, B>;
+        hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, src, dst,
+                           L.seg, L.digit_start, p, n_passes, shift, L.lookback, L.ctrl, nullptr, n_tiles);
+      });
     }
-#define RW_ONESWEEP(B)                                                                          \
-  case B: {                                                                                     \
-    constexpr auto kern = k_onesweep<BLOCK, KPT, B>;                                            \
-    hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, \
-                       src, dst, L.seg, L.digit_start, p, n_passes, 8 * p, L.lookback, L.ctrl,  \
-                       nullptr, n_tiles);                                                       \
-  } break;
-    switch (bits) {
-      RW_ONESWEEP(1) RW_ONESWEEP(2) RW_ONESWEEP(3) RW_ONESWEEP(4)
-      RW_ONESWEEP(5) RW_ONESWEEP(6) RW_ONESWEEP(7) default: RW_ONESWEEP(8)
-    }
-#undef RW_ONESWEEP
     src = dst;
   }
 }
 
 void launch_sort(const SortLaunch& L, hipStream_t stream) {
-  const uint32_t n_passes = (L.key_bits + 7) / 8;
-  const uint32_t last_bits = L.key_bits - 8 * (n_passes - 1);
+  const uint32_t n_passes = L.plan.n_passes;
   const int variant = (L.n_segs > 1 && !sort_tile_fits_chunk(L.variant)) ? kSortChunkVariant : L.variant;
-  // pass 0 from tile prefixes: one cloud, whole histogram units, 8-bit pass 0
-  const bool table = L.tile_counts && L.n_segs == 1 && L.hist_unit_keys == kPlainHistUnit && n_passes >= 2;
+  // pass 0 from tile prefixes: one cloud, whole histogram units, a pass 0
+  // with a table kernel
+  const bool table = L.tile_counts && L.n_segs == 1 && L.hist_unit_keys == kPlainHistUnit && table_pass0(L.plan);
   if (!L.hist_ready)
     hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
-                       L.seg, L.hist_unit_keys, L.n_hist_units, n_passes, last_bits, L.hist,
+                       L.seg, L.hist_unit_keys, L.n_hist_units, L.plan, L.hist,
                        sort_tile_shift(variant), table ? L.tile_counts : nullptr);
   if (table) {
     const uint32_t n_tiles = (L.n_index + (1u << sort_tile_shift(variant)) - 1) >> sort_tile_shift(variant);
@@ -715,14 +750,10 @@ void launch_sort(const SortLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
                        L.digit_start, n_passes);
   }
-  if (table) {
-    with_variant(variant, [&](auto V) {
-      launch_table_pass<kSortVariants[V].block, kSortVariants[V].kpt>(
-          L, n_passes, ((n_passes % 2) == 1) ? L.keys_out : L.keys_tmp, stream);
-    });
-  }
   with_variant(variant, [&](auto V) {
-    launch_passes<kSortVariants[V].block, kSortVariants[V].kpt>(L, n_passes, table ? 1u : 0u, stream);
+    constexpr int B = kSortVariants[V].block, K = kSortVariants[V].kpt;
+    if (table) launch_table_pass<B, K>(L, ((n_passes % 2) == 1) ? L.keys_out : L.keys_tmp, stream);
+    launch_passes<B, K>(L, table ? 1u : 0u, stream);
   });
 }
 

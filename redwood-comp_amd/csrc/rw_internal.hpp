@@ -50,6 +50,53 @@ constexpr bool sort_tiles_nest() {
 static_assert(sort_tiles_nest(), "tiles are powers of two dividing the histogram unit");
 constexpr int kLookbackWindow = 4;           // look-back words loaded per round trip
 
+// The digit passes of a sort: pass p (least significant first) orders the
+// keys by bits [shift[p], shift[p] + bits[p]), with 1 <= bits[p] <= 8.
+struct DigitPlan {
+  uint32_t n_passes;
+  uint32_t shift[kMaxPasses];
+  uint32_t bits[kMaxPasses];
+};
+// Digit widths of the plans for 30-bit keys (Morton codes), least
+// significant pass first; a workspace runs one of them (the host layer's
+// autotuner picks per size bucket).  Sorts of keys of any other width run
+// 8-bit passes with the remaining bits in the last one.
+constexpr uint8_t kPlanWidths[][kMaxPasses] = {{8, 8, 8, 6}, {8, 7, 7, 8}, {8, 8, 7, 7},
+                                                 {8, 6, 8, 8}, {7, 7, 8, 8}, {6, 8, 8, 8}};
+constexpr int kNumSortPlans = sizeof(kPlanWidths) / sizeof(kPlanWidths[0]);
+constexpr bool plans_cover_30_bits() {
+  for (int p = 0; p < kNumSortPlans; ++p) {
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < kMaxPasses; ++k) {
+      if (kPlanWidths[p][k] < 1 || kPlanWidths[p][k] > 8) return false;
+      sum += kPlanWidths[p][k];
+    }
+    if (sum != 30 || kPlanWidths[p][0] < 6) return false;  // pass 0 has a 6..8-bit kernel
+  }
+  return true;
+}
+static_assert(plans_cover_30_bits(), "each plan sorts 30 bits in four passes of 1..8 bits");
+
+constexpr DigitPlan make_plan(uint32_t key_bits, int plan) {
+  DigitPlan d{};
+  if (key_bits == 30 && plan >= 0 && plan < kNumSortPlans) {
+    d.n_passes = kMaxPasses;
+    uint32_t sh = 0;
+    for (uint32_t p = 0; p < kMaxPasses; ++p) {
+      d.shift[p] = sh;
+      d.bits[p] = kPlanWidths[plan][p];
+      sh += d.bits[p];
+    }
+    return d;
+  }
+  d.n_passes = (key_bits + 7) / 8;
+  for (uint32_t p = 0; p < d.n_passes; ++p) {
+    d.shift[p] = 8 * p;
+    d.bits[p] = (p + 1 == d.n_passes) ? key_bits - 8 * p : 8u;
+  }
+  return d;
+}
+
 // Compaction / scan tiles.
 constexpr int kScanBlock = 512;
 constexpr int kScanKpt = 16;
@@ -124,7 +171,7 @@ struct SortLaunch {
   uint32_t n_index;        // index-space size (tiles are derived per variant)
   uint32_t hist_unit_keys;  // keys per histogram workgroup
   uint32_t n_hist_units;
-  uint32_t key_bits;       // significant key bits (30 for Morton codes, 32 generic)
+  DigitPlan plan;          // the digit passes (make_plan of the significant key bits)
   uint32_t* hist;          // [segs][kHistCopies][kMaxPasses*256], zero between sorts
   uint32_t* digit_start;   // [segs][passes][256]
   uint64_t* lookback;      // [n_tiles][256]
@@ -136,12 +183,12 @@ struct SortLaunch {
 };
 void launch_sort(const SortLaunch& L, hipStream_t stream);
 // Stage 1 for one cloud of n points fused with the sort's histogram sweep
-// (the digit histograms of key_bits-bit keys go to hist's copies, and pass
+// (the digit histograms of the plan's passes go to hist's copies, and pass
 // 0's counts per tile of sort_variant to tile_counts unless it is nullptr).
 // status (device word or nullptr) is cleared by the launch: the build's
 // first kernel.
 void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
-                        uint32_t key_bits, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
+                        const DigitPlan& plan, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
                         uint32_t* status, hipStream_t stream);
 
 // status (per segment, or nullptr) is cleared by the launch: the first kernel

@@ -16,15 +16,25 @@ namespace redwood::tune {
 
 namespace {
 std::mutex g_mu;
-std::map<uint32_t, int> g_best;  // bucket -> variant
+// key: "<bucket>" for single clouds, "chunk/<bucket>" for batches
+std::map<std::string, SortConfig> g_best;
 bool g_loaded = false;
 
+std::string key_of(uint32_t n, bool chunk) {
+  return (chunk ? "chunk/" : "") + std::to_string(bucket_of(n));
+}
+
+bool valid(SortConfig c) {
+  return c.variant >= 0 && c.variant < rwh_sort_variant_count() && c.plan >= 0 && c.plan < rwh_sort_plan_count();
+}
+
+// format: {"sort": {"<key>": [<variant>, <plan>], ...}}; an entry written as a
+// bare <variant> (older files) reads as plan 0
 void load_cache() {
   if (g_loaded) return;
   g_loaded = true;
   std::ifstream f(cache_path());
   if (!f) return;
-  // format: {"sort": {"<bucket>": <variant>, ...}}
   std::stringstream ss;
   ss << f.rdbuf();
   const std::string t = ss.str();
@@ -37,10 +47,22 @@ void load_cache() {
     if (q0 == std::string::npos || q0 > end) break;
     const size_t q1 = t.find('"', q0 + 1);
     const size_t colon = t.find(':', q1);
-    const uint32_t bucket = static_cast<uint32_t>(std::strtoul(t.c_str() + q0 + 1, nullptr, 10));
-    const int v = static_cast<int>(std::strtol(t.c_str() + colon + 1, nullptr, 10));
-    if (v >= 0 && v < rwh_sort_variant_count()) g_best[bucket] = v;
-    p = t.find(',', colon);
+    if (q1 == std::string::npos || colon == std::string::npos || colon > end) break;
+    const std::string key = t.substr(q0 + 1, q1 - q0 - 1);
+    SortConfig c;
+    size_t v = t.find_first_not_of(" \t\n", colon + 1);
+    if (v != std::string::npos && t[v] == '[') {
+      char* e = nullptr;
+      c.variant = static_cast<int>(std::strtol(t.c_str() + v + 1, &e, 10));
+      const size_t comma = t.find(',', v);
+      c.plan = comma != std::string::npos && comma < end ? static_cast<int>(std::strtol(t.c_str() + comma + 1, nullptr, 10)) : 0;
+      p = t.find(']', v);
+      if (p != std::string::npos) p = t.find(',', p);
+    } else {
+      c.variant = static_cast<int>(std::strtol(t.c_str() + colon + 1, nullptr, 10));
+      p = t.find(',', colon);
+    }
+    if (valid(c)) g_best[key] = c;
   }
 }
 
@@ -52,8 +74,8 @@ void save_cache() {
   if (!f) return;
   f << "{\"sort\": {";
   bool first = true;
-  for (const auto& [b, v] : g_best) {
-    f << (first ? "" : ", ") << '"' << b << "\": " << v;
+  for (const auto& [k, c] : g_best) {
+    f << (first ? "" : ", ") << '"' << k << "\": [" << c.variant << ", " << c.plan << "]";
     first = false;
   }
   f << "}}\n";
@@ -74,7 +96,7 @@ std::string cache_path() {
   return std::string(home ? home : "/tmp") + "/.cache/redwood_hip/tune.json";
 }
 
-std::vector<SortTiming> measure_sort_variants(uint32_t n, int reps) {
+std::vector<SortTiming> measure_sort_configs(uint32_t n, int reps, bool chunk) {
   using redwood::hip::check;
   std::vector<SortTiming> out;
   std::vector<uint32_t> h(n);
@@ -92,19 +114,25 @@ std::vector<SortTiming> measure_sort_variants(uint32_t n, int reps) {
     check(rwh_event_create(&e0), "rwh_event_create");
     check(rwh_event_create(&e1), "rwh_event_create");
     for (int v = 0; v < rwh_sort_variant_count(); ++v) {
+      uint32_t block = 0, kpt = 0;
+      check(rwh_sort_variant_shape(v, &block, &kpt), "rwh_sort_variant_shape");
+      if (chunk && block * kpt > 8192) continue;
       check(rwh_workspace_set_sort_variant(ws, v), "set variant");
-      const auto sort = [&] {
-        check(rwh_sort_u32(ws, static_cast<uint32_t*>(d_in), static_cast<uint32_t*>(d_out), n, 30, st),
-              "rwh_sort_u32");
-      };
-      sort();
-      check(rwh_event_record(e0, st), "record");
-      for (int r = 0; r < reps; ++r) sort();
-      check(rwh_event_record(e1, st), "record");
-      check(rwh_event_sync(e1), "sync");
-      float ms = 0;
-      check(rwh_event_elapsed_ms(&ms, e0, e1), "elapsed");
-      out.push_back({v, static_cast<double>(ms) / reps});
+      for (int p = 0; p < rwh_sort_plan_count(); ++p) {
+        check(rwh_workspace_set_sort_plan(ws, p), "set plan");
+        const auto sort = [&] {
+          check(rwh_sort_u32(ws, static_cast<uint32_t*>(d_in), static_cast<uint32_t*>(d_out), n, 30, st),
+                "rwh_sort_u32");
+        };
+        sort();
+        check(rwh_event_record(e0, st), "record");
+        for (int r = 0; r < reps; ++r) sort();
+        check(rwh_event_record(e1, st), "record");
+        check(rwh_event_sync(e1), "sync");
+        float ms = 0;
+        check(rwh_event_elapsed_ms(&ms, e0, e1), "elapsed");
+        out.push_back({{v, p}, static_cast<double>(ms) / reps});
+      }
     }
   } catch (...) {
     out.clear();
@@ -118,24 +146,37 @@ std::vector<SortTiming> measure_sort_variants(uint32_t n, int reps) {
   return out;
 }
 
-int best_sort_variant(uint32_t n) {
-  const uint32_t b = bucket_of(n);
+bool lookup(uint32_t n, bool chunk, SortConfig* out) {
   std::lock_guard<std::mutex> g(g_mu);
   load_cache();
-  if (auto it = g_best.find(b); it != g_best.end()) return it->second;
-  const uint32_t size = b >= 31 ? (1u << 30) : (1u << b);
-  const auto t = measure_sort_variants(size, 3);
-  if (t.empty()) return 0;
-  int best = t[0].variant;
-  double best_ms = t[0].ms;
-  for (const auto& x : t)
-    if (x.ms < best_ms) {
-      best = x.variant;
-      best_ms = x.ms;
-    }
-  g_best[b] = best;
-  save_cache();
-  return best;
+  const auto it = g_best.find(key_of(n, chunk));
+  if (it == g_best.end()) return false;
+  *out = it->second;
+  return true;
 }
+
+void store(uint32_t n, bool chunk, SortConfig c) {
+  std::lock_guard<std::mutex> g(g_mu);
+  load_cache();
+  if (!valid(c)) return;
+  g_best[key_of(n, chunk)] = c;
+  save_cache();
+}
+
+SortConfig best_sort_config(uint32_t n, bool chunk) {
+  SortConfig c;
+  if (lookup(n, chunk, &c)) return c;
+  const uint32_t b = bucket_of(n);
+  const uint32_t size = b >= 30 ? (1u << 30) : (1u << b);
+  const auto t = measure_sort_configs(size, 3, chunk);
+  if (t.empty()) return SortConfig{chunk ? 3 : 0, 0};
+  SortTiming best = t[0];
+  for (const auto& x : t)
+    if (x.ms < best.ms) best = x;
+  store(n, chunk, best.config);
+  return best.config;
+}
+
+int best_sort_variant(uint32_t n) { return best_sort_config(n).variant; }
 
 }  // namespace redwood::tune

@@ -85,14 +85,15 @@ BuildServer::BuildServer(const ServerConfig& cfg) : cfg_(cfg), t0_(std::chrono::
   if (ndev == 0) throw Error(RWH_ERR_NO_DEVICE, "BuildServer: no GPU device");
   const uint64_t P = cfg_.slot_points, O = planner_.slot_oct;
   // one tile shape for every slot, tuned at the size of a typical batch
-  const int variant = redwood::tune::best_sort_variant(std::min<uint32_t>(cfg_.slot_points, 1u << 22));
+  const auto tuned = redwood::tune::best_sort_config(std::min<uint32_t>(cfg_.slot_points, 1u << 22), !cfg_.plain);
   for (uint32_t i = 0; i < cfg_.n_slots; ++i) {
     auto s = std::make_unique<Slot>();
     s->id = i;
     s->max_clouds = cfg_.max_clouds;
     check(rwh_stream_create(&s->stream), "rwh_stream_create");
     check(rwh_workspace_create(static_cast<uint32_t>(P), cfg_.max_clouds, &s->ws), "rwh_workspace_create");
-    check(rwh_workspace_set_sort_variant(s->ws, variant), "rwh_workspace_set_sort_variant");
+    check(rwh_workspace_set_sort_variant(s->ws, tuned.variant), "rwh_workspace_set_sort_variant");
+    check(rwh_workspace_set_sort_plan(s->ws, tuned.plan), "rwh_workspace_set_sort_plan");
     check(rwh_event_create(&s->done_ev), "rwh_event_create");
     void* h = nullptr;
     check(rwh_host_alloc(&h, 3 * sizeof(uint32_t) * cfg_.max_clouds), "rwh_host_alloc");

@@ -166,8 +166,9 @@ DeviceState& state_of(AppData& a) {
   if (!a.backend_state) {
     auto s = std::make_shared<DeviceState>();
     check(rwh_workspace_create(a.n_input, 1, &s->ws), "rwh_workspace_create");
-    check(rwh_workspace_set_sort_variant(s->ws, redwood::tune::best_sort_variant(a.n_input)),
-          "rwh_workspace_set_sort_variant");
+    const auto tuned = redwood::tune::best_sort_config(a.n_input);
+    check(rwh_workspace_set_sort_variant(s->ws, tuned.variant), "rwh_workspace_set_sort_variant");
+    check(rwh_workspace_set_sort_plan(s->ws, tuned.plan), "rwh_workspace_set_sort_plan");
     void* h = nullptr;
     check(rwh_host_alloc(&h, 3 * sizeof(uint32_t)), "rwh_host_alloc");
     s->h_counts = static_cast<uint32_t*>(h);

@@ -95,13 +95,41 @@ int rwt_dropin_build(const float* xyzw, uint32_t n, int fail_stage, uint32_t* so
 // Best sort tile variant for n keys (measured or read from the tune cache).
 int rwt_best_sort_variant(uint32_t n) { return redwood::tune::best_sort_variant(n); }
 
-// ms per sort of every variant at size n; returns the number of variants written.
-int rwt_measure_sort_variants(uint32_t n, int reps, double* ms_out, int cap) {
-  const auto t = redwood::tune::measure_sort_variants(n, reps);
+// The tuned (tile shape, digit plan) for n keys; chunk != 0: a batch's sort.
+void rwt_best_sort_config(uint32_t n, int chunk, int* variant, int* plan) {
+  const auto c = redwood::tune::best_sort_config(n, chunk != 0);
+  *variant = c.variant;
+  *plan = c.plan;
+}
+
+// ms per sort of every configuration at size n (chunk != 0: chunk-sized
+// tiles only); returns the number of entries written.
+int rwt_measure_sort_configs(uint32_t n, int reps, int chunk, double* ms_out, int* variant_out,
+                             int* plan_out, int cap) {
+  const auto t = redwood::tune::measure_sort_configs(n, reps, chunk != 0);
   int k = 0;
   for (const auto& x : t)
-    if (k < cap) ms_out[k++] = x.ms;
+    if (k < cap) {
+      ms_out[k] = x.ms;
+      variant_out[k] = x.config.variant;
+      plan_out[k] = x.config.plan;
+      ++k;
+    }
   return k;
+}
+
+// The tuning cache without measuring: lookup returns 1 and the cached entry
+// of n's size bucket, or 0; store records an entry and writes the file.
+int rwt_tune_lookup(uint32_t n, int chunk, int* variant, int* plan) {
+  redwood::tune::SortConfig c;
+  if (!redwood::tune::lookup(n, chunk != 0, &c)) return 0;
+  *variant = c.variant;
+  *plan = c.plan;
+  return 1;
+}
+
+void rwt_tune_store(uint32_t n, int chunk, int variant, int plan) {
+  redwood::tune::store(n, chunk != 0, redwood::tune::SortConfig{variant, plan});
 }
 
 // ---- serving ------------------------------------------------------------------

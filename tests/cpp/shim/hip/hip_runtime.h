@@ -36,7 +36,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define __HIP_MEMORY_SCOPE_AGENT 0
 
 using hipStream_t = void*;

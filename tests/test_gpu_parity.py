@@ -13,6 +13,10 @@ from parity_helpers import bits, compare
 pytestmark = pytest.mark.gpu
 
 
+# the benchmark's tile shape (1024 x 16)
+BENCH_SORT_VARIANT = 5
+
+
 def inputs(rw, kind, n, seed):
     if kind == "uniform":
         return rw.uniform_points(n, seed)
@@ -45,10 +49,14 @@ def test_build_matches_oracle(gpu, orc, pool, n, kind):
     # sparse clouds can need more octree nodes than the reference's 0.6 n slots:
     # both sides then report it and keep the first 0.6 n slots (D11)
     assert rc == (-1 if no > rw.octree_capacity(n) else 0)
-    for v in range(len(rw.sort_variants())):
-        print(f"case build kind={kind} n={n} sort_variant={v}", flush=True)
-        g_counts, g = rw.build_octree(pts, strict=False, sort_variant=v)
-        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}")
+    # every tile shape with the default digit plan, every plan with the
+    # benchmark's tile shape
+    configs = [(v, 0) for v in range(len(rw.sort_variants()))]
+    configs += [(BENCH_SORT_VARIANT, p) for p in range(1, len(rw.sort_plans()))]
+    for v, p in configs:
+        print(f"case build kind={kind} n={n} sort_variant={v} plan={p}", flush=True)
+        g_counts, g = rw.build_octree(pts, strict=False, sort_variant=v, sort_plan=p)
+        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}p{p}")
 
 
 @pytest.mark.parametrize("mn,rg", [(0.0, 1024.0), (0.25, 0.5), (-64.0, 2.0 ** 20), (-3.7, 1000.0),
@@ -68,15 +76,12 @@ def test_build_box_matches_oracle(gpu, orc, pool, mn, rg):
     compare(rw, n, g_counts, g, (nu, nb, no), o, f"box {mn} {rg}")
 
 
-# the benchmark's shape (1024 x 16), the default (256 x 16) and the tuner's pick
-BENCH_SORT_VARIANT = 5
-
-
 @pytest.mark.parametrize("kind,n", [("uniform", 1 << 24), ("gmm", 1 << 24), ("gmm", 1 << 22)])
 def test_full_size_matches_oracle(gpu, orc, kind, n):
     """The benchmarked configuration itself (2^24 uniform points, rwh_build_octree
     with the 1024 x 16 sort tile: the fused sweep's 16384-key pass-0 table),
-    the clustered configurations, and the autotuner's shape for the size."""
+    the clustered configurations, and the autotuner's shape and digit plan for
+    the size (the pair the benchmark runs)."""
     rw = gpu
     assert rw.sort_variants()[BENCH_SORT_VARIANT] == (1024, 16)
     pts = inputs(rw, kind, n, seed=114514 if kind == "uniform" else 1)
@@ -84,10 +89,10 @@ def test_full_size_matches_oracle(gpu, orc, kind, n):
     (nu, nb, no, rc), o, _ = orc.build(big, pts)
     assert rc == 0
     ws = rw.Workspace(n)
-    for v in sorted({0, BENCH_SORT_VARIANT, rw.best_sort_variant(n)}):
-        print(f"case full kind={kind} n={n} sort_variant={v}", flush=True)
-        g_counts, g = rw.build_octree(pts, ws=ws, sort_variant=v)
-        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}")
+    for v, p in sorted({(0, 0), (BENCH_SORT_VARIANT, 0), rw.best_sort_config(n)}):
+        print(f"case full kind={kind} n={n} sort_variant={v} plan={p}", flush=True)
+        g_counts, g = rw.build_octree(pts, ws=ws, sort_variant=v, sort_plan=p)
+        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}p{p}")
 
 
 @pytest.mark.parametrize("n", [1, 3, 4, 1000, 4097, 123457, 1 << 20])
