@@ -267,6 +267,93 @@ int rwh_graph_destroy(rwh_graph_t graph);
 int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts,
                           rwh_stream_t stream);
 
+/* ---- packed result: one build's outputs in one buffer (format version 1)
+ * The reference exports nothing beyond printed counts (apps/tree/main.cpp:12-16);
+ * its outputs are AppData's arrays (apps/tree/app_data.hpp:48-96) at the
+ * lengths its count getters give (app_data.hpp:110-133).  A packed result
+ * holds those arrays at exactly those lengths, so a client needs one copy to
+ * the host, and says what it holds, so a reader can check it.
+ *
+ * Layout (little-endian, as host and GPU are): a 256-byte rwh_result_header,
+ * then each selected section in section order, starting at a multiple of 256
+ * bytes from the start of the buffer, `count` entries of `entry_bytes` each,
+ * zero bytes up to the next section.  Counts: sorted keys n_input, unique
+ * keys n_unique, radix-tree and edge arrays n_brt = max(n_unique - 1, 0),
+ * octree arrays oct_rows = min(n_octree_nodes, oct_capacity). */
+#define RWH_RESULT_MAGIC 0x424f5752u /* the bytes "RWOB" */
+#define RWH_RESULT_VERSION 1u
+#define RWH_RESULT_HEADER_BYTES 256u
+#define RWH_RESULT_ALIGN 256u
+enum rwh_result_section {
+  RWH_SEC_SORTED_KEYS = 0,      /* u32: S2 output (u_morton_keys_alt) */
+  RWH_SEC_UNIQUE_KEYS = 1,      /* u32: S3 output (u_morton_keys) */
+  RWH_SEC_PREFIX_N = 2,         /* u8 */
+  RWH_SEC_HAS_LEAF_LEFT = 3,    /* u8 */
+  RWH_SEC_HAS_LEAF_RIGHT = 4,   /* u8 */
+  RWH_SEC_LEFT_CHILD = 5,       /* i32 */
+  RWH_SEC_PARENT = 6,           /* i32 */
+  RWH_SEC_EDGE_COUNT = 7,       /* i32 */
+  RWH_SEC_EDGE_OFFSET = 8,      /* i32 */
+  RWH_SEC_CHILDREN = 9,         /* i32[8] per octree node */
+  RWH_SEC_CORNER = 10,          /* f32[4] per octree node */
+  RWH_SEC_CELL_SIZE = 11,       /* f32 */
+  RWH_SEC_CHILD_NODE_MASK = 12, /* i32 */
+  RWH_SEC_CHILD_LEAF_MASK = 13, /* i32 */
+  RWH_SEC_COUNT = 14
+};
+#define RWH_SECTIONS_ALL 0x3fffu    /* every section */
+#define RWH_SECTIONS_OCTREE 0x3e00u /* the five octree arrays */
+#define RWH_RESULT_TRUNCATED 0x1u   /* flags: n_octree_nodes above oct_capacity (rows cut) */
+typedef struct rwh_result_header {
+  uint32_t magic;          /* RWH_RESULT_MAGIC */
+  uint32_t version;        /* RWH_RESULT_VERSION */
+  uint32_t header_bytes;   /* RWH_RESULT_HEADER_BYTES */
+  uint32_t sections;       /* bit s set: section s present */
+  uint32_t status;         /* the build's status word (RWH_STATUS_*); 0 = valid */
+  uint32_t flags;          /* RWH_RESULT_* */
+  uint32_t n_input, n_unique, n_brt, n_octree_nodes, oct_rows, oct_capacity;
+  uint64_t total_bytes;    /* header + sections + padding */
+  uint64_t offset[RWH_SEC_COUNT];      /* from the start of the buffer; 0 if absent */
+  uint32_t count[RWH_SEC_COUNT];       /* entries; 0 if absent */
+  uint16_t entry_bytes[RWH_SEC_COUNT]; /* 0 if absent */
+  uint32_t reserved;                   /* 0 */
+} rwh_result_header;
+
+/* Size of the packed result of a build with these counts (what a client
+ * copies once it knows them); the largest size any build of n_input points
+ * can give is rwh_result_bytes(n_input, n_input, oct_capacity, oct_capacity,
+ * sections).  0 for sections outside RWH_SECTIONS_ALL. */
+uint64_t rwh_result_bytes(uint32_t n_input, uint32_t n_unique, uint32_t n_octree_nodes,
+                          uint32_t oct_capacity, uint32_t sections);
+/* Enqueues one kernel that writes the packed result of a finished build
+ * (enqueued before it on the stream) to device memory d_out (16-byte
+ * aligned, out_bytes at least the largest size above).  bufs: the cloud's
+ * arrays (for cloud s of a batch, offset by point_base[s] and oct_base[s],
+ * oct_capacity = oct_cap[s]); its counts are read on the device:
+ * n_unique = d_counts[0], n_octree_nodes = d_counts[counts_stride], status =
+ * d_counts[2 * counts_stride] (the workspace counts: d_counts =
+ * rwh_workspace_counts(ws) + s, counts_stride = max_segments; the
+ * allocation-free build: its 3 words, counts_stride = 1).  No host wait. */
+int rwh_pack_result(const rwh_tree_buffers* bufs, const uint32_t* d_counts, uint32_t counts_stride,
+                    uint32_t n_input, uint32_t sections, void* d_out, uint64_t out_bytes,
+                    rwh_stream_t stream);
+/* Host-side reading (no GPU needed).  rwh_result_check returns RWH_OK for a
+ * well-formed result of a valid build: RWH_ERR_INVALID_ARG if the buffer is
+ * not a packed result of this version or its layout disagrees with its
+ * counts (rwh_last_error says what), RWH_ERR_OVERFLOW if the build's status
+ * word is set, RWH_ERR_CAPACITY if the octree rows were cut.
+ * rwh_result_section gives a present section's first entry and its entry
+ * count after the format checks (RWH_ERR_INVALID_ARG if absent), whatever the
+ * build's status.  The buffer may sit at any address (the header is read by
+ * copy); a section's entries are aligned as the buffer is, plus a multiple of
+ * 256 bytes. */
+int rwh_result_check(const void* buf, uint64_t bytes);
+int rwh_result_section(const void* buf, uint64_t bytes, int section, const void** data,
+                       uint32_t* count);
+/* The header of a well-formed result (the format checks above), whatever the
+ * build's status. */
+int rwh_result_header_get(const void* buf, uint64_t bytes, rwh_result_header* out);
+
 /* ---- testing hook: the next call of stage k (1..7; 0 = any) fails with
  * RWH_ERR_INJECTED before enqueuing anything.  Thread-local, one shot. */
 void rwh_debug_fail_next(int stage);

@@ -99,6 +99,11 @@ ABI = {
     "rwh_graph_launch": (_I, [_P, _P]),
     "rwh_graph_destroy": (_I, [_P]),
     "rwh_debug_fail_next": (None, [_I]),
+    "rwh_result_bytes": (C.c_uint64, [_U32, _U32, _U32, _U32, _U32]),
+    "rwh_pack_result": (_I, [_P, _P, _U32, _U32, _U32, _P, C.c_uint64, _P]),
+    "rwh_result_check": (_I, [_P, C.c_uint64]),
+    "rwh_result_section": (_I, [_P, C.c_uint64, _I, C.POINTER(_P), C.POINTER(_U32)]),
+    "rwh_result_header_get": (_I, [_P, C.c_uint64, _P]),
 }
 
 HOST_ABI = {
@@ -108,6 +113,7 @@ HOST_ABI = {
     "rwt_generate_gmm": (None, [C.c_uint64, C.c_uint64, _U32, C.c_double, _P]),
     "rwt_log_uniform_sizes": (None, [C.c_uint64, _U32, C.c_double, C.c_double, _P]),
     "rwt_dropin_build": (_I, [_P, _U32, _I] + [_P] * 14 + [_P]),
+    "rwt_dropin_packed": (_I, [_P, _U32, _U32, _P, C.c_uint64, _P]),
     "rwt_plan_batch": (_U32, [C.c_uint64, _U32, _P, _U32]),
     "rwt_best_sort_variant": (_I, [_U32]),
     "rwt_best_sort_config": (None, [_U32, _I, C.POINTER(_I), C.POINTER(_I)]),
@@ -118,7 +124,7 @@ HOST_ABI = {
     "rwt_server_destroy": (None, [_P]),
     "rwt_server_slots": (_U32, [_P]),
     "rwt_serve_load": (_I, [_P, _U32, _P, _P, C.c_double, _U32, C.c_uint64, _P]),
-    "rwt_serve_mix": (_I, [_P, _U32, _P, _P, _P, _P, _P]),
+    "rwt_serve_mix": (_I, [_P, _U32, _P, _P, _U32, _P, _P, _P, _P]),
 }
 
 
@@ -484,6 +490,96 @@ def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
 
 
 # ---------------------------------------------------------------------------
+# packed results (format version 1 of include/rwhip.h)
+# ---------------------------------------------------------------------------
+RESULT_MAGIC = 0x424F5752
+RESULT_VERSION = 1
+RESULT_HEADER_BYTES = 256
+# section s of a packed result: (name, dtype, elements per entry); names as the
+# keys of build_octree's outputs ("unique" for S3's keys)
+RESULT_SECTIONS = (("morton_keys_alt", np.uint32, 1), ("unique", np.uint32, 1),
+                   ("prefix_n", np.uint8, 1), ("has_leaf_left", np.uint8, 1),
+                   ("has_leaf_right", np.uint8, 1), ("left_child", np.int32, 1),
+                   ("parents", np.int32, 1), ("edge_count", np.int32, 1),
+                   ("edge_offset", np.int32, 1), ("children", np.int32, 8),
+                   ("corner", np.float32, 4), ("cell_size", np.float32, 1),
+                   ("child_node_mask", np.int32, 1), ("child_leaf_mask", np.int32, 1))
+SECTIONS_ALL = 0x3FFF
+SECTIONS_OCTREE = 0x3E00
+
+
+class ResultHeader(C.Structure):
+    """rwh_result_header of include/rwhip.h (256 bytes)."""
+
+    _fields_ = [(f, _U32) for f in ("magic", "version", "header_bytes", "sections", "status", "flags",
+                                    "n_input", "n_unique", "n_brt", "n_octree_nodes", "oct_rows",
+                                    "oct_capacity")] + [
+        ("total_bytes", C.c_uint64), ("offset", C.c_uint64 * 14), ("count", _U32 * 14),
+        ("entry_bytes", C.c_uint16 * 14), ("reserved", _U32)]
+
+
+assert C.sizeof(ResultHeader) == RESULT_HEADER_BYTES
+
+
+def result_bytes(n_input: int, n_unique: int, n_octree_nodes: int, oct_capacity: int,
+                 sections: int = SECTIONS_ALL) -> int:
+    """Size of the packed result of a build with these counts."""
+    return int(lib().rwh_result_bytes(n_input, n_unique, n_octree_nodes, oct_capacity, sections))
+
+
+def _host_bytes(buf):
+    a = np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else buf.view(np.uint8).reshape(-1)
+    return np.ascontiguousarray(a)
+
+
+def result_check(buf) -> int:
+    """rwh_result_check of a host buffer (bytes or numpy): the status code."""
+    a = _host_bytes(buf)
+    return int(lib().rwh_result_check(a.ctypes.data, a.size))
+
+
+def unpack_result(buf):
+    """(header fields as a dict, {section name: array}) of a packed result in
+    host memory, read through the library's checked reader (raises RwhError
+    if the buffer is malformed; the build's status is in the header)."""
+    a = _host_bytes(buf)
+    h = ResultHeader()
+    check(lib().rwh_result_header_get(a.ctypes.data, a.size, C.byref(h)), "rwh_result_header_get")
+    head = {f: getattr(h, f) for f, _ in ResultHeader._fields_ if f not in ("offset", "count", "entry_bytes")}
+    head["offset"], head["count"] = list(h.offset), list(h.count)
+    head["entry_bytes"] = list(h.entry_bytes)
+    out = {}
+    for s, (name, dt, width) in enumerate(RESULT_SECTIONS):
+        if not (h.sections >> s) & 1:
+            continue
+        p, cnt = _P(), _U32()
+        check(lib().rwh_result_section(a.ctypes.data, a.size, s, C.byref(p), C.byref(cnt)),
+              "rwh_result_section")
+        off = p.value - a.ctypes.data
+        arr = a[off:off + cnt.value * width * np.dtype(dt).itemsize].view(dt)
+        out[name] = arr.reshape(-1, width) if width > 1 else arr
+    return head, out
+
+
+def pack_result(tb: "TreeBuild", counts_ptr: int, counts_stride: int, n_input: int,
+                sections: int = SECTIONS_ALL, stream=None, out: DeviceArray | None = None,
+                tree: "TreeBuffers | None" = None) -> np.ndarray:
+    """Packs a finished build on the device (rwh_pack_result) and copies the
+    whole result to the host in one transfer; returns it as uint8."""
+    t = tree if tree is not None else tb.struct
+    cap = t.oct_capacity
+    need = result_bytes(n_input, n_input, cap, cap, sections)
+    if need == 0:
+        check(RWH_ERR_INVALID_ARG, "pack_result: sections")
+    dev = out if out is not None else DeviceArray(need, np.uint8)
+    check(lib().rwh_pack_result(C.addressof(t), counts_ptr, counts_stride, n_input, sections, dev.ptr,
+                                dev.nbytes, stream), "rwh_pack_result")
+    head = dev.numpy(RESULT_HEADER_BYTES, stream)
+    total = int(head[48:56].view(np.uint64)[0])
+    return dev.numpy(min(total, dev.nbytes), stream)
+
+
+# ---------------------------------------------------------------------------
 # batched build
 # ---------------------------------------------------------------------------
 def batch_layout(sizes):
@@ -520,6 +616,15 @@ class Batch:
         check(lib().rwh_read_batch_counts(self.ws.ptr, k, out.ctypes.data, stream), "rwh_read_batch_counts")
         return out[:k], out[k:]
 
+    def cloud_buffers(self, s: int) -> TreeBuffers:
+        """rwh_tree_buffers of cloud s alone (pointers offset by its bases,
+        oct_capacity = its capacity), e.g. for rwh_pack_result."""
+        b, ob = int(self.pb[s]), int(self.ob[s])
+        fields = {}
+        for name, (dt, k, per_point) in TREE_FIELDS.items():
+            fields[name] = self.tb.arrays[name].ptr + (b if per_point else ob) * k * np.dtype(dt).itemsize
+        return TreeBuffers(points=self.tb.points.ptr + 16 * b, oct_capacity=int(self.oc[s]), **fields)
+
     def read(self):
         """Per-cloud host outputs, cut to each cloud's counts."""
         nu, no = self.counts()
@@ -540,6 +645,20 @@ class Batch:
                 r[k] = full[k][ob:ob + rows]
             res.append(r)
         return res
+
+
+def dropin_packed(points: np.ndarray, sections: int = SECTIONS_ALL) -> np.ndarray:
+    """One build through the drop-in path, read back with the C++
+    redwood::hip::read_result (a packed result, uint8)."""
+    n = points.shape[0]
+    cap = octree_capacity(n)
+    buf = np.zeros(max(result_bytes(n, n, cap, cap, sections), 16), np.uint8)
+    size = C.c_uint64(0)
+    pts = np.ascontiguousarray(points, np.float32)
+    rc = host().rwt_dropin_packed(pts.ctypes.data, n, sections, buf.ctypes.data, buf.size, C.byref(size))
+    if rc != 0:
+        raise RwhError(rc, host().rwt_last_error().decode())
+    return buf[:size.value]
 
 
 def dropin_build(points: np.ndarray, fail_stage: int = 0):
@@ -593,42 +712,26 @@ class Server:
             raise RwhError(rc, host().rwt_last_error().decode())
         return dict(zip(SERVE_STATS, stats.tolist()))
 
-    # the arrays a kept result carries, in rwt_serve_mix's field order
-    MIX_FIELDS = ("morton_keys_alt", "unique", "prefix_n", "has_leaf_left", "has_leaf_right",
-                  "left_child", "parents", "edge_count", "edge_offset", "children", "corner",
-                  "cell_size", "child_node_mask", "child_leaf_mask")
-
-    def mix(self, clouds):
-        """All clouds submitted together, outputs kept; per-request results
-        ((n_unique, n_brt, n_oct), arrays, batch size)."""
+    def mix(self, clouds, sections: int = SECTIONS_ALL):
+        """All clouds submitted together, each asking for a packed result of
+        the given sections; per request (packed result as uint8, batch size)."""
         k = len(clouds)
-        outs = []
+        bufs = []
         for c in clouds:
             n = c.shape[0]
-            cap = max(octree_capacity(n), 1)
-            o = {}
-            for f in self.MIX_FIELDS:
-                src = "morton_keys" if f == "unique" else f
-                dt, width, per_point = TREE_FIELDS[src]
-                o[f] = np.zeros((max(n, 1) if per_point else cap) * width, dt)
-            outs.append(o)
-        table = (C.c_void_p * (len(self.MIX_FIELDS) * k))(
-            *[outs[r][f].ctypes.data for f in self.MIX_FIELDS for r in range(k)])
+            cap = octree_capacity(n)
+            bufs.append(np.zeros(max(result_bytes(n, n, cap, cap, sections), 16), np.uint8))
+        outs = (C.c_void_p * k)(*[b.ctypes.data for b in bufs])
+        caps = np.array([b.size for b in bufs], np.uint64)
+        sizes_out = np.zeros(k, np.uint64)
         ptrs = (C.c_void_p * k)(*[c.ctypes.data for c in clouds])
         sizes = np.array([c.shape[0] for c in clouds], np.uint32)
-        counts = np.zeros(3 * k, np.uint32)
         bsz = np.zeros(k, np.uint32)
-        rc = host().rwt_serve_mix(self.ptr, k, ptrs, sizes.ctypes.data, table,
-                                  counts.ctypes.data, bsz.ctypes.data)
+        rc = host().rwt_serve_mix(self.ptr, k, ptrs, sizes.ctypes.data, sections, outs, caps.ctypes.data,
+                                  sizes_out.ctypes.data, bsz.ctypes.data)
         if rc != 0:
             raise RwhError(rc, host().rwt_last_error().decode())
-        res = []
-        for r in range(k):
-            o = outs[r]
-            o["children"] = o["children"].reshape(-1, 8)
-            o["corner"] = o["corner"].reshape(-1, 4)
-            res.append((tuple(int(x) for x in counts[3 * r:3 * r + 3]), o, int(bsz[r])))
-        return res
+        return [(bufs[r][:int(sizes_out[r])], int(bsz[r])) for r in range(k)]
 
     def close(self):
         if self.ptr:

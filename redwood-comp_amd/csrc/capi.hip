@@ -959,4 +959,115 @@ int rwh_read_batch_counts(rwh_workspace* ws, uint32_t n_clouds, uint32_t* counts
   return RWH_OK;
 }
 
+// ---- packed result ----
+
+uint64_t rwh_result_bytes(uint32_t n_input, uint32_t n_unique, uint32_t n_octree_nodes,
+                          uint32_t oct_capacity, uint32_t sections) {
+  if ((sections & ~RWH_SECTIONS_ALL) != 0) return 0;
+  return result_layout(n_input, std::min(n_unique, n_input), std::min(n_octree_nodes, oct_capacity), sections)
+      .total;
+}
+
+int rwh_pack_result(const rwh_tree_buffers* bufs, const uint32_t* d_counts, uint32_t counts_stride,
+                    uint32_t n_input, uint32_t sections, void* d_out, uint64_t out_bytes,
+                    rwh_stream_t stream) {
+  RWH_CHECK(bufs && d_counts && d_out, "null argument");
+  RWH_CHECK(sections != 0 && (sections & ~RWH_SECTIONS_ALL) == 0, "sections outside RWH_SECTIONS_ALL");
+  RWH_CHECK(counts_stride >= 1, "counts_stride must be at least 1");
+  RWH_CHECK(n_input <= kMaxPoints, "n_input above 2^30");
+  RWH_CHECK(aligned16(d_out), "d_out must be 16-byte aligned");
+  const uint32_t cap = bufs->oct_capacity;
+  const uint64_t need = result_layout(n_input, n_input, cap, sections).total;
+  if (out_bytes < need)
+    return fail(RWH_ERR_CAPACITY, "d_out holds " + std::to_string(out_bytes) + " bytes; a result of " +
+                                      std::to_string(n_input) + " points can need " + std::to_string(need));
+  const void* src[RWH_SEC_COUNT] = {bufs->morton_keys_alt, bufs->morton_keys,  bufs->prefix_n,
+                                    bufs->has_leaf_left,   bufs->has_leaf_right, bufs->left_child,
+                                    bufs->parents,         bufs->edge_count,   bufs->edge_offset,
+                                    bufs->children,        bufs->corner,       bufs->cell_size,
+                                    bufs->child_node_mask, bufs->child_leaf_mask};
+  PackArgs a{};
+  for (int s = 0; s < RWH_SEC_COUNT; ++s) {
+    if (!((sections >> s) & 1u)) continue;
+    RWH_CHECK(src[s] != nullptr, "a selected section's buffer is null");
+    RWH_CHECK(aligned16(src[s]), "a selected section's buffer is not 16-byte aligned");
+    a.src[s] = static_cast<const unsigned char*>(src[s]);
+  }
+  a.counts = d_counts;
+  a.stride = counts_stride;
+  a.n_input = n_input;
+  a.oct_capacity = cap;
+  a.sections = sections;
+  a.out = static_cast<unsigned char*>(d_out);
+  launch_pack_result(a, need, static_cast<hipStream_t>(stream));
+  return launched();
+}
+
+}  // extern "C"
+
+namespace {
+
+// The format checks shared by rwh_result_check and rwh_result_section.
+int check_result_format(const void* buf, uint64_t bytes, rwh_result_header* out) {
+  RWH_CHECK(buf, "null buffer");
+  RWH_CHECK(bytes >= RWH_RESULT_HEADER_BYTES, "buffer shorter than a result header");
+  rwh_result_header h;
+  std::memcpy(&h, buf, sizeof h);
+  RWH_CHECK(h.magic == RWH_RESULT_MAGIC, "not a packed result (magic)");
+  RWH_CHECK(h.version == RWH_RESULT_VERSION,
+            "packed result version " + std::to_string(h.version) + "; this library reads version " +
+                std::to_string(RWH_RESULT_VERSION));
+  RWH_CHECK(h.header_bytes == RWH_RESULT_HEADER_BYTES && h.reserved == 0, "header size or reserved word");
+  RWH_CHECK(h.sections != 0 && (h.sections & ~RWH_SECTIONS_ALL) == 0, "sections outside RWH_SECTIONS_ALL");
+  RWH_CHECK(h.n_unique <= h.n_input && h.n_brt == (h.n_unique > 0 ? h.n_unique - 1 : 0),
+            "counts disagree (n_unique, n_brt, n_input)");
+  RWH_CHECK(h.oct_rows == std::min(h.n_octree_nodes, h.oct_capacity) &&
+                h.flags == (h.n_octree_nodes > h.oct_capacity ? RWH_RESULT_TRUNCATED : 0u),
+            "octree rows or flags disagree with the counts");
+  const ResultLayout L = result_layout(h.n_input, h.n_unique, h.oct_rows, h.sections);
+  RWH_CHECK(h.total_bytes == L.total, "total_bytes disagrees with the counts");
+  RWH_CHECK(h.total_bytes <= bytes, "buffer shorter than total_bytes (" + std::to_string(bytes) + " < " +
+                                        std::to_string(h.total_bytes) + ")");
+  for (int s = 0; s < RWH_SEC_COUNT; ++s) {
+    const bool on = (h.sections >> s) & 1u;
+    RWH_CHECK(h.offset[s] == L.offset[s] && h.count[s] == L.count[s] &&
+                  h.entry_bytes[s] == (on ? result_entry_bytes(s) : 0u),
+              "section " + std::to_string(s) + ": offset, count or entry size disagrees with the layout");
+  }
+  *out = h;
+  return RWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rwh_result_check(const void* buf, uint64_t bytes) {
+  rwh_result_header h;
+  RWH_TRY(check_result_format(buf, bytes, &h));
+  if (h.status != 0)
+    return fail(RWH_ERR_OVERFLOW, "build status " + std::to_string(h.status) + ": " + status_text(h.status));
+  if (h.flags & RWH_RESULT_TRUNCATED)
+    return fail(RWH_ERR_CAPACITY, "octree of " + std::to_string(h.n_octree_nodes) + " nodes above its capacity " +
+                                      std::to_string(h.oct_capacity) + "; rows cut");
+  return RWH_OK;
+}
+
+int rwh_result_section(const void* buf, uint64_t bytes, int section, const void** data,
+                       uint32_t* count) {
+  RWH_CHECK(data && count, "null argument");
+  RWH_CHECK(section >= 0 && section < RWH_SEC_COUNT, "section id outside [0, RWH_SEC_COUNT)");
+  rwh_result_header h;
+  RWH_TRY(check_result_format(buf, bytes, &h));
+  RWH_CHECK((h.sections >> section) & 1u, "section " + std::to_string(section) + " not in this result");
+  *data = static_cast<const unsigned char*>(buf) + h.offset[section];
+  *count = h.count[section];
+  return RWH_OK;
+}
+
+int rwh_result_header_get(const void* buf, uint64_t bytes, rwh_result_header* out) {
+  RWH_CHECK(out, "null argument");
+  return check_result_format(buf, bytes, out);
+}
+
 }  // extern "C"

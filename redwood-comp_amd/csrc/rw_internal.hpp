@@ -258,4 +258,53 @@ void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, bool check, 
 void launch_fill_u32(uint32_t* dst, uint32_t value, uint32_t count, hipStream_t stream);
 void launch_link_leaf(const OctArgs& a, SegArgs seg, uint32_t n_index, hipStream_t stream);
 
+// ---- packed result (format version 1 of include/rwhip.h) ----
+static_assert(sizeof(rwh_result_header) == RWH_RESULT_HEADER_BYTES, "header is 256 bytes");
+__host__ __device__ constexpr uint32_t result_entry_bytes(int s) {
+  return s <= RWH_SEC_UNIQUE_KEYS ? 4u
+         : s <= RWH_SEC_HAS_LEAF_RIGHT ? 1u
+         : s <= RWH_SEC_EDGE_OFFSET ? 4u
+         : s == RWH_SEC_CHILDREN ? 32u
+         : s == RWH_SEC_CORNER ? 16u
+                               : 4u;
+}
+struct ResultLayout {
+  uint64_t offset[RWH_SEC_COUNT];
+  uint32_t count[RWH_SEC_COUNT];
+  uint64_t total;
+};
+// Sections in order after the header, each at a multiple of RWH_RESULT_ALIGN;
+// n_unique <= n_input and rows <= the octree capacity are the caller's.
+__host__ __device__ constexpr ResultLayout result_layout(uint32_t n_input, uint32_t n_unique,
+                                                         uint32_t rows, uint32_t sections) {
+  ResultLayout L{};
+  const uint32_t n_brt = n_unique > 0 ? n_unique - 1 : 0;
+  uint64_t at = RWH_RESULT_HEADER_BYTES;
+  for (int s = 0; s < RWH_SEC_COUNT; ++s) {
+    L.offset[s] = 0;
+    L.count[s] = 0;
+    if (!((sections >> s) & 1u)) continue;
+    const uint32_t c = s == RWH_SEC_SORTED_KEYS   ? n_input
+                       : s == RWH_SEC_UNIQUE_KEYS ? n_unique
+                       : s <= RWH_SEC_EDGE_OFFSET ? n_brt
+                                                  : rows;
+    L.offset[s] = at;
+    L.count[s] = c;
+    at = (at + static_cast<uint64_t>(c) * result_entry_bytes(s) + RWH_RESULT_ALIGN - 1) &
+         ~static_cast<uint64_t>(RWH_RESULT_ALIGN - 1);
+  }
+  L.total = at;
+  return L;
+}
+struct PackArgs {
+  const unsigned char* src[RWH_SEC_COUNT];  // the cloud's arrays (absent sections: nullptr)
+  const uint32_t* counts;                   // n_unique, n_oct, status at 0, stride, 2 stride
+  uint32_t stride;
+  uint32_t n_input, oct_capacity, sections;
+  unsigned char* out;
+};
+// One kernel: header + sections, 16 bytes per work-item step; the grid covers
+// max_bytes (the largest result of n_input points).
+void launch_pack_result(const PackArgs& a, uint64_t max_bytes, hipStream_t stream);
+
 }  // namespace rw

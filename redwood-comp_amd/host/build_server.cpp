@@ -30,7 +30,7 @@ struct BuildServer::Request {
   uint64_t id;
   const float* pts;
   uint32_t n;
-  bool keep;
+  uint32_t sections;  // packed result sections (0: counts only)
   double t_arrival;
   std::promise<BuildResult> done;
 };
@@ -47,6 +47,11 @@ struct BuildServer::Slot {
   // the batch in flight
   std::vector<std::unique_ptr<Request>> batch;
   std::vector<uint32_t> n, pb, ob, oc;
+  // packed results of the batch in flight: device arena (grown on demand)
+  // and each request's offset in it
+  void* arena = nullptr;
+  uint64_t arena_bytes = 0;
+  std::vector<uint64_t> arena_off;
   bool busy = false;
   bool has_work = false;
   bool stop = false;
@@ -140,6 +145,7 @@ BuildServer::~BuildServer() {
     if (s->completion.joinable()) s->completion.join();
     (void)rwh_stream_sync(s->stream);
     for (void* p : s->allocs) (void)rwh_free(p);
+    if (s->arena) (void)rwh_free(s->arena);
     (void)rwh_workspace_destroy(s->ws);
     (void)rwh_event_destroy(s->done_ev);
     (void)rwh_host_free(s->h_counts);
@@ -151,14 +157,19 @@ double BuildServer::now() const {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
 }
 
-std::future<BuildResult> BuildServer::submit(const float* xyzw, uint32_t n, bool keep,
+std::future<BuildResult> BuildServer::submit(const float* xyzw, uint32_t n, uint32_t sections,
                                              double t_arrival) {
   auto r = std::make_unique<Request>();
   r->pts = xyzw;
   r->n = n;
-  r->keep = keep;
+  r->sections = sections;
   r->t_arrival = t_arrival < 0 ? now() : t_arrival;
   std::future<BuildResult> f = r->done.get_future();
+  if ((sections & ~RWH_SECTIONS_ALL) != 0) {
+    r->done.set_exception(std::make_exception_ptr(
+        Error(RWH_ERR_INVALID_ARG, "BuildServer: sections outside RWH_SECTIONS_ALL")));
+    return f;
+  }
   if (!planner_.fits_empty(n) || (n > 0 && !xyzw)) {
     r->done.set_exception(std::make_exception_ptr(
         Error(RWH_ERR_CAPACITY, "BuildServer: a cloud of " + std::to_string(n) +
@@ -214,6 +225,57 @@ void BuildServer::scheduler_loop() {
   }
 }
 
+// Enqueues the packing of every request of the batch that asked for
+// sections, into the slot's arena (grown when too small: the slot's stream is
+// idle here, its last batch finished before the slot was freed).
+int BuildServer::pack_results(Slot* s, const std::vector<std::unique_ptr<Request>>& batch) {
+  const uint32_t k = static_cast<uint32_t>(batch.size());
+  s->arena_off.assign(k, 0);
+  uint64_t need = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    if (batch[i]->sections == 0) continue;
+    s->arena_off[i] = need;
+    need += (rwh_result_bytes(s->n[i], s->n[i], s->oc[i], s->oc[i], batch[i]->sections) + 255) / 256 * 256;
+  }
+  if (need == 0) return RWH_OK;
+  if (need > s->arena_bytes) {
+    if (s->arena) (void)rwh_free(s->arena);
+    s->arena = nullptr;
+    s->arena_bytes = 0;
+    const int st = rwh_malloc(&s->arena, need);
+    if (st != RWH_OK) return st;
+    s->arena_bytes = need;
+  }
+  const uint32_t* d_counts = rwh_workspace_counts(s->ws);
+  for (uint32_t i = 0; i < k; ++i) {
+    const uint32_t sections = batch[i]->sections;
+    if (sections == 0) continue;
+    rwh_tree_buffers b = s->bufs;  // the cloud's arrays
+    const size_t p = s->pb[i], o = s->ob[i];
+    b.points += 4 * p;
+    b.morton_keys += p;
+    b.morton_keys_alt += p;
+    b.prefix_n += p;
+    b.has_leaf_left += p;
+    b.has_leaf_right += p;
+    b.left_child += p;
+    b.parents += p;
+    b.edge_count += p;
+    b.edge_offset += p;
+    b.children += 8 * o;
+    b.corner += 4 * o;
+    b.cell_size += o;
+    b.child_node_mask += o;
+    b.child_leaf_mask += o;
+    b.oct_capacity = s->oc[i];
+    const int st = rwh_pack_result(&b, d_counts + i, s->max_clouds, s->n[i], sections,
+                                   static_cast<char*>(s->arena) + s->arena_off[i],
+                                   s->arena_bytes - s->arena_off[i], s->stream);
+    if (st != RWH_OK) return st;
+  }
+  return RWH_OK;
+}
+
 void BuildServer::launch(Slot* s, std::vector<std::unique_ptr<Request>> batch) {
   const uint32_t k = static_cast<uint32_t>(batch.size());
   s->n.assign(k, 0);
@@ -243,6 +305,7 @@ void BuildServer::launch(Slot* s, std::vector<std::unique_ptr<Request>> batch) {
   if (st == RWH_OK)
     st = rwh_memcpy_d2h(s->h_counts + 2 * s->max_clouds, d_counts + 2 * s->max_clouds,
                         sizeof(uint32_t) * k, s->stream);
+  if (st == RWH_OK) st = pack_results(s, batch);
   if (st == RWH_OK) st = rwh_event_record(s->done_ev, s->stream);
   if (st != RWH_OK) {
     const std::string msg = std::string("BuildServer launch: ") + rwh_error_string(st) + " (" +
@@ -293,28 +356,14 @@ void BuildServer::completion_loop(Slot* s) {
                                             std::to_string(s->h_counts[2 * s->max_clouds + i]) + ")");
         if (res.n_octree_nodes > s->oc[i])
           throw Error(RWH_ERR_CAPACITY, "BuildServer: octree above the cloud's 0.6 n capacity");
-        if (r.keep) {
-          const size_t b = s->pb[i], o = s->ob[i];
-          const size_t nu = res.n_unique, nb = nu > 0 ? nu - 1 : 0, rows = res.n_octree_nodes;
-          const auto get = [&](auto& v, const auto* src, size_t count) {
-            v.resize(count);
-            if (count) check(rwh_memcpy_d2h(v.data(), src, sizeof(v[0]) * count, s->stream), "d2h");
-          };
-          get(res.sorted, s->bufs.morton_keys_alt + b, r.n);
-          get(res.unique_keys, s->bufs.morton_keys + b, nu);
-          get(res.prefix_n, s->bufs.prefix_n + b, nb);
-          get(res.has_leaf_left, s->bufs.has_leaf_left + b, nb);
-          get(res.has_leaf_right, s->bufs.has_leaf_right + b, nb);
-          get(res.left_child, s->bufs.left_child + b, nb);
-          get(res.parents, s->bufs.parents + b, nb);
-          get(res.edge_count, s->bufs.edge_count + b, nb);
-          get(res.edge_offset, s->bufs.edge_offset + b, nb);
-          get(res.children, s->bufs.children + 8 * o, 8 * rows);
-          get(res.corner, s->bufs.corner + 4 * o, 4 * rows);
-          get(res.cell_size, s->bufs.cell_size + o, rows);
-          get(res.child_node_mask, s->bufs.child_node_mask + o, rows);
-          get(res.child_leaf_mask, s->bufs.child_leaf_mask + o, rows);
+        if (r.sections != 0) {
+          const uint64_t bytes =
+              rwh_result_bytes(r.n, res.n_unique, res.n_octree_nodes, s->oc[i], r.sections);
+          std::vector<uint8_t> blob(bytes);
+          check(rwh_memcpy_d2h(blob.data(), static_cast<char*>(s->arena) + s->arena_off[i], bytes, s->stream),
+                "rwh_memcpy_d2h");
           check(rwh_stream_sync(s->stream), "rwh_stream_sync");
+          res.packed = redwood::result::PackedResult(std::move(blob));
         }
         r.done.set_value(std::move(res));
       } catch (...) {
@@ -351,7 +400,7 @@ LoadStats run_open_loop(BuildServer& server, const std::vector<const float*>& cl
     const double ta = t_start + arrival[r];
     const double wait = ta - server.now();
     if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-    fut.push_back(server.submit(clouds[which[r]], sizes[which[r]], false, ta));
+    fut.push_back(server.submit(clouds[which[r]], sizes[which[r]], 0, ta));
   }
   std::vector<double> lat;
   lat.reserve(n_requests);

@@ -17,7 +17,9 @@
 //     requests that arrive while a batch is on the GPU are admitted into the
 //     next free slot and overlap with it on the device.
 //   * Completion.  Each slot has a completion thread; results (counts, and
-//     on request copies of every output array) are delivered through futures.
+//     on request the build's arrays in the packed result format, packed on
+//     the device right after the batch and copied in one transfer per
+//     request) are delivered through futures.
 #pragma once
 
 #include <atomic>
@@ -31,6 +33,7 @@
 #include <thread>
 #include <vector>
 
+#include "result.hpp"
 #include "rwhip.h"
 
 namespace redwood::serve {
@@ -81,12 +84,9 @@ struct BuildResult {
   double t_arrival = 0;  // seconds, server clock
   double t_done = 0;
   uint32_t batch_size = 0;
-  // filled when keep_outputs: the AppData arrays of the build
-  std::vector<uint32_t> sorted, unique_keys;
-  std::vector<uint8_t> prefix_n, has_leaf_left, has_leaf_right;
-  std::vector<int32_t> left_child, parents, edge_count, edge_offset, children, child_node_mask,
-      child_leaf_mask;
-  std::vector<float> corner, cell_size;
+  // the selected arrays of the build in the packed result format
+  // (include/rwhip.h), when the request asked for sections; empty otherwise
+  redwood::result::PackedResult packed;
 };
 
 class BuildServer {
@@ -97,9 +97,11 @@ class BuildServer {
   BuildServer& operator=(const BuildServer&) = delete;
 
   // Queues a build of n points (x,y,z,w floats; the memory must stay valid
-  // until the future is ready; pinned memory uploads fastest).  t_arrival:
-  // arrival time on now()'s clock (< 0: now).
-  std::future<BuildResult> submit(const float* xyzw, uint32_t n, bool keep_outputs = false,
+  // until the future is ready; pinned memory uploads fastest).  sections:
+  // the arrays the result carries (RWH_SEC_* bits, e.g. RWH_SECTIONS_ALL;
+  // 0: counts only), packed on the device and copied to the host in one
+  // transfer.  t_arrival: arrival time on now()'s clock (< 0: now).
+  std::future<BuildResult> submit(const float* xyzw, uint32_t n, uint32_t sections = 0,
                                   double t_arrival = -1.0);
   double now() const;
   void drain();  // waits until every submitted request is finished
@@ -114,6 +116,7 @@ class BuildServer {
   void scheduler_loop();
   void completion_loop(Slot* s);
   void launch(Slot* s, std::vector<std::unique_ptr<Request>> batch);
+  int pack_results(Slot* s, const std::vector<std::unique_ptr<Request>>& batch);
 
   ServerConfig cfg_;
   Planner planner_;

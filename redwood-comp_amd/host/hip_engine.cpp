@@ -334,4 +334,23 @@ std::future<void> run_link_leaf(AppData& a) {
       [](DeviceState& s) { raise_status(s, "run_link_leaf"); });
 }
 
+redwood::result::PackedResult read_result(AppData& a, uint32_t sections) {
+  Engine& e = engine_of(a);
+  DeviceState& s = state_of(a);
+  const rwh_tree_buffers b = buffers_of(a);
+  const uint64_t max_bytes = rwh_result_bytes(a.n_input, a.n_input, b.oct_capacity, b.oct_capacity, sections);
+  if (max_bytes == 0) throw Error(RWH_ERR_INVALID_ARG, "read_result: sections outside RWH_SECTIONS_ALL");
+  void* scratch = nullptr;
+  check(rwh_malloc(&scratch, max_bytes), "rwh_malloc");
+  try {
+    auto out = redwood::result::fetch(b, d_n_unique(s), 1, a.n_input, sections, scratch, max_bytes, e.stream());
+    check(rwh_free(scratch), "rwh_free");
+    return out;
+  } catch (...) {
+    (void)rwh_stream_sync(e.stream());
+    (void)rwh_free(scratch);
+    throw;
+  }
+}
+
 }  // namespace redwood::hip

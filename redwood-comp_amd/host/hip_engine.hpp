@@ -19,6 +19,7 @@
 
 #include "app_data.hpp"
 #include "redwood_base.hpp"
+#include "result.hpp"
 #include "rwhip.h"
 
 namespace redwood::hip {
@@ -127,5 +128,11 @@ std::future<void> run_stage6(AppData& app_data);  // edge counts -> offsets
 std::future<void> run_stage7(AppData& app_data);  // everything -> octree nodes
 // Opt-in leaf linking (the reference's never-launched k_LinkLeafNodes).
 std::future<void> run_link_leaf(AppData& app_data);
+
+// The build's arrays in the packed result format (include/rwhip.h), after
+// the stages run on app_data have finished: packed on the device and copied
+// to the host in one transfer (waits for the engine's stream).  sections:
+// RWH_SEC_* bits.
+redwood::result::PackedResult read_result(AppData& app_data, uint32_t sections = RWH_SECTIONS_ALL);
 
 }  // namespace redwood::hip

@@ -91,6 +91,29 @@ int rwt_dropin_build(const float* xyzw, uint32_t n, int fail_stage, uint32_t* so
   }
 }
 
+// One build through run_stage1..7 on a fresh Engine, read back with
+// redwood::hip::read_result: the packed result (sections) goes to out (host,
+// cap bytes), its size to *bytes.  Returns 0, or -1 with the message in
+// rwt_last_error().
+int rwt_dropin_packed(const float* xyzw, uint32_t n, uint32_t sections, void* out, uint64_t cap,
+                      uint64_t* bytes) {
+  using namespace redwood::hip;
+  try {
+    auto engine = Engine::create();
+    AppData app(engine->get_mr(), xyzw, n);
+    for (auto run : {run_stage1, run_stage2, run_stage3, run_stage4, run_stage5, run_stage6, run_stage7})
+      run(app).get();
+    const auto packed = read_result(app, sections);
+    if (packed.bytes().size() > cap) throw std::runtime_error("packed result larger than its buffer");
+    std::memcpy(out, packed.bytes().data(), packed.bytes().size());
+    *bytes = packed.bytes().size();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 // ---- autotuning ----------------------------------------------------------------
 // Best sort tile variant for n keys (measured or read from the tune cache).
 int rwt_best_sort_variant(uint32_t n) { return redwood::tune::best_sort_variant(n); }
@@ -188,54 +211,34 @@ int rwt_serve_load(void* server, uint32_t n_clouds, const float* const* clouds,
   }
 }
 
-// Submits every cloud at once with outputs kept, waits, and copies the
-// results of request r into the arrays at the r-th entry of each pointer
-// table (sizes as in rwt_dropin_build; counts3[r*3..]).  Returns 0, or 1 + r
-// for the first failed request.
-// Submits every cloud at once (keep_outputs) and copies each result back.
-// outs is field-major: outs[f * n_clouds + r] is request r's host array for
-// field f, in the order sorted, unique keys, prefix_n, has_leaf_left,
-// has_leaf_right, left_child, parents, edge_count, edge_offset, children,
-// corner, cell_size, child_node_mask, child_leaf_mask (sized as the AppData
-// buffers of that cloud).  counts3[3r..] = {n_unique, n_brt, n_octree_nodes};
-// batch_sizes[r] = clouds in the batch request r ran in.
-int rwt_serve_mix(void* server, uint32_t n_clouds, const float* const* clouds,
-                  const uint32_t* sizes, void* const* outs, uint32_t* counts3, uint32_t* batch_sizes) {
+// Submits every cloud at once, each asking for the given packed result
+// sections, waits, and copies request r's packed result into outs[r] (host,
+// out_caps[r] bytes; out_bytes[r] gets its size).  batch_sizes[r] = clouds
+// in the batch request r ran in.  Returns 0, or 1 + r for the first failed
+// request (the message in rwt_last_error()).
+int rwt_serve_mix(void* server, uint32_t n_clouds, const float* const* clouds, const uint32_t* sizes,
+                  uint32_t sections, void* const* outs, const uint64_t* out_caps, uint64_t* out_bytes,
+                  uint32_t* batch_sizes) {
   auto& s = *static_cast<redwood::serve::BuildServer*>(server);
   std::vector<std::future<redwood::serve::BuildResult>> f;
-  for (uint32_t r = 0; r < n_clouds; ++r) f.push_back(s.submit(clouds[r], sizes[r], true));
+  for (uint32_t r = 0; r < n_clouds; ++r) f.push_back(s.submit(clouds[r], sizes[r], sections));
+  int failed = 0;
   for (uint32_t r = 0; r < n_clouds; ++r) {
     try {
       const auto res = f[r].get();
-      int field = 0;
-      const auto cp = [&](const auto& v) {
-        void* dst = outs[static_cast<size_t>(field++) * n_clouds + r];
-        if (!v.empty()) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
-      };
-      cp(res.sorted);
-      cp(res.unique_keys);
-      cp(res.prefix_n);
-      cp(res.has_leaf_left);
-      cp(res.has_leaf_right);
-      cp(res.left_child);
-      cp(res.parents);
-      cp(res.edge_count);
-      cp(res.edge_offset);
-      cp(res.children);
-      cp(res.corner);
-      cp(res.cell_size);
-      cp(res.child_node_mask);
-      cp(res.child_leaf_mask);
-      counts3[3 * r] = res.n_unique;
-      counts3[3 * r + 1] = res.n_unique > 0 ? res.n_unique - 1 : 0;
-      counts3[3 * r + 2] = res.n_octree_nodes;
+      const auto& bytes = res.packed.bytes();
+      if (bytes.size() > out_caps[r]) throw std::runtime_error("packed result larger than its buffer");
+      if (!bytes.empty()) std::memcpy(outs[r], bytes.data(), bytes.size());
+      out_bytes[r] = bytes.size();
       batch_sizes[r] = res.batch_size;
     } catch (const std::exception& e) {
-      g_err = e.what();
-      return 1 + static_cast<int>(r);
+      if (failed == 0) {
+        g_err = e.what();
+        failed = 1 + static_cast<int>(r);
+      }
     }
   }
-  return 0;
+  return failed;
 }
 
 }  // extern "C"

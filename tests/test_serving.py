@@ -1,7 +1,8 @@
 """Serving: batch planning (CPU) and the continuous-batching server (GPU).
 
-The server's results must equal one plain build of the same cloud, whatever
-batch a request landed in and whatever ran beside it.
+The server's results (packed results, include/rwhip.h) must equal one plain
+build of the same cloud, whatever batch a request landed in and whatever ran
+beside it.
 """
 import numpy as np
 import pytest
@@ -37,20 +38,24 @@ def test_mix_equals_alone(gpu):
               for i, n in enumerate(sizes)]
     srv = rw.Server(pool_bytes=8 << 30, n_slots=2, slot_points=1 << 19, max_clouds=8)
     res = srv.mix(clouds)
-    assert max(b for _, _, b in res) > 1  # requests really were coalesced
+    assert max(b for _, b in res) > 1  # requests really were coalesced
     for i, c in enumerate(clouds):
-        (nu, nb, no), o, _ = res[i]
+        blob, _ = res[i]
+        assert rw.result_check(blob) == rw.RWH_OK, i
+        head, o = rw.unpack_result(blob)
+        nu, nb, no = head["n_unique"], head["n_brt"], head["n_octree_nodes"]
         g_counts, g = rw.build_octree(c)
         assert (nu, nb, no) == g_counts, i
         n = c.shape[0]
-        assert np.array_equal(o["morton_keys_alt"][:n], g["morton_keys_alt"][:n])
-        assert np.array_equal(o["unique"][:nu], g["morton_keys"][:nu])
+        assert np.array_equal(o["morton_keys_alt"], g["morton_keys_alt"][:n])
+        assert np.array_equal(o["unique"], g["morton_keys"][:nu])
         for k in ("prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parents",
                   "edge_count", "edge_offset"):
-            assert np.array_equal(o[k][:nb], g[k][:nb]), (i, k)
+            assert np.array_equal(o[k], g[k][:nb]), (i, k)
         rows = min(no, rw.octree_capacity(n))
         for k in ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"):
-            assert np.array_equal(o[k][:rows].view(np.uint8), g[k][:rows].view(np.uint8)), (i, k)
+            assert len(o[k]) == rows, (i, k)
+            assert np.array_equal(o[k].view(np.uint8), g[k][:rows].view(np.uint8)), (i, k)
     srv.close()
 
 
@@ -63,8 +68,13 @@ def test_oversized_request_is_rejected_and_server_continues(gpu):
         srv.mix([big])
     assert "does not fit" in str(e.value)
     small = [rw.uniform_points(1000, 2), rw.uniform_points(3000, 3)]
-    res = srv.mix(small)
-    assert res[0][0] == rw.build_octree(small[0])[0]
+    res = srv.mix(small, sections=rw.SECTIONS_OCTREE)
+    head, arrays = rw.unpack_result(res[0][0])
+    assert (head["n_unique"], head["n_brt"], head["n_octree_nodes"]) == rw.build_octree(small[0])[0]
+    assert set(arrays) == {"children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"}
+    # a section bit outside the format fails the request through its future
+    with pytest.raises(rw.RwhError):
+        srv.mix(small, sections=1 << 14)
     srv.close()
 
 

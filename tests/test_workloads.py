@@ -56,9 +56,12 @@ def test_served_workload_matches_oracle(gpu, orc, big_pool):
         res = srv.mix(clouds)
     finally:
         srv.close()
-    assert max(b for _, _, b in res) > 1  # requests were coalesced into batches
+    assert max(b for _, b in res) > 1  # requests were coalesced into batches
     for i, c in enumerate(clouds):
-        (nu, nb, no), g, _ = res[i]
+        blob, _ = res[i]
+        assert rw.result_check(blob) == rw.RWH_OK, i
+        head, g = rw.unpack_result(blob)
+        nu, nb, no = head["n_unique"], head["n_brt"], head["n_octree_nodes"]
         (onu, onb, ono, rc), o, _ = orc.build(big_pool, c)
         assert rc == 0, i
         compare(rw, c.shape[0], (nu, nb, no), dict(g, morton_keys=g["unique"]), (onu, onb, ono), o,
