@@ -122,20 +122,77 @@ def live_arrays(o, n, nu, nb, rows, unique_key="morton_keys"):
     return out
 
 
-def verify_build(rw, orc, pool, tb, pts, counts):
-    """Every output array of the build left in tb against the oracle's build
-    of the same points: (ok, list of mismatching arrays)."""
+def verify_build(rw, orc, pool, ws, tb, pts, counts, stream):
+    """Every output array of the build left in tb, read back as one packed
+    result (rwh_pack_result, one transfer), against the oracle's build of the
+    same points: (ok, list of mismatching arrays)."""
     n = pts.shape[0]
+    blob = rw.pack_result(tb, ws.counts_ptr, 1, n, stream=stream.ptr)
+    head, g = rw.unpack_result(blob)
+    got = (head["n_unique"], head["n_brt"], head["n_octree_nodes"])
+    if got != tuple(counts):
+        return False, [f"packed counts {got} != read counts {tuple(counts)}"]
     nu, nb, no = counts
     (onu, onb, ono, rc), o, _ = orc.build(pool, pts)
     if (onu, onb, ono) != (nu, nb, no):
         return False, [f"counts {counts} != oracle {(onu, onb, ono)}"]
-    rows = min(max(no, 1), rw.octree_capacity(n))
-    g = live_arrays(tb.read(n, rows), n, nu, nb, rows)
+    bad = [] if rw.result_check(blob) == rw.RWH_OK else ["packed result check"]
+    rows = min(no, rw.octree_capacity(n))
     want = live_arrays(o, n, nu, nb, rows, unique_key="unique")
-    bad = [f for f in FIELDS if not np.array_equal(np.ascontiguousarray(g[f]).view(np.uint8),
-                                                   np.ascontiguousarray(want[f]).view(np.uint8))]
+    g = dict(g, morton_keys=g["unique"])
+    bad += [f for f in FIELDS if not np.array_equal(np.ascontiguousarray(g[f]).view(np.uint8),
+                                                    np.ascontiguousarray(want[f]).view(np.uint8))]
     return not bad, bad
+
+
+def export_section(rw, ws, tb, n, stream, reps=5):
+    """The device-to-host result path of the last build: the packing kernel
+    (bytes read + written over its event time), the one transfer of the
+    packed result into pinned host memory, and, for comparison, the 14 live
+    arrays copied one by one into pinned memory."""
+    L = rw.lib()
+    cap = tb.struct.oct_capacity
+    dev = rw.DeviceArray(rw.result_bytes(n, n, cap, cap), np.uint8)
+
+    def pack():
+        rw.check(L.rwh_pack_result(tb.ptr(), ws.counts_ptr, 1, n, rw.SECTIONS_ALL, dev.ptr, dev.nbytes,
+                                   stream.ptr), "rwh_pack_result")
+
+    def timed(fn):
+        fn()
+        e0, e1 = rw.Event(), rw.Event()
+        e0.record(stream.ptr)
+        for _ in range(reps):
+            fn()
+        e1.record(stream.ptr)
+        e1.sync()
+        return e0.elapsed_ms(e1) / reps
+
+    pack_ms = timed(pack)
+    head = dev.numpy(rw.RESULT_HEADER_BYTES, stream.ptr)
+    total = int(head[48:56].view(np.uint64)[0])
+    host_buf = rw.PinnedArray((total,), np.uint8)
+    d2h_ms = timed(lambda: rw.check(L.rwh_memcpy_d2h(host_buf.ptr, dev.ptr, total, stream.ptr), "d2h"))
+    head, arrays = rw.unpack_result(host_buf.array)
+    ok = rw.result_check(host_buf.array) == rw.RWH_OK
+    # the same bytes as 14 separate copies of the live arrays
+    pieces = []
+    for s, (name, dt, width) in enumerate(rw.RESULT_SECTIONS):
+        src_name = {"unique": "morton_keys"}.get(name, name)
+        pieces.append((tb[src_name].ptr, head["offset"][s], head["count"][s] * head["entry_bytes"][s]))
+
+    def separate():
+        for src, off, nbytes in pieces:
+            if nbytes:
+                rw.check(L.rwh_memcpy_d2h(host_buf.ptr + off, src, nbytes, stream.ptr), "d2h")
+
+    sep_ms = timed(separate)
+    live = sum(nb for _, _, nb in pieces)
+    return {"sections": "all 14", "total_bytes": total, "live_bytes": live,
+            "pack_ms": round(pack_ms, 4), "pack_GBs": round(2 * live / (pack_ms * 1e-3) / 1e9, 1),
+            "d2h_pinned_ms": round(d2h_ms, 3), "d2h_GBs": round(total / (d2h_ms * 1e-3) / 1e9, 1),
+            "pack_plus_d2h_ms": round(pack_ms + d2h_ms, 3),
+            "fourteen_copies_ms": round(sep_ms, 3), "result_check_ok": ok}
 
 
 def timed_builds(rw, ws, tb, n, stream, steps, warmup):
@@ -193,7 +250,7 @@ def clustered_section(rw, orc, pool, steps, warmup):
         stream = rw.Stream()
         ms = timed_builds(rw, ws, tb, n, stream, steps, warmup)
         counts = read_counts(rw, ws, cap, stream)
-        ok, bad = verify_build(rw, orc, pool, tb, pts, counts)
+        ok, bad = verify_build(rw, orc, pool, ws, tb, pts, counts, stream)
         out[f"2^{log2n}"] = {"points": n, "ms": round(ms, 4), "mpts_per_s": round(n / (ms * 1e-3) / 1e6, 1),
                              "n_unique": counts[0], "n_octree_nodes": counts[2],
                              "sort_config": config_name(rw, variant, plan),
@@ -423,7 +480,7 @@ def main():
     if not args.no_verify:
         progress("checking the last build against the oracle")
         orc = ge.load_oracle()
-        verified, mismatches = verify_build(rw, orc, orc.Pool(threads), tb, pts, (nu, nb, no))
+        verified, mismatches = verify_build(rw, orc, orc.Pool(threads), ws, tb, pts, (nu, nb, no), stream)
 
     # per-stage device time: events on the build stream around each stage
     progress("per-stage times")
@@ -435,6 +492,9 @@ def main():
         acc += [evs[i].elapsed_ms(evs[i + 1]) for i in range(7)]
     stage_ms = acc / args.stage_reps
     copy_gbs = copy_rate(rw, stream)
+    progress("result export (packed result, one transfer)")
+    export = export_section(rw, ws, tb, n, stream)
+    export["pack_frac_of_copy"] = round(export["pack_GBs"] / copy_gbs, 3)
     if args.calib:
         a = rw.DeviceArray(64 << 20, np.uint32, zero=True)
         b = rw.DeviceArray(64 << 20, np.uint32, zero=True)
@@ -479,6 +539,7 @@ def main():
                           "traffic": profile_figure(args.log2n, "sort_hbm_bytes_per_call")},
         "copy_GBs": round(copy_gbs, 1),
         "stages": stages,
+        "result_export": export,
         "sort_config": config_name(rw, variant, plan),
         "sort_config_ms": {config_name(rw, v, p): round(ms, 4) for v, p, ms in config_ms},
         "device": rw.device_name(),
