@@ -86,7 +86,7 @@ def enqueue_stages(rw, ws, tb, n, stream, events=None):
                                  s.has_leaf_right, s.left_child, s.parents, stream),
         lambda: L.rwh_edge_count(s.prefix_n, s.parents, nu_p, n, s.edge_count, stream),
         lambda: L.rwh_prefix_sum(ws.ptr, s.edge_count, nu_p, n, s.edge_offset, no_p, stream),
-        lambda: L.rwh_octree(tb.ptr(), nu_p, no_p, n, 0.0, 1024.0, ws.status_ptr(), stream),
+        lambda: L.rwh_octree(ws.ptr, tb.ptr(), nu_p, no_p, n, 0.0, 1024.0, ws.status_ptr(), stream),
     ]
     for i, c in enumerate(calls):
         if events is not None:

@@ -188,8 +188,11 @@ typedef struct rwh_tree_buffers {
  * Octree nodes for brt nodes [1, n_unique-1); rows [0, min(n_oct, cap)) of
  * children / masks are rewritten, corner / cell_size of those slots are set.
  * d_status: a device word that gets RWH_STATUS_PULL OR-ed in when the radix
- * tree is malformed (NULL: not reported; the caller clears it). */
-int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
+ * tree is malformed (NULL: not reported; the caller clears it).
+ * ws (may be NULL; else of at least n_max points): scratch for the few link
+ * walks that leave a tile's window, which a second short kernel then runs;
+ * with NULL they run inside the node kernel (same output, slower). */
+int rwh_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
                const uint32_t* d_n_octree_nodes, uint32_t n_max, float min_coord, float range,
                uint32_t* d_status, rwh_stream_t stream);
 
@@ -209,7 +212,7 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
  * For callers that cannot allocate device memory: code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc.  The build of
  * rwh_build_octree, with its workspace in a static device array of this
- * library (about 19 MiB per GPU, held until the process ends), so at most
+ * library (about 20 MiB per GPU, held until the process ends), so at most
  * RWH_NOALLOC_MAX_POINTS points (larger n: RWH_ERR_CAPACITY, nothing
  * enqueued).  d_counts (device, 3 words) gets {n_unique, n_octree_nodes,
  * status word (RWH_STATUS_*)}; compare n_octree_nodes with bufs->oct_capacity.  Calls on one GPU share that

@@ -85,7 +85,7 @@ ABI = {
     "rwh_radix_tree": (_I, [_P, _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
     "rwh_edge_count": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rwh_prefix_sum": (_I, [_P, _P, _P, _U32, _P, _P, _P]),
-    "rwh_octree": (_I, [_P, _P, _P, _U32, _F, _F, _P, _P]),
+    "rwh_octree": (_I, [_P, _P, _P, _P, _U32, _F, _F, _P, _P]),
     "rwh_link_leaf": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rwh_build_octree": (_I, [_P, _P, _U32, _F, _F, _P]),
     "rwh_build_octree_noalloc": (_I, [_P, _U32, _F, _F, _P, _P]),

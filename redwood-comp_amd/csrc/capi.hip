@@ -3,6 +3,7 @@
 // enqueue the kernels of k_sort.hip, k_stages.hip and k_octree.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -98,6 +99,7 @@ SegArgs single_seg_dev(const uint32_t* d_count) {
 }  // namespace
 
 struct rwh_workspace {
+  uint32_t magic = 0;       // ws_magic(): the layout this library built it with
   uint32_t max_points = 0;  // index-space capacity
   uint32_t max_segs = 0;
   uint32_t max_chunks = 0;
@@ -117,6 +119,7 @@ struct rwh_workspace {
   uint32_t* tables = nullptr;  // seg_base, seg_n, oct_base, oct_cap [max_segs each], chunk_seg
   uint32_t* tree_list = nullptr;  // radix tree: edge-crossing nodes per tile
   uint32_t* tree_sums = nullptr;  // radix tree: edge counts summed per tile (build path)
+  uint2* far_item = nullptr;      // octree: far walks queued per tile (kOctFarCap each)
   uint32_t* h_stage = nullptr;  // pinned staging of tables
   uint32_t* h_counts = nullptr;  // pinned counts read-back
   hipEvent_t upload_done = nullptr;
@@ -139,13 +142,22 @@ struct rwh_workspace {
 
 namespace {
 
+// A fingerprint of rwh_workspace's layout.  Two builds of this library with
+// different layouts can be loaded in one process; a workspace one of them
+// made is refused by the other instead of being read at the wrong offsets.
+constexpr uint32_t ws_magic() {
+  return 0x52575300u ^ static_cast<uint32_t>(sizeof(rwh_workspace)) ^
+         (static_cast<uint32_t>(offsetof(rwh_workspace, tree_sums)) << 8) ^
+         (static_cast<uint32_t>(offsetof(rwh_workspace, far_item)) << 16);
+}
+
 constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Byte offsets of a workspace's areas inside its one device block, for
 // max_points index-space points and max_segs clouds.
 struct WsLayout {
   uint32_t max_chunks, max_sort_tiles, max_scan_tiles;
-  size_t tmp, hist, ds, lbs, tc, lbu, lbc, ctrl, cnt, tab, tl, ts;
+  size_t tmp, hist, ds, lbs, tc, lbu, lbc, ctrl, cnt, tab, tl, ts, fi;
   size_t bytes;
 };
 
@@ -173,12 +185,14 @@ constexpr WsLayout ws_layout(uint32_t max_points, uint32_t max_segs) {
   L.tab = take(sizeof(uint32_t) * (4ull * max_segs + L.max_chunks));
   L.tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
   L.ts = take(sizeof(uint32_t) * static_cast<size_t>(tree_tiles(mp)));
+  L.fi = take(sizeof(uint2) * static_cast<size_t>(oct_tiles(mp)) * kOctFarCap);
   L.bytes = off;
   return L;
 }
 
 // Points the workspace's areas into base (a block laid out by L).
 void bind_workspace(rwh_workspace* ws, char* base, const WsLayout& L, uint32_t max_points, uint32_t max_segs) {
+  ws->magic = ws_magic();
   ws->max_points = max_points;
   ws->max_segs = max_segs;
   ws->max_chunks = L.max_chunks;
@@ -197,6 +211,7 @@ void bind_workspace(rwh_workspace* ws, char* base, const WsLayout& L, uint32_t m
   ws->tables = reinterpret_cast<uint32_t*>(base + L.tab);
   ws->tree_list = reinterpret_cast<uint32_t*>(base + L.tl);
   ws->tree_sums = reinterpret_cast<uint32_t*>(base + L.ts);
+  ws->far_item = reinterpret_cast<uint2*>(base + L.fi);
 }
 
 // The allocation-free build's workspace: a static device array of the
@@ -250,6 +265,8 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
 
 int check_ws(const rwh_workspace* ws, uint32_t n) {
   if (!ws) return fail(RWH_ERR_INVALID_ARG, "null workspace");
+  if (ws->magic != ws_magic())
+    return fail(RWH_ERR_INVALID_ARG, "not a workspace of this build of the library");
   if (n > ws->max_points)
     return fail(RWH_ERR_CAPACITY, "n = " + std::to_string(n) + " exceeds the workspace capacity " +
                                       std::to_string(ws->max_points));
@@ -627,17 +644,19 @@ int rwh_prefix_sum(rwh_workspace* ws, const int32_t* d_edge_count, const uint32_
   return launched();
 }
 
-int rwh_octree(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
+int rwh_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
                const uint32_t* d_n_octree_nodes, uint32_t n_max, float min_coord, float range,
                uint32_t* d_status, rwh_stream_t stream) {
   if (injected(7)) return RWH_ERR_INJECTED;
   RWH_TRY(check_bufs(bufs));
   RWH_CHECK(d_n_unique && d_n_octree_nodes, "null count pointer");
   RWH_CHECK(n_max <= kMaxPoints, "n_max above 2^30");
+  if (ws) RWH_TRY(check_ws(ws, n_max));
   if (n_max == 0) return RWH_OK;
   OctArgs a = oct_args(bufs, min_coord, range);
   a.n_oct = d_n_octree_nodes;
   a.status = d_status;
+  if (ws) a.far_item = ws->far_item;
   launch_octree(a, single_seg_dev(d_n_unique), n_max, true, static_cast<hipStream_t>(stream));
   return launched();
 }
@@ -737,6 +756,7 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   OctArgs oa = oct_args(bufs, min_coord, range);
   oa.n_oct = d_no;
   oa.status = d_status;
+  oa.far_item = ws->far_item;
   if (injected(7)) return RWH_ERR_INJECTED;
   if (n > 0) launch_octree(oa, single_seg_dev(d_nu), n, false, st);
   return launched();
@@ -818,6 +838,7 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
   oa.oct_base = ws->oct_base();
   oa.oct_cap = ws->oct_cap();
   oa.status = ws->status();
+  oa.far_item = ws->far_item;
   launch_octree(oa, uni, n_index, false, st);
   return launched();
 }

@@ -30,6 +30,13 @@
 //     also pulls for node 0.
 // Child slots keep the largest top and the mask ORs all links (D6).  Slots
 // outside [0, min(n_oct, capacity)) are never written.
+//
+// A pull reads its nodes from the tile's LDS window.  The few walks that leave
+// the window (about one bottom in 500 on uniform points, at most a dozen per
+// tile) would make their whole wave wait on global memory while the
+// workgroup holds its LDS, so they are queued per tile instead and a second
+// kernel walks them from global memory, merging its links into the rows with
+// atomicMax / atomicOr (the same max and OR, in any order).
 #include "rw_device.hpp"
 #include "rw_internal.hpp"
 
@@ -54,22 +61,32 @@ __device__ __forceinline__ uint32_t level_shift(int level) {
   return static_cast<uint32_t>(kMortonBits - 3 * level) & 31u;
 }
 
-// One incoming link: chain c's top slot at its top level's octant (the
-// reference's set_child arguments, func_octree.hpp:96-105).
-__device__ __forceinline__ void link_into(int cnt, int off, uint32_t code, int pfx, int32_t (&row)[8],
-                                          int32_t& mask) {
-  const int32_t top = off + cnt - 1;
-  const int top_level = pfx / 3 - cnt + 1;
-  const uint32_t which = (code >> level_shift(top_level)) & 7u;
-#pragma unroll
-  for (uint32_t w = 0; w < 8; ++w) row[w] = (w == which) ? max(row[w], top) : row[w];
-  mask |= static_cast<int32_t>(1u << which);
+// A node's pull descriptor:
+//   count > 0:  x = top slot (offset + count - 1), y = kLinkBit | octant of
+//               the chain's top level (the reference's set_child arguments,
+//               func_octree.hpp:96-105);
+//   otherwise:  x = left_child, y = has_leaf_left | has_leaf_right << 1.
+constexpr uint32_t kLinkBit = 0x80000000u;
+
+__device__ __forceinline__ uint2 make_desc(int cnt, int off, uint32_t code, int pfx, int lc,
+                                           bool leaf_l, bool leaf_r) {
+  uint2 d;
+  if (cnt > 0) {
+    const int top_level = pfx / 3 - cnt + 1;
+    d.x = static_cast<uint32_t>(off + cnt - 1);
+    d.y = kLinkBit | ((code >> level_shift(top_level)) & 7u);
+  } else {
+    d.x = static_cast<uint32_t>(lc);
+    d.y = (leaf_l ? 1u : 0u) | (leaf_r ? 2u : 0u);
+  }
+  return d;
 }
 
-__device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c, int32_t (&row)[8],
-                                         int32_t& mask) {
-  link_into(a.edge_count[base + c], a.edge_offset[base + c], a.codes[base + c],
-            a.prefix_n[base + c], row, mask);
+// The descriptor of node gq from global memory: its seven fields are loaded
+// together, so a walk waits for one round trip per node.
+__device__ __forceinline__ uint2 desc_global(const OctArgs& a, uint32_t gq) {
+  return make_desc(a.edge_count[gq], a.edge_offset[gq], a.codes[gq], a.prefix_n[gq],
+                   a.left_child[gq], a.has_leaf_left[gq] != 0, a.has_leaf_right[gq] != 0);
 }
 
 // Steps a pull walk may take.  On a tree built from sorted unique codes a
@@ -79,35 +96,37 @@ __device__ __forceinline__ void add_link(const OctArgs& a, uint32_t base, int c,
 constexpr int kPullBudget = 1024;
 
 // Pulls every chain reached from the given roots through nodes of count 0
-// (depth-first, one node per step).  kCheck: nodes outside [0, n), a full
-// stack or the step budget set RWH_STATUS_PULL in flags (a tree handed in
-// through the C ABI); without it the walk trusts the tree (one built by the
-// radix-tree kernels, whose bounds it provably keeps).
+// (depth-first, one node per step, from global memory) into a register row:
+// child slots keep the largest top, mask ORs the octants.  kCheck: nodes
+// outside [0, n), a full stack or the step budget set RWH_STATUS_PULL in
+// flags (a tree handed in through the C ABI); without it the walk trusts the
+// tree (one built by the radix-tree kernels, whose bounds it provably keeps).
+// steps counts the steps of all walks of one slot against kPullBudget (in
+// and out).
 template <bool kCheck>
 __device__ __forceinline__ void pull_links(const OctArgs& a, uint32_t base, int n, int* stack, int sp,
-                                           int32_t (&row)[8], int32_t& mask, uint32_t& flags) {
+                                           int32_t (&row)[8], int32_t& mask, uint32_t& flags, int& steps) {
   constexpr int kStack = 16;
-  int steps = 0;
   for (; sp > 0 && steps < kPullBudget; ++steps) {
     const int q = stack[--sp];
     if (kCheck && (q < 0 || q >= n)) {
       flags |= RWH_STATUS_PULL;
       continue;
     }
-    if (a.edge_count[base + q] > 0) {
-      add_link(a, base, q, row, mask);
+    const uint2 d = desc_global(a, base + static_cast<uint32_t>(q));
+    if (d.y & kLinkBit) {
+      const uint32_t which = d.y & 7u;
+      const int32_t top = static_cast<int32_t>(d.x);
+#pragma unroll
+      for (uint32_t w = 0; w < 8; ++w) row[w] = (w == which) ? max(row[w], top) : row[w];
+      mask |= static_cast<int32_t>(1u << which);
       continue;
     }
-    const int lc = a.left_child[base + q];
-    if (kCheck) {
-      for (int side = 0; side < 2; ++side) {
-        if (side == 0 ? a.has_leaf_left[base + q] : a.has_leaf_right[base + q]) continue;
-        if (sp < kStack) stack[sp++] = lc + side;
-        else flags |= RWH_STATUS_PULL;
-      }
-    } else {
-      if (!a.has_leaf_left[base + q] && sp < kStack) stack[sp++] = lc;
-      if (!a.has_leaf_right[base + q] && sp < kStack) stack[sp++] = lc + 1;
+    const int lc = static_cast<int>(d.x);
+    for (int side = 0; side < 2; ++side) {
+      if ((d.y >> side) & 1u) continue;  // a leaf
+      if (sp < kStack) stack[sp++] = lc + side;
+      else flags |= RWH_STATUS_PULL;
     }
   }
   if (sp > 0) flags |= RWH_STATUS_PULL;
@@ -138,38 +157,19 @@ __device__ int full_root_split(const uint32_t* codes, int n) {
 // ---------------------------------------------------------------------------
 // The node kernel.  A workgroup owns kOctSpan consecutive radix nodes (a
 // "tile"; loader thread t reads the four nodes starting at 4t) and first builds, in
-// LDS, a pull descriptor for every node of a window reaching kOctHalo nodes
-// beyond the tile on either side:
-//   count > 0:  x = top slot (offset + count - 1), y = kLinkBit | octant of
-//               the chain's top level (the reference's set_child arguments,
-//               func_octree.hpp:96-105);
-//   otherwise:  x = left_child, y = has_leaf_left | has_leaf_right << 1.
-// All descendants of a node lie inside its key range, so the pull of a bottom
-// whose range fits the window never leaves LDS; the rest read the same
-// descriptor from global memory.  The node arrays are read with 16-byte
-// (int) and 4-byte (byte arrays) loads, four nodes per thread.
+// LDS, the pull descriptor (make_desc) of every node of a window reaching
+// kOctHalo nodes beyond the tile on either side.  All descendants of a node
+// lie inside its key range, so the pull of a bottom whose range fits the
+// window never leaves LDS; walks that do are queued for k_oct_far (or read
+// the same descriptors from global memory).  The node arrays are read with
+// 16-byte (int) and 4-byte (byte arrays) loads, four nodes per thread.
 // ---------------------------------------------------------------------------
 constexpr int kOctBlock = 512;                // threads per workgroup
 constexpr int kOctLoaders = 256;              // threads that read four nodes each
-constexpr int kOctSpan = kOctLoaders * 4;     // radix nodes per workgroup
+constexpr int kSpan = static_cast<int>(kOctSpan);
+static_assert(kOctLoaders * 4 == kSpan, "loader threads read four nodes each");
 constexpr int kOctHalo = 128;            // window margin on each side (multiple of 4)
-constexpr int kOctWin = kOctSpan + 2 * kOctHalo;
-constexpr uint32_t kLinkBit = 0x80000000u;
-static_assert(kChunk % kOctSpan == 0, "tiles must not straddle segments");
-
-__device__ __forceinline__ uint2 make_desc(int cnt, int off, uint32_t code, int pfx, int lc,
-                                           bool leaf_l, bool leaf_r) {
-  uint2 d;
-  if (cnt > 0) {
-    const int top_level = pfx / 3 - cnt + 1;
-    d.x = static_cast<uint32_t>(off + cnt - 1);
-    d.y = kLinkBit | ((code >> level_shift(top_level)) & 7u);
-  } else {
-    d.x = static_cast<uint32_t>(lc);
-    d.y = (leaf_l ? 1u : 0u) | (leaf_r ? 2u : 0u);
-  }
-  return d;
-}
+constexpr int kOctWin = kSpan + 2 * kOctHalo;
 
 // Four consecutive radix nodes q0 .. q0+3 of a segment (q0 a multiple of 4);
 // nodes at or beyond n read as count 0.
@@ -226,11 +226,6 @@ struct Window {
 
 // ---- rare paths ----------------------------------------------------------------
 
-// Descriptor of a node outside the window.
-__device__ __forceinline__ uint2 desc_global(const OctArgs& a, uint32_t gq) {
-  return make_desc(a.edge_count[gq], a.edge_offset[gq], a.codes[gq], a.prefix_n[gq],
-                   a.left_child[gq], a.has_leaf_left[gq] != 0, a.has_leaf_right[gq] != 0);
-}
 
 // Merges a register row into this thread's LDS row (entry w at row[w *
 // kOctBlock]); returns the mask.
@@ -241,16 +236,21 @@ __device__ __forceinline__ int32_t merge_row(const int32_t (&r)[8], int32_t m, i
   return m;
 }
 
-// Depth-first pull from global memory below node q (count 0), for a walk
-// deeper than the pull stack (not reached below a node of count > 0).
+// The whole pull of a chain bottom from global memory, for the rare walk
+// that did not fit the LDS pull stack or the tile's queue of far walks (links
+// merged twice are harmless: max and OR).  Kept out of pull_pair's loop: a
+// global walk inside it, even one never taken, slows the common LDS walk.
 template <bool kCheck>
-__device__ __forceinline__ int32_t pull_overflow(const OctArgs& a, uint32_t base, int n, int q, int32_t* row,
-                                                 uint32_t& flags) {
+__device__ __forceinline__ int32_t pull_spill(const OctArgs& a, uint32_t base, int n, uint32_t pair, int32_t* row,
+                                           uint32_t& flags, int& steps) {
   int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int32_t m = 0;
   int stack[16];
-  stack[0] = q;
-  pull_links<kCheck>(a, base, n, stack, 1, r, m, flags);
+  int sp = 0;
+  const int c = static_cast<int>(pair & 0x3fffffffu);
+  for (int side = 0; side < 2; ++side)
+    if (!((pair >> (30 + side)) & 1u) && c + side < n) stack[sp++] = c + side;
+  pull_links<kCheck>(a, base, n, stack, sp, r, m, flags, steps);
   return merge_row(r, m, row);
 }
 
@@ -266,12 +266,23 @@ __device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, in
   int sp = 0;
   if (g != 0) stack[sp++] = g;
   if (g + 1 < n) stack[sp++] = g + 1;
-  pull_links<kCheck>(a, base, n, stack, sp, r, m, flags);
+  int steps = 0;
+  pull_links<kCheck>(a, base, n, stack, sp, r, m, flags, steps);
   return merge_row(r, m, row);
 }
 
 // ---- the pull ------------------------------------------------------------------
 constexpr int kPullStack = 4;
+
+// The tile's queue of far walks: the number taken (LDS) and the tile's
+// kOctFarCap items in the workspace (nullptr: none, every walk runs in
+// place).  An item is (bottom slot relative to the segment's octree base,
+// node relative to the segment); unused items hold kNoFarItem.
+struct FarQueue {
+  uint32_t* count;  // LDS
+  uint2* item;
+};
+constexpr uint32_t kNoFarItem = 0xffffffffu;
 
 // Pull of a chain bottom whose radix node has the children pair `pair` (left
 // child | has_leaf_left << 30 | has_leaf_right << 31): depth-first through
@@ -280,14 +291,18 @@ constexpr int kPullStack = 4;
 // of count > 0 at most two nodes of count 0 follow each other on a path
 // (prefix lengths increase strictly and a count-0 node has its parent's
 // level), so at most three pairs wait on the stack; a deeper walk is still
-// handled, from global memory.
+// handled (pull_spill).  A node outside the window is queued with the
+// bottom's slot (slot_rel, relative to the segment's octree base) while the
+// tile's queue has room; otherwise the whole walk is redone by pull_spill.
 template <bool kCheck>
 __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, const Window& w,
                                              int n, uint32_t pair, int32_t* row, uint32_t* stk,
-                                             uint32_t& flags) {
+                                             uint32_t& flags, const FarQueue& far, uint32_t slot_rel) {
+  const uint32_t pair0 = pair;
   int32_t m = 0;
   int sp = 0;
   int steps = 0;
+  bool spill = false;
   for (; steps < kPullBudget; ++steps) {
     const int c = static_cast<int>(pair & 0x3fffffffu);
 #pragma unroll
@@ -295,21 +310,28 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
       const int q = c + side;
       if (((pair >> (30 + side)) & 1u) || q >= n) continue;  // a leaf (or no such node)
       uint2 d;
-      if (q >= w.w0 && q < w.w1) d = w.desc[q - w.w0];
-      else d = desc_global(a, base + static_cast<uint32_t>(q));
+      if (q >= w.w0 && q < w.w1) {
+        d = w.desc[q - w.w0];
+      } else {
+        const uint32_t idx = far.item ? atomicAdd(far.count, 1u) : kOctFarCap;
+        if (idx < kOctFarCap) far.item[idx] = make_uint2(slot_rel, static_cast<uint32_t>(q));
+        else spill = true;
+        continue;
+      }
       if (d.y & kLinkBit) {
         atomicMax(&row[(d.y & 7u) * kOctBlock], static_cast<int32_t>(d.x));
         m |= static_cast<int32_t>(1u << (d.y & 7u));
       } else if (sp < kPullStack) {
         stk[(sp++) * kOctBlock] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
       } else {
-        m |= pull_overflow<kCheck>(a, base, n, q, row, flags);
+        spill = true;
       }
     }
     if (sp == 0) break;
     pair = stk[(--sp) * kOctBlock];
   }
-  if (steps == kPullBudget) flags |= RWH_STATUS_PULL;  // a malformed tree: the walk was cut short
+  if (spill) m |= pull_spill<kCheck>(a, base, n, pair0, row, flags, steps);
+  if (steps >= kPullBudget) flags |= RWH_STATUS_PULL;  // a malformed tree: the walk was cut short
   return m;
 }
 
@@ -329,26 +351,29 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
 template <bool kCheck>
 __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
   __shared__ uint2 s_desc[kOctWin];
-  __shared__ __align__(16) int32_t s_off[kOctSpan];
-  __shared__ __align__(16) uint32_t s_code[kOctSpan];
-  __shared__ __align__(16) uint32_t s_lcf[kOctSpan];  // left_child | leaf_l << 30 | leaf_r << 31
-  __shared__ uint32_t s_lvl[kOctSpan / 4];  // prefix_n / 3, one byte per node
+  __shared__ __align__(16) int32_t s_off[kSpan];
+  __shared__ __align__(16) uint32_t s_code[kSpan];
+  __shared__ __align__(16) uint32_t s_lcf[kSpan];  // left_child | leaf_l << 30 | leaf_r << 31
+  __shared__ uint32_t s_lvl[kSpan / 4];  // prefix_n / 3, one byte per node
   // each thread's pulled row and pull stack, entry-major (entry e of thread t
   // at [e * kOctBlock + t]) so that a wave's accesses fall in distinct banks
   __shared__ int32_t s_row[8 * kOctBlock];
   __shared__ uint32_t s_stk[kPullStack * kOctBlock];
   __shared__ float s_cell[32];  // range / 2^k
   __shared__ int64_t s_end;
+  __shared__ uint32_t s_nfar;  // far walks queued by this tile
 
   // neighbouring tiles share window margins: XCD-grouped order
   const uint32_t tile = xcd_tile();
   if (tile >= n_tiles) return;
   const uint32_t first = tile * kOctSpan;
   const Seg sg = seg_of(seg, first);
-  if (first < sg.base) return;
   const int b = static_cast<int>(first - sg.base);
   const int n = static_cast<int>(sg.count);  // n_brt
-  if (b >= n) return;  // a segment's padding (uniform across the workgroup)
+  uint2* far_items = a.far_item ? a.far_item + static_cast<size_t>(tile) * kOctFarCap : nullptr;
+  // the tile's queue starts empty (k_oct_far skips unused items)
+  if (far_items && threadIdx.x < kOctFarCap) far_items[threadIdx.x] = make_uint2(kNoFarItem, 0u);
+  if (first < sg.base || b >= n) return;  // a segment's padding (uniform across the workgroup)
   const uint32_t tid = threadIdx.x;
   const OctSeg os = oct_seg(a, sg);
   const int64_t rows = os.rows;
@@ -358,22 +383,23 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   float* cell = a.cell_size + os.base;
   int32_t* mask = a.child_node_mask + os.base;
   int32_t* leaf_mask = a.child_leaf_mask + os.base;
-  const int nv = min(kOctSpan, n - b);  // valid nodes of the tile
+  const int nv = min(kSpan, n - b);  // valid nodes of the tile
   uint32_t flags = 0;  // RWH_STATUS_* bits of this thread's walks
 
   Window w;
   w.desc = s_desc;
   w.w0 = max(b - kOctHalo, 0);
-  w.w1 = min(b + kOctSpan + kOctHalo, n);
+  w.w1 = min(b + kSpan + kOctHalo, n);
 
   // 1. the tile's nodes (four per loader thread) and the window's
   // descriptors (the margins read by the other threads meanwhile)
   if (tid < 32) s_cell[tid] = __fdiv_rn(a.range, static_cast<float>(1u << tid));
+  if (tid == 0) s_nfar = 0;
   constexpr int kHaloGroups = kOctHalo / 4;
   static_assert(kOctBlock - kOctLoaders >= 2 * kHaloGroups, "one thread per margin group");
   const int ht = static_cast<int>(tid) - kOctLoaders;
   if (ht >= 0 && ht < 2 * kHaloGroups) {
-    const int h = (ht < kHaloGroups) ? b - kOctHalo + 4 * ht : b + kOctSpan + 4 * (ht - kHaloGroups);
+    const int h = (ht < kHaloGroups) ? b - kOctHalo + 4 * ht : b + kSpan + 4 * (ht - kHaloGroups);
     if (h >= w.w0 && h < w.w1) {
       const Nodes4 hg = load_nodes4(a, sg.base, h, n);
 #pragma unroll
@@ -420,12 +446,13 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   // 2. the slots, one per thread
   int32_t* row = s_row + tid;
   uint32_t* stk = s_stk + tid;
+  const FarQueue far{&s_nfar, far_items};
   const int64_t lo = max<int64_t>(s_off[0], 0);
   const int64_t hi = min<int64_t>(s_end, rows);
   for (int64_t slot = lo + tid; slot < hi; slot += kOctBlock) {
     int k = 0;  // last node with offset <= slot
 #pragma unroll
-    for (int step = kOctSpan / 2; step >= 1; step >>= 1)
+    for (int step = kSpan / 2; step >= 1; step >>= 1)
       if (k + step < nv && s_off[k + step] <= slot) k += step;
     if (b + k == 0) continue;  // node 0 owns no chain (the reference starts at 1)
     const int j = static_cast<int>(slot - s_off[k]);
@@ -446,7 +473,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     } else {  // the bottom: pull the chains that link here
 #pragma unroll
       for (int e = 0; e < 8; ++e) row[e * kOctBlock] = 0;
-      m = pull_pair<kCheck>(a, sg.base, w, n, s_lcf[k], row, stk, flags);
+      m = pull_pair<kCheck>(a, sg.base, w, n, s_lcf[k], row, stk, flags, far, static_cast<uint32_t>(slot));
       if (slot == 0) m |= pull_root<kCheck>(a, sg.base, n, row, flags);
       r0 = make_int4(row[0], row[kOctBlock], row[2 * kOctBlock], row[3 * kOctBlock]);
       r1 = make_int4(row[4 * kOctBlock], row[5 * kOctBlock], row[6 * kOctBlock], row[7 * kOctBlock]);
@@ -459,14 +486,98 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
   if (kCheck && flags && a.status) atomicOr(&a.status[sg.id], flags);
 }
 
+// The queued far walks: thread k of a tile's group of kOctFarCap walks the
+// tile's item k (if queued) from global memory and merges its links into the
+// bottom's row: atomicMax per child slot and atomicOr into the node mask,
+// over the row k_oct_nodes wrote.  The walk goes level by level, the loads of
+// a level's nodes issued together: below a node of count > 0 at most two
+// nodes of count 0 follow each other on a path, so on a tree built from
+// sorted unique codes three levels (one round trip each) reach every link.
+// What a malformed tree leaves after them (more levels, a wider frontier)
+// continues depth-first with pull_links' bounds and checks.
+constexpr int kFarBlock = 256;
+static_assert(kFarBlock % kOctFarCap == 0, "a workgroup covers whole tiles");
+constexpr int kFarLevels = 3;
+constexpr int kFarWidth = 8;  // frontier nodes per level
+
+template <bool kCheck>
+__global__ __launch_bounds__(kFarBlock) void k_oct_far(OctArgs a, SegArgs seg, uint32_t n_tiles) {
+  const uint32_t gid = blockIdx.x * kFarBlock + threadIdx.x;
+  const uint32_t tile = gid / kOctFarCap;
+  if (tile >= n_tiles) return;
+  const uint2 it = a.far_item[gid];
+  if (it.x == kNoFarItem) return;
+  const Seg sg = seg_of(seg, tile * kOctSpan);
+  const OctSeg os = oct_seg(a, sg);
+  const int n = static_cast<int>(sg.count);
+  int32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t m = 0;
+  uint32_t flags = 0;
+  int stack[16];  // the depth-first rest
+  int sp = 0;
+  int fr[kFarWidth];
+  int nf = 1;
+  fr[0] = static_cast<int>(it.y);
+  int steps = 0;
+  for (int level = 0; level < kFarLevels && nf > 0; ++level) {
+    uint2 d[kFarWidth];
+#pragma unroll
+    for (int i = 0; i < kFarWidth; ++i) {
+      const bool in = i < nf && static_cast<uint32_t>(fr[i]) < static_cast<uint32_t>(n);
+      if (kCheck && i < nf && !in) flags |= RWH_STATUS_PULL;
+      d[i] = in ? desc_global(a, sg.base + static_cast<uint32_t>(fr[i])) : make_uint2(0u, 3u);
+    }
+    steps += nf;
+    int nx[kFarWidth];
+    int nn = 0;
+#pragma unroll
+    for (int i = 0; i < kFarWidth; ++i) {
+      if (i >= nf) continue;
+      if (d[i].y & kLinkBit) {
+        const uint32_t which = d[i].y & 7u;
+        const int32_t top = static_cast<int32_t>(d[i].x);
+#pragma unroll
+        for (uint32_t w = 0; w < 8; ++w) r[w] = (w == which) ? max(r[w], top) : r[w];
+        m |= static_cast<int32_t>(1u << which);
+        continue;
+      }
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        if ((d[i].y >> side) & 1u) continue;  // a leaf (or a node skipped above)
+        const int c = static_cast<int>(d[i].x) + side;
+        if (nn < kFarWidth) nx[nn++] = c;
+        else if (sp < 16) stack[sp++] = c;
+        else flags |= RWH_STATUS_PULL;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kFarWidth; ++i) fr[i] = nx[i];
+    nf = nn;
+  }
+  for (int i = 0; i < nf && sp < 16; ++i) stack[sp++] = fr[i];
+  if (sp > 0) pull_links<kCheck>(a, sg.base, n, stack, sp, r, m, flags, steps);
+  const size_t slot = static_cast<size_t>(os.base) + it.x;
+#pragma unroll
+  for (int w = 0; w < 8; ++w)
+    if ((m >> w) & 1) atomicMax(&a.children[slot * 8 + w], r[w]);
+  if (m) atomicOr(&a.child_node_mask[slot], m);
+  if (kCheck && flags && a.status) atomicOr(&a.status[sg.id], flags);
+}
+
 void launch_octree(const OctArgs& a, SegArgs seg, uint32_t n_index, bool check, hipStream_t stream) {
-  const uint32_t n_tiles = (n_index + kOctSpan - 1) / kOctSpan;
+  const uint32_t n_tiles = oct_tiles(n_index);
   if (n_tiles == 0) return;
   seg.count_sub = 1;
-  if (check)
+  const uint32_t far_grid = (n_tiles * kOctFarCap + kFarBlock - 1) / kFarBlock;
+  if (check) {
     hipLaunchKernelGGL(k_oct_nodes<true>, dim3(xcd_grid(n_tiles)), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
-  else
+    if (a.far_item)
+      hipLaunchKernelGGL(k_oct_far<true>, dim3(far_grid), dim3(kFarBlock), 0, stream, a, seg, n_tiles);
+  } else {
     hipLaunchKernelGGL(k_oct_nodes<false>, dim3(xcd_grid(n_tiles)), dim3(kOctBlock), 0, stream, a, seg, n_tiles);
+    if (a.far_item)
+      hipLaunchKernelGGL(k_oct_far<false>, dim3(far_grid), dim3(kFarBlock), 0, stream, a, seg, n_tiles);
+  }
 }
 
 // Leaf linking (opt-in): each leaf child of brt node i goes into the bottom

@@ -228,6 +228,14 @@ void launch_prefix_sum_tiles(const int32_t* in, int32_t* out, SegArgs seg, uint3
                              uint32_t* total, const uint32_t* tile_sum, hipStream_t stream);
 constexpr uint32_t tree_tiles(uint32_t n_index) { return (n_index + kTreeTile - 1) / kTreeTile; }
 
+// Octree nodes (S7): tiles of radix nodes.  A tile's pull walks that leave
+// its LDS window are queued per tile (at most kOctFarCap, the rest walked in
+// place) and walked from global memory by a second, short kernel.
+constexpr uint32_t kOctSpan = 1024;   // radix nodes per tile
+constexpr uint32_t kOctFarCap = 16;   // queued walks per tile
+static_assert(kChunk % kOctSpan == 0, "octree tiles must not straddle segments");
+constexpr uint32_t oct_tiles(uint32_t n_index) { return (n_index + kOctSpan - 1) / kOctSpan; }
+
 struct OctArgs {
   const int32_t* edge_offset;
   const int32_t* edge_count;
@@ -248,6 +256,10 @@ struct OctArgs {
   int32_t* child_node_mask;
   int32_t* child_leaf_mask;
   uint32_t* status;  // per segment, or nullptr: RWH_STATUS_* bits OR-ed in
+  // queued far walks: kOctFarCap items per tile (bottom slot relative to
+  // the segment's octree base, node relative to the segment); nullptr:
+  // every walk runs in place
+  uint2* far_item;
 };
 // check: a tree handed in through the C ABI (child indices checked, walk
 // bounds reported in a.status); the whole builds pass the tree their own

@@ -314,8 +314,8 @@ std::future<void> run_stage7(AppData& a) {
       a, "run_stage7",
       [&](Engine& e, DeviceState& s) {
         const rwh_tree_buffers b = buffers_of(a);
-        int st = rwh_octree(&b, d_n_unique(s), d_n_oct(s), a.n_input, AppData::min_coord, AppData::range,
-                            d_status(s), e.stream());
+        int st = rwh_octree(s.ws, &b, d_n_unique(s), d_n_oct(s), a.n_input, AppData::min_coord,
+                            AppData::range, d_status(s), e.stream());
         if (st == RWH_OK) st = fetch_status(e, s);
         return st;
       },

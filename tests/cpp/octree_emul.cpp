@@ -7,13 +7,15 @@
 //
 //   octree_emul <in.bin> <out.bin>
 // in.bin:  u32 n_unique, n_octree_nodes, capacity, mode (bit 0: run leaf
-//          linking, bit 1: the kernel without checks); f32 min, range;
+//          linking, bit 1: the kernel without checks, bit 2: queue the
+//          walks that leave a tile's window for k_oct_far, as a whole build
+//          does); f32 min, range;
 //          codes (u32 x n_unique); prefix_n, has_leaf_left, has_leaf_right
 //          (u8 x n_brt); left_child, parents, edge_count, edge_offset
 //          (i32 x n_brt)
 // out.bin: u32 status; children (i32 x 8 x capacity), corner (f32 x 4 x
 //          capacity), cell_size (f32), child_node_mask, child_leaf_mask
-//          (i32 x capacity)
+//          (i32 x capacity); u32 walks queued for k_oct_far (all tiles)
 #include "k_octree.hip"
 
 #include <cstdio>
@@ -43,6 +45,7 @@ int main(int argc, char** argv) {
   if (std::fread(hdr, 4, 4, f) != 4 || std::fread(box, 4, 2, f) != 2) return 2;
   const uint32_t nu = hdr[0], n_oct = hdr[1], cap = hdr[2], leaf = hdr[3] & 1u;
   const bool check = (hdr[3] & 2u) == 0;
+  const bool queue = (hdr[3] & 4u) != 0;
   const uint32_t nb = nu - 1;
   auto* codes = read_array<uint32_t>(f, nu);
   auto* prefix_n = read_array<uint8_t>(f, nb);
@@ -81,6 +84,10 @@ int main(int argc, char** argv) {
   a.child_node_mask = mask;
   a.child_leaf_mask = leaf_mask;
   a.status = status;
+  const uint32_t n_tiles = rw::oct_tiles(nu);
+  const size_t n_items = static_cast<size_t>(n_tiles) * rw::kOctFarCap;
+  auto* far_item = new uint2[n_items == 0 ? 1 : n_items]();
+  if (queue) a.far_item = far_item;
   rw::SegArgs seg{};
   seg.n_segs = 1;
   seg.count = d_nu;
@@ -95,6 +102,9 @@ int main(int argc, char** argv) {
   std::fwrite(cell, 4, cap, o);
   std::fwrite(mask, 4, cap, o);
   std::fwrite(leaf_mask, 4, cap, o);
+  uint32_t queued = 0;
+  for (size_t i = 0; queue && i < n_items; ++i) queued += far_item[i].x != 0xffffffffu;
+  std::fwrite(&queued, 4, 1, o);
   std::fclose(o);
   delete[] codes;
   delete[] prefix_n;
@@ -112,5 +122,6 @@ int main(int argc, char** argv) {
   delete[] status;
   delete[] d_nu;
   delete[] d_no;
+  delete[] far_item;
   return 0;
 }

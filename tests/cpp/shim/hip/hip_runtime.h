@@ -53,6 +53,7 @@ struct int2 { int32_t x, y; };
 struct uint4 { uint32_t x, y, z, w; };
 struct int4 { int32_t x, y, z, w; };
 struct float4 { float x, y, z, w; };
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return {x, y}; }
 inline int4 make_int4(int32_t x, int32_t y, int32_t z, int32_t w) { return {x, y, z, w}; }
 inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return {x, y, z, w}; }
 inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }

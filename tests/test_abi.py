@@ -46,9 +46,14 @@ def test_argument_errors_need_no_gpu(rw):
     assert L.rwh_morton(None, (1 << 30) + 1, 0.0, 1024.0, None, None) == rw.RWH_ERR_INVALID_ARG
     assert L.rwh_sort_u32(None, None, None, 5, 30, None) == rw.RWH_ERR_INVALID_ARG
     assert b"null workspace" in L.rwh_last_error()
+    # a block that is not a workspace of this build (another build's layout,
+    # or no workspace at all) is refused before anything reads its pointers
+    forged = np.zeros(512, np.uint8)
+    assert L.rwh_sort_u32(forged.ctypes.data, None, None, 5, 30, None) == rw.RWH_ERR_INVALID_ARG
+    assert b"not a workspace of this build" in L.rwh_last_error()
     assert L.rwh_radix_tree(None, None, None, (1 << 30) + 1, None, None, None, None, None, None) == \
         rw.RWH_ERR_INVALID_ARG
-    assert L.rwh_octree(None, None, None, 10, 0.0, 1024.0, None, None) == rw.RWH_ERR_INVALID_ARG
+    assert L.rwh_octree(None, None, None, None, 10, 0.0, 1024.0, None, None) == rw.RWH_ERR_INVALID_ARG
     assert L.rwh_error_string(rw.RWH_ERR_CAPACITY) == b"capacity exceeded"
     counts = np.zeros(4, np.uint32)
     n = np.array([1, 2], np.uint32)

@@ -89,20 +89,31 @@ def test_radix_tree_reports_keys_out_of_order(gpu, kind):
 def malformed_tree(rw, orc, pool, how):
     """The oracle's radix tree, edge counts and offsets of a small cloud with
     one node of edge count 0 (below a contributing parent) rewired: its left
-    child points at itself ("cycle") or past the tree ("outside")."""
+    child points at itself ("cycle"), past the tree ("outside"), or at a node
+    of count 0 far outside the node's tile, whose own left child is itself
+    ("far_cycle": the walk leaves the tile's window before it cycles)."""
     pts = rw.uniform_points(6000, 77)
     (nu, nb, no, rc), o, _ = orc.build(pool, pts)
     cnt, par = o["edge_count"][:nb], o["parents"][:nb]
     q = next(i for i in range(1, nb) if cnt[i] == 0 and cnt[par[i]] > 0)
-    o["left_child"][q] = q if how == "cycle" else nb + 100
+    if how == "far_cycle":
+        r = next(i for i in range(nb - 2, 0, -1) if cnt[i] == 0)
+        assert r - q > 2048
+        o["left_child"][q] = r
+        o["left_child"][r] = r
+        o["has_leaf_left"][r] = 0
+        o["has_leaf_right"][r] = 0
+    else:
+        o["left_child"][q] = q if how == "cycle" else nb + 100
     o["has_leaf_left"][q] = 0
     o["has_leaf_right"][q] = 1
     return nu, nb, no, o
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("how", ["cycle", "outside"])
-def test_octree_reports_a_malformed_tree(gpu, orc, pool, how):
+@pytest.mark.parametrize("with_ws", [False, True])
+@pytest.mark.parametrize("how", ["cycle", "outside", "far_cycle"])
+def test_octree_reports_a_malformed_tree(gpu, orc, pool, how, with_ws):
     rw = gpu
     L = rw.lib()
     nu, nb, no, o = malformed_tree(rw, orc, pool, how)
@@ -115,6 +126,9 @@ def test_octree_reports_a_malformed_tree(gpu, orc, pool, how):
     d_nu = rw.DeviceArray.from_numpy(np.array([nu], np.uint32))
     d_no = rw.DeviceArray.from_numpy(np.array([no], np.uint32))
     d_status = rw.DeviceArray(1, np.uint32, zero=True)
-    print(f"case malformed tree {how} n_brt={nb}", flush=True)
-    rw.check(L.rwh_octree(tb.ptr(), d_nu.ptr, d_no.ptr, nu, 0.0, 1024.0, d_status.ptr, None))
+    # with a workspace, walks that leave a tile's window run in a second kernel
+    ws = rw.Workspace(nu) if with_ws else None
+    print(f"case malformed tree {how} n_brt={nb} workspace={with_ws}", flush=True)
+    rw.check(L.rwh_octree(ws.ptr if ws else None, tb.ptr(), d_nu.ptr, d_no.ptr, nu, 0.0, 1024.0,
+                          d_status.ptr, None))
     assert int(d_status.numpy()[0]) & rw.STATUS_PULL

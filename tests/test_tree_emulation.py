@@ -140,14 +140,19 @@ NODE_FIELDS = (("prefix_n", np.uint8), ("has_leaf_left", np.uint8), ("has_leaf_r
                ("edge_offset", np.int32))
 
 
-def run_octree(o, nu, no, cap, tmp_path, link_leaf=False, check=True, min_coord=0.0, range_=1024.0):
+def run_octree(o, nu, no, cap, tmp_path, link_leaf=False, check=True, min_coord=0.0, range_=1024.0,
+               queue=False):
+    """queue: walks that leave a tile's LDS window go to the per-tile queue and
+    the second kernel (k_oct_far), as in the whole builds; otherwise the node
+    kernel walks them itself."""
     if not os.path.exists(OCT_EMU):
         subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "cpp"), "all"], check=True,
                        capture_output=True)
     nb = nu - 1
     ip, op = tmp_path / "tree.bin", tmp_path / "oct.bin"
     with open(ip, "wb") as f:
-        f.write(np.array([nu, no, cap, int(link_leaf) | (0 if check else 2)], np.uint32).tobytes())
+        f.write(np.array([nu, no, cap, int(link_leaf) | (0 if check else 2) | (4 if queue else 0)],
+                         np.uint32).tobytes())
         f.write(np.array([min_coord, range_], np.float32).tobytes())
         f.write(np.ascontiguousarray(o["unique"][:nu], np.uint32).tobytes())
         for name, dt in NODE_FIELDS:
@@ -163,6 +168,7 @@ def run_octree(o, nu, no, cap, tmp_path, link_leaf=False, check=True, min_coord=
         a = np.frombuffer(b[o_:o_ + cnt * np.dtype(dt).itemsize], dt)
         out[name] = a.reshape(cap, w) if w > 1 else a
         o_ += cnt * np.dtype(dt).itemsize
+    out["queued"] = int(np.frombuffer(b[o_:o_ + 4], np.uint32)[0])
     return out
 
 
@@ -179,25 +185,36 @@ def octree_cases(rw):
         yield kind, p
 
 
-@pytest.mark.parametrize("link_leaf,check", [(False, True), (False, False), (True, True)])
-def test_octree_kernels_on_cpu_match_oracle(rw, orc, pool, tmp_path, link_leaf, check):
+@pytest.mark.parametrize("link_leaf,check,queue", [(False, True, False), (False, False, False),
+                                                   (True, True, False), (False, False, True),
+                                                   (True, True, True)])
+def test_octree_kernels_on_cpu_match_oracle(rw, orc, pool, tmp_path, link_leaf, check, queue):
     """check: the kernel of the stage entry point (a tree from the caller);
-    without: the one the whole builds run on the tree they built."""
+    without: the one the whole builds run on the tree they built.  queue: far
+    walks run in the second kernel (the whole builds, and the stage entry
+    point given a workspace)."""
+    queued = 0
     for name, pts in octree_cases(rw):
         (nu, nb, no, rc), o, _ = orc.build(pool, pts)
         cap = rw.octree_capacity(pts.shape[0])
         assert rc == 0 and no <= cap, name
-        got = run_octree(o, nu, no, cap, tmp_path, link_leaf, check)
+        got = run_octree(o, nu, no, cap, tmp_path, link_leaf, check, queue=queue)
         assert got["status"] == 0, name
+        queued += got["queued"]
         if link_leaf:
             orc.link_leaf(pool, o, (nu, nb, no))
         for f in ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"):
             assert np.array_equal(got[f][:no].view(np.uint8), o[f][:no].view(np.uint8)), (name, f)
+    # the clouds have walks that leave a tile's window: the queue is exercised
+    assert (queued > 0) == queue
 
 
-@pytest.mark.parametrize("how", ["cycle", "outside"])
-def test_octree_kernel_on_cpu_reports_a_malformed_tree(rw, orc, pool, tmp_path, how):
+@pytest.mark.parametrize("queue", [False, True])
+@pytest.mark.parametrize("how", ["cycle", "outside", "far_cycle"])
+def test_octree_kernel_on_cpu_reports_a_malformed_tree(rw, orc, pool, tmp_path, how, queue):
     from test_status import malformed_tree
     nu, nb, no, o = malformed_tree(rw, orc, pool, how)
-    got = run_octree(o, nu, no, rw.octree_capacity(nu), tmp_path, link_leaf=True)
+    got = run_octree(o, nu, no, rw.octree_capacity(nu), tmp_path, link_leaf=True, queue=queue)
     assert got["status"] & 0x10  # RWH_STATUS_PULL; every walk ended, nothing read out of bounds
+    if queue and how == "far_cycle":
+        assert got["queued"] > 0  # the cycle was walked by the second kernel
