@@ -460,8 +460,8 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     const int lvl = static_cast<int>((s_lvl[k >> 2] >> (8 * (k & 3))) & 0xffu);
     const int level = lvl - j;
     const uint32_t sh = level_shift(level);
-    corner[slot] = morton_decode((ci >> sh) << sh, a.min_coord, a.range);
-    cell[slot] = s_cell[static_cast<uint32_t>(level - root_level) & 31u];
+    store_stream(corner + slot, morton_decode((ci >> sh) << sh, a.min_coord, a.range));
+    store_stream(cell + slot, s_cell[static_cast<uint32_t>(level - root_level) & 31u]);
     int4 r0, r1;
     int32_t m;
     if (j > 0) {  // the child below, in the same chain
@@ -478,10 +478,11 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
       r0 = make_int4(row[0], row[kOctBlock], row[2 * kOctBlock], row[3 * kOctBlock]);
       r1 = make_int4(row[4 * kOctBlock], row[5 * kOctBlock], row[6 * kOctBlock], row[7 * kOctBlock]);
     }
-    rows8[slot * 2] = r0;
-    rows8[slot * 2 + 1] = r1;
-    mask[slot] = m;
-    leaf_mask[slot] = 0;
+    // final outputs, read by no later kernel of the build: nontemporal stores
+    store_stream(rows8 + slot * 2, r0);
+    store_stream(rows8 + slot * 2 + 1, r1);
+    store_stream(mask + slot, m);
+    store_stream(leaf_mask + slot, 0);
   }
   if (kCheck && flags && a.status) atomicOr(&a.status[sg.id], flags);
 }

@@ -180,8 +180,8 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 
 // ---------------------------------------------------------------------------
 // Histogram sweep fused with stage 1, for a whole single-cloud build: each
-// workgroup encodes a unit of points (16-byte float4 loads, four in flight per
-// thread), writes the codes and counts their digits in LDS as k_digit_hist
+// workgroup encodes a unit of points (16-byte nontemporal float4 loads, four
+// in flight per thread), writes the codes and counts their digits in LDS as k_digit_hist
 // does, so the sort does not read the codes an extra time.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __restrict__ pts,
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
     float4 q[kDepth];
 #pragma unroll
     for (uint32_t r = 0; r < kDepth; ++r)
-      if (i0 + r * kHistBlock < end) q[r] = pts[i0 + r * kHistBlock];
+      if (i0 + r * kHistBlock < end) q[r] = load_stream(pts + i0 + r * kHistBlock);
 #pragma unroll
     for (uint32_t r = 0; r < kDepth; ++r) {
       const uint32_t i = i0 + r * kHistBlock;

@@ -9,7 +9,8 @@ namespace rw {
 // ---------------------------------------------------------------------------
 // S1  Morton encode (reference: apps/tree/cuda/01_morton.cu:8-19,
 // apps/tree/host/func_morton.hpp:31-39).  A workgroup covers 4*256 points;
-// every load is a whole float4 (16 bytes) and lanes read consecutive points.
+// every load is a whole float4 (16 bytes, nontemporal: each point is read
+// once) and lanes read consecutive points.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMortonPerThread = 4;
 constexpr uint32_t kMortonTile = kElemBlock * kMortonPerThread;  // 1024
@@ -26,7 +27,7 @@ __global__ __launch_bounds__(kElemBlock) void k_morton(const float4* __restrict_
   for (uint32_t j = 0; j < kMortonPerThread; ++j) {
     const uint32_t i = first + j * kElemBlock + threadIdx.x;
     if (i < lim) {
-      const float4 p = pts[i];
+      const float4 p = load_stream(pts + i);
       codes[i] = morton_encode(p.x, p.y, p.z, q);
     }
   }
@@ -578,8 +579,9 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
       *reinterpret_cast<int4*>(lc + x) =
           make_int4(lo + static_cast<int>(r4.x & 0xffffu), lo + static_cast<int>(r4.y & 0xffffu),
                     lo + static_cast<int>(r4.z & 0xffffu), lo + static_cast<int>(r4.w & 0xffffu));
-      *reinterpret_cast<int4*>(par + x) = make_int4(parent_of(p2.x & 0xffffu), parent_of(p2.x >> 16),
-                                                     parent_of(p2.y & 0xffffu), parent_of(p2.y >> 16));
+      // no later kernel of the build reads parent[]: a nontemporal store
+      store_stream(reinterpret_cast<int4*>(par + x), make_int4(parent_of(p2.x & 0xffffu), parent_of(p2.x >> 16),
+                                                               parent_of(p2.y & 0xffffu), parent_of(p2.y >> 16)));
       if (ec) {
         const int4 c4 = make_int4(count_of(r4.x, p2.x & 0xffffu), count_of(r4.y, p2.x >> 16),
                                   count_of(r4.z, p2.y & 0xffffu), count_of(r4.w, p2.y >> 16));

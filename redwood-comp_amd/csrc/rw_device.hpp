@@ -47,6 +47,41 @@ __device__ __forceinline__ Seg seg_of(SegArgs a, uint32_t index) {
   return s;
 }
 
+// A float4 that is read once (the input points): a nontemporal load, so the
+// stream does not displace from the caches the data the next kernels reuse.
+__device__ __forceinline__ float4 load_stream(const float4* p) {
+  typedef float f4v __attribute__((vector_size(16)));
+  const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  return make_float4(t[0], t[1], t[2], t[3]);
+}
+
+__device__ __forceinline__ uint4 load_stream(const uint4* p) {
+  typedef uint32_t u4v __attribute__((vector_size(16)));
+  const u4v t = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(p));
+  return make_uint4(t[0], t[1], t[2], t[3]);
+}
+template <class T>
+__device__ __forceinline__ T load_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// Stores of outputs that no later kernel of the build reads (the octree
+// arrays): nontemporal, for the same reason.
+__device__ __forceinline__ void store_stream(int4* p, int4 v) {
+  typedef int i4v __attribute__((vector_size(16)));
+  const i4v t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<i4v*>(p));
+}
+__device__ __forceinline__ void store_stream(float4* p, float4 v) {
+  typedef float f4v __attribute__((vector_size(16)));
+  const f4v t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+}
+template <class T>
+__device__ __forceinline__ void store_stream(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level helpers.
 // ---------------------------------------------------------------------------

@@ -253,6 +253,11 @@ inline uint32_t __builtin_amdgcn_bitop3_b32(uint32_t a, uint32_t b, uint32_t c, 
   return r;
 }
 inline void __builtin_amdgcn_s_waitcnt(int) {}
+// a cache hint only: a plain load on the CPU
+template <class T>
+inline T __builtin_nontemporal_load(const T* p) { return *p; }
+template <class T>
+inline void __builtin_nontemporal_store(T v, T* p) { *p = v; }
 
 template <class T, class U>
 inline T atomicAdd(T* p, U v) { return std::atomic_ref<T>(*p).fetch_add(static_cast<T>(v)); }
