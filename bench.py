@@ -506,7 +506,9 @@ def main():
         gbs = b / (ms * 1e-3) / 1e9
         stages[name] = {"ms": round(float(ms), 4), "bytes": int(b), "GB/s": round(gbs, 1),
                         "frac": round(gbs / HBM_PEAK_GBS, 3), "frac_of_copy": round(gbs / copy_gbs, 3)}
-    # the dominant kernel: S7 is one launch (k_oct_nodes), the longest of the build
+    # the dominant stage: S7, the node kernel and the short kernel that walks
+    # the links that left a tile's LDS window (k_oct_nodes + k_oct_far), timed
+    # together by the stage's events
     s7 = stages["S7_octree"]
     out_bytes_true = 17 * nb + 60 * no  # every slot's row, corner, cell and both masks
     sort = stages["S2_sort"]
@@ -527,7 +529,7 @@ def main():
                                f"in [0,1024)^3, seed 114514, 7 stages, inputs resident in HBM",
                    "n_points": n, "n_unique": nu, "n_octree_nodes": no},
         "verified": verified,
-        "roofline": {"bound": "hbm", "kernel": "k_oct_nodes (S7 octree nodes, one launch)",
+        "roofline": {"bound": "hbm", "kernel": "k_oct_nodes + k_oct_far (S7 octree nodes, two launches)",
                      "achieved": s7["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": s7["frac"], "traffic": profile_figure(args.log2n, "k_oct_nodes_hbm_bytes_per_call"),
                      "algorithmic_bytes": s7["bytes"], "ms": s7["ms"],
