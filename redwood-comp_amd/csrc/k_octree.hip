@@ -272,7 +272,31 @@ __device__ __forceinline__ int32_t pull_root(const OctArgs& a, uint32_t base, in
 }
 
 // ---- the pull ------------------------------------------------------------------
+// Pending sibling pairs of a pull, in registers: a push shifts every entry
+// down and puts the new pair on top, a pop shifts them up (no indexed
+// access, so the stack stays in four VGPRs and costs no LDS).
 constexpr int kPullStack = 4;
+struct PullStack {
+  uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // e0 on top
+  int sp = 0;
+  __device__ __forceinline__ bool full() const { return sp >= kPullStack; }
+  __device__ __forceinline__ bool empty() const { return sp == 0; }
+  __device__ __forceinline__ void push(uint32_t v) {
+    e3 = e2;
+    e2 = e1;
+    e1 = e0;
+    e0 = v;
+    ++sp;
+  }
+  __device__ __forceinline__ uint32_t pop() {
+    const uint32_t v = e0;
+    e0 = e1;
+    e1 = e2;
+    e2 = e3;
+    --sp;
+    return v;
+  }
+};
 
 // The tile's queue of far walks: the number taken (LDS) and the tile's
 // kOctFarCap items in the workspace (nullptr: none, every walk runs in
@@ -296,11 +320,11 @@ constexpr uint32_t kNoFarItem = 0xffffffffu;
 // tile's queue has room; otherwise the whole walk is redone by pull_spill.
 template <bool kCheck>
 __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, const Window& w,
-                                             int n, uint32_t pair, int32_t* row, uint32_t* stk,
+                                             int n, uint32_t pair, int32_t* row,
                                              uint32_t& flags, const FarQueue& far, uint32_t slot_rel) {
   const uint32_t pair0 = pair;
   int32_t m = 0;
-  int sp = 0;
+  PullStack stk;
   int steps = 0;
   bool spill = false;
   for (; steps < kPullBudget; ++steps) {
@@ -321,14 +345,14 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
       if (d.y & kLinkBit) {
         atomicMax(&row[(d.y & 7u) * kOctBlock], static_cast<int32_t>(d.x));
         m |= static_cast<int32_t>(1u << (d.y & 7u));
-      } else if (sp < kPullStack) {
-        stk[(sp++) * kOctBlock] = (d.x & 0x3fffffffu) | ((d.y & 3u) << 30);
+      } else if (!stk.full()) {
+        stk.push((d.x & 0x3fffffffu) | ((d.y & 3u) << 30));
       } else {
         spill = true;
       }
     }
-    if (sp == 0) break;
-    pair = stk[(--sp) * kOctBlock];
+    if (stk.empty()) break;
+    pair = stk.pop();
   }
   if (spill) m |= pull_spill<kCheck>(a, base, n, pair0, row, flags, steps);
   if (steps >= kPullBudget) flags |= RWH_STATUS_PULL;  // a malformed tree: the walk was cut short
@@ -348,17 +372,21 @@ __device__ __forceinline__ int32_t pull_pair(const OctArgs& a, uint32_t base, co
 // indices are checked and walks that stop at their bounds are reported in
 // a.status (RWH_STATUS_PULL).  The whole builds run the kernel without the
 // checks on the tree their radix-tree kernels built.
+// LDS (about 40 KiB) and registers (at most 64 VGPRs and 100 SGPRs, which
+// waves_per_eu(8) asks of the compiler) fit four workgroups, 32 waves, per
+// CU: the kernel waits on memory, and a fourth workgroup per CU took it from
+// 244 to 227 us (uniform 2^24) and from 227 to 196 us (clustered).
 template <bool kCheck>
-__global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg, uint32_t n_tiles) {
+__global__ __launch_bounds__(kOctBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_oct_nodes(
+    OctArgs a, SegArgs seg, uint32_t n_tiles) {
   __shared__ uint2 s_desc[kOctWin];
   __shared__ __align__(16) int32_t s_off[kSpan];
   __shared__ __align__(16) uint32_t s_code[kSpan];
   __shared__ __align__(16) uint32_t s_lcf[kSpan];  // left_child | leaf_l << 30 | leaf_r << 31
   __shared__ uint32_t s_lvl[kSpan / 4];  // prefix_n / 3, one byte per node
-  // each thread's pulled row and pull stack, entry-major (entry e of thread t
-  // at [e * kOctBlock + t]) so that a wave's accesses fall in distinct banks
+  // each thread's pulled row, entry-major (entry e of thread t at
+  // [e * kOctBlock + t]) so that a wave's accesses fall in distinct banks
   __shared__ int32_t s_row[8 * kOctBlock];
-  __shared__ uint32_t s_stk[kPullStack * kOctBlock];
   __shared__ float s_cell[32];  // range / 2^k
   __shared__ int64_t s_end;
   __shared__ uint32_t s_nfar;  // far walks queued by this tile
@@ -445,7 +473,6 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
 
   // 2. the slots, one per thread
   int32_t* row = s_row + tid;
-  uint32_t* stk = s_stk + tid;
   const FarQueue far{&s_nfar, far_items};
   const int64_t lo = max<int64_t>(s_off[0], 0);
   const int64_t hi = min<int64_t>(s_end, rows);
@@ -473,7 +500,7 @@ __global__ __launch_bounds__(kOctBlock) void k_oct_nodes(OctArgs a, SegArgs seg,
     } else {  // the bottom: pull the chains that link here
 #pragma unroll
       for (int e = 0; e < 8; ++e) row[e * kOctBlock] = 0;
-      m = pull_pair<kCheck>(a, sg.base, w, n, s_lcf[k], row, stk, flags, far, static_cast<uint32_t>(slot));
+      m = pull_pair<kCheck>(a, sg.base, w, n, s_lcf[k], row, flags, far, static_cast<uint32_t>(slot));
       if (slot == 0) m |= pull_root<kCheck>(a, sg.base, n, row, flags);
       r0 = make_int4(row[0], row[kOctBlock], row[2 * kOctBlock], row[3 * kOctBlock]);
       r1 = make_int4(row[4 * kOctBlock], row[5 * kOctBlock], row[6 * kOctBlock], row[7 * kOctBlock]);
