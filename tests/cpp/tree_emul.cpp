@@ -11,6 +11,7 @@
 // sums (u32, one per tree tile).
 #include "k_stages.hip"
 
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -31,12 +32,19 @@ int main(int argc, char** argv) {
   // exact-size heap buffers (ASan guards both ends)
   auto* codes = new uint32_t[nu];
   std::copy(keys.begin(), keys.end(), codes);
-  auto* prefix_n = new uint8_t[nu]();
-  auto* hll = new uint8_t[nu]();
-  auto* hlr = new uint8_t[nu]();
-  auto* lc = new int32_t[nu]();
-  auto* par = new int32_t[nu]();
-  auto* ec = new int32_t[nu]();
+  // outputs poisoned (bytes 0x5a): an entry no kernel writes shows as a mismatch
+  auto* prefix_n = new uint8_t[nu];
+  auto* hll = new uint8_t[nu];
+  auto* hlr = new uint8_t[nu];
+  auto* lc = new int32_t[nu];
+  auto* par = new int32_t[nu];
+  auto* ec = new int32_t[nu];
+  std::fill_n(prefix_n, nu, uint8_t{0x5a});
+  std::fill_n(hll, nu, uint8_t{0x5a});
+  std::fill_n(hlr, nu, uint8_t{0x5a});
+  std::fill_n(lc, nu, 0x5a5a5a5a);
+  std::fill_n(par, nu, 0x5a5a5a5a);
+  std::fill_n(ec, nu, 0x5a5a5a5a);
   const uint32_t tiles = rw::tree_tiles(nu);
   auto* list = new uint32_t[rw::tree_list_words(nu)]();
   auto* tile_sum = new uint32_t[tiles]();
