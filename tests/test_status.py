@@ -132,3 +132,33 @@ def test_octree_reports_a_malformed_tree(gpu, orc, pool, how, with_ws):
     rw.check(L.rwh_octree(ws.ptr if ws else None, tb.ptr(), d_nu.ptr, d_no.ptr, nu, 0.0, 1024.0,
                           d_status.ptr, None))
     assert int(d_status.numpy()[0]) & rw.STATUS_PULL
+
+
+@pytest.mark.gpu
+def test_octree_queue_overflow_equals_walks_in_place(gpu, orc, pool):
+    """rwh_octree with a workspace (far walks queued, the overflow redone from
+    global memory) and without one (every walk in place) give the same rows,
+    on a tree whose first tile has more far walks than its queue holds."""
+    from parity_helpers import far_fanout_tree
+    rw = gpu
+    L = rw.lib()
+    nu, nb, no, o = far_fanout_tree(rw, orc, pool)
+    cap = rw.octree_capacity(nu)
+    d_nu = rw.DeviceArray.from_numpy(np.array([nu], np.uint32))
+    d_no = rw.DeviceArray.from_numpy(np.array([no], np.uint32))
+    outs = []
+    for with_ws in (False, True):
+        tb = rw.TreeBuild(nu, cap)
+        tb["morton_keys"].upload(o["unique"][:nu])
+        for f in ("prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parents", "edge_count",
+                  "edge_offset"):
+            tb[f].upload(o[f][:nb])
+        d_status = rw.DeviceArray(1, np.uint32, zero=True)
+        ws = rw.Workspace(nu) if with_ws else None
+        print(f"case far fan-out n_brt={nb} workspace={with_ws}", flush=True)
+        rw.check(L.rwh_octree(ws.ptr if ws else None, tb.ptr(), d_nu.ptr, d_no.ptr, nu, 0.0, 1024.0,
+                              d_status.ptr, None))
+        assert int(d_status.numpy()[0]) == 0
+        outs.append(tb.read(nu, min(no, cap)))
+    for f in ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"):
+        assert np.array_equal(outs[0][f].view(np.uint8), outs[1][f].view(np.uint8)), f

@@ -218,3 +218,19 @@ def test_octree_kernel_on_cpu_reports_a_malformed_tree(rw, orc, pool, tmp_path, 
     assert got["status"] & 0x10  # RWH_STATUS_PULL; every walk ended, nothing read out of bounds
     if queue and how == "far_cycle":
         assert got["queued"] > 0  # the cycle was walked by the second kernel
+
+
+def test_octree_queue_overflow_equals_walks_in_place(rw, orc, pool, tmp_path):
+    """A tile with more far walks than its queue holds: the walks queued for
+    k_oct_far and the ones redone from global memory after the pull loop give
+    the same rows as every walk run in place (max and OR in any order)."""
+    from parity_helpers import far_fanout_tree
+    nu, nb, no, o = far_fanout_tree(rw, orc, pool)
+    cap = rw.octree_capacity(nu)
+    for check in (True, False):
+        inplace = run_octree(o, nu, no, cap, tmp_path, check=check)
+        queued = run_octree(o, nu, no, cap, tmp_path, check=check, queue=True)
+        assert inplace["status"] == queued["status"] == 0
+        assert queued["queued"] >= 16  # the first tile's queue is full
+        for f in ("children", "corner", "cell_size", "child_node_mask", "child_leaf_mask"):
+            assert np.array_equal(inplace[f][:no].view(np.uint8), queued[f][:no].view(np.uint8)), (check, f)
