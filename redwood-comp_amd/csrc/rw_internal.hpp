@@ -32,7 +32,7 @@ constexpr uint32_t kSortMinTile = 4096;  // smallest tile: sizes the look-back t
 constexpr int kHistBlock = 1024;
 constexpr uint32_t kHistCopies = 16;
 constexpr uint32_t kMaxPasses = 4;
-constexpr uint32_t kPlainHistUnit = 65536;  // keys per histogram workgroup, one segment
+constexpr uint32_t kPlainHistUnit = 16384;  // keys per histogram workgroup, one segment (the largest sort tile)
 constexpr uint32_t kTilesPerHistUnit = kPlainHistUnit / kSortMinTile;
 // log2 of a variant's tile size; a single-cloud histogram unit holds whole tiles
 constexpr uint32_t sort_tile_shift(int v) {
