@@ -729,9 +729,9 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   if (injected(1) || injected(2)) return RWH_ERR_INJECTED;
   RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
   if (n > 0) {
-    launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n,
-                       min_coord, range, make_plan(30, ws->sort_plan), ws->hist, ws->sort_variant,
-                       ws->tile_counts, d_status, st);
+    launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, single_seg(n), n,
+                       kPlainHistUnit, min_coord, range, make_plan(30, ws->sort_plan), ws->hist,
+                       ws->sort_variant, ws->tile_counts, d_status, st);
     RWH_TRY(launched());
     RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
                          st, true));
@@ -821,11 +821,14 @@ int enqueue_batch(rwh_workspace* ws, const rwh_tree_buffers* bufs, float min_coo
   SegArgs uni = pts;
   uni.count = ws->n_unique();
 
-  launch_morton(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, pts, n_index,
-                min_coord, range, ws->status(), st);
+  // S1 fused with the sort's histogram sweep (chunk-sized units, each in one
+  // cloud), then the digit passes
+  launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, pts, n_index, kChunk,
+                     min_coord, range, make_plan(30, ws->sort_plan), ws->hist, ws->sort_variant, nullptr,
+                     ws->status(), st);
   RWH_TRY(launched());
   RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, pts, n_clouds, n_index, 30,
-                       false, st));
+                       false, st, true));
   launch_unique(bufs->morton_keys_alt, bufs->morton_keys, pts, n_index, ws->n_unique(),
                 ws->lb_unique, ws->ctrl + 1 * kCtrlWords, st);
   RWH_TRY(launched());

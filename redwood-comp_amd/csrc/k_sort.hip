@@ -179,13 +179,15 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
 }
 
 // ---------------------------------------------------------------------------
-// Histogram sweep fused with stage 1, for a whole single-cloud build: each
-// workgroup encodes a unit of points (16-byte nontemporal float4 loads, four
-// in flight per thread), writes the codes and counts their digits in LDS as k_digit_hist
-// does, so the sort does not read the codes an extra time.
+// Histogram sweep fused with stage 1, for a whole build (one cloud, or every
+// cloud of a batch): each workgroup encodes a unit of points of one cloud
+// (16-byte nontemporal float4 loads, four in flight per thread), writes the
+// codes and counts their digits in LDS into the cloud's histogram copies as
+// k_digit_hist does, so the sort does not read the codes an extra time.  The
+// first unit of each cloud clears its status word (the build's first kernel).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __restrict__ pts,
-                                                            uint32_t* __restrict__ codes, uint32_t n,
+                                                            uint32_t* __restrict__ codes, SegArgs seg,
                                                             Quant quant,
                                                             uint32_t unit_keys, DigitPlan plan,
                                                             uint32_t* __restrict__ hist,
@@ -208,10 +210,12 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
   if (per_tile)
     for (uint32_t i = threadIdx.x; i < kTilesPerHistUnit * 256; i += kHistBlock) s_tc[i] = 0;
   const uint32_t u = blockIdx.x;
-  if (u == 0 && threadIdx.x == 0 && status) *status = 0;  // the build's first kernel
-  __syncthreads();
   const uint32_t first = u * unit_keys;
-  const uint32_t end = min(first + unit_keys, n);
+  const Seg sg = seg_of(seg, first);
+  const uint32_t lim = sg.base + sg.count;
+  if (threadIdx.x == 0 && status && first == sg.base) status[sg.id] = 0;  // the build's first kernel
+  __syncthreads();
+  const uint32_t end = min(first + unit_keys, lim);
   constexpr uint32_t kDepth = 4;
   for (uint32_t i0 = first + threadIdx.x; i0 < end; i0 += kDepth * kHistBlock) {
     float4 q[kDepth];
@@ -236,10 +240,11 @@ __global__ __launch_bounds__(kHistBlock) void k_morton_hist(const float4* __rest
   }
   __syncthreads();
   if (per_tile) {
-    if (first < end) store_tile_counts(s_tc, first, end, n, tile_shift, tile_counts);
+    if (first < end) store_tile_counts(s_tc, first, end, lim, tile_shift, tile_counts);
     finish_pass0(s_tc, s_h);
   }
-  uint32_t* dst = hist + (u % kHistCopies) * (kMaxPasses * 256);
+  uint32_t* dst = hist + (static_cast<size_t>(sg.id) * kHistCopies + (u % kHistCopies)) *
+                             (kMaxPasses * 256);
   for (uint32_t i = threadIdx.x; i < n_passes * 256; i += kHistBlock) {
     const uint32_t c = s_h[i];
     if (c) atomicAdd(&dst[i], c);
@@ -252,15 +257,15 @@ constexpr bool table_pass0(const DigitPlan& plan) {
   return plan.n_passes >= 2 && plan.bits[0] >= 6 && plan.bits[0] <= 8;
 }
 
-void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
-                        const DigitPlan& plan, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
-                        uint32_t* status, hipStream_t stream) {
-  if (n == 0) return;
-  const uint32_t units = (n + kPlainHistUnit - 1) / kPlainHistUnit;
-  if (!table_pass0(plan)) tile_counts = nullptr;
-  hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, n,
-                     make_quant(min_coord, range), kPlainHistUnit, plan, hist,
-                     sort_tile_shift(sort_variant), tile_counts, status);
+void launch_morton_hist(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index, uint32_t unit_keys,
+                        float min_coord, float range, const DigitPlan& plan, uint32_t* hist, int sort_variant,
+                        uint32_t* tile_counts, uint32_t* status, hipStream_t stream) {
+  if (n_index == 0) return;
+  const uint32_t units = (n_index + unit_keys - 1) / unit_keys;
+  if (!table_pass0(plan) || seg.n_segs != 1) tile_counts = nullptr;
+  hipLaunchKernelGGL(k_morton_hist, dim3(units), dim3(kHistBlock), 0, stream, pts, codes, seg,
+                     make_quant(min_coord, range), unit_keys, plan, hist, sort_tile_shift(sort_variant),
+                     tile_counts, status);
 }
 
 // Pass 0's exclusive digit prefix of every tile of a single-cloud sort, in

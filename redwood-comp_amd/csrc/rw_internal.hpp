@@ -182,14 +182,15 @@ struct SortLaunch {
   uint32_t* tile_counts;
 };
 void launch_sort(const SortLaunch& L, hipStream_t stream);
-// Stage 1 for one cloud of n points fused with the sort's histogram sweep
-// (the digit histograms of the plan's passes go to hist's copies, and pass
-// 0's counts per tile of sort_variant to tile_counts unless it is nullptr).
-// status (device word or nullptr) is cleared by the launch: the build's
-// first kernel.
-void launch_morton_hist(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
-                        const DigitPlan& plan, uint32_t* hist, int sort_variant, uint32_t* tile_counts,
-                        uint32_t* status, hipStream_t stream);
+// Stage 1 fused with the sort's histogram sweep, over units of unit_keys
+// points each inside one segment (the sort's plain unit for one cloud, a
+// chunk for a batch): the digit histograms of the plan's passes go to each
+// segment's hist copies, and, for one cloud only, pass 0's counts per tile of
+// sort_variant to tile_counts unless it is nullptr.  status (per segment, or
+// nullptr) is cleared by the launch: the build's first kernel.
+void launch_morton_hist(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index, uint32_t unit_keys,
+                        float min_coord, float range, const DigitPlan& plan, uint32_t* hist, int sort_variant,
+                        uint32_t* tile_counts, uint32_t* status, hipStream_t stream);
 
 // status (per segment, or nullptr) is cleared by the launch: the first kernel
 // of a batched build.
