@@ -436,6 +436,11 @@ __global__ __launch_bounds__(kOctBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
     }
   }
   const int q0 = b + 4 * static_cast<int>(tid);
+  // offsets of the tile's nodes for the slot owners' search; past the last
+  // valid node they read as INT32_MAX, above every slot
+  constexpr int32_t kPastEnd = 0x7fffffff;
+  if (tid < kOctLoaders && q0 >= n)
+    reinterpret_cast<int4*>(s_off)[tid] = make_int4(kPastEnd, kPastEnd, kPastEnd, kPastEnd);
   if (tid < kOctLoaders && q0 < n) {
     const Nodes4 g = load_nodes4(a, sg.base, q0, n);
     uint32_t lcf[4], lv = 0;
@@ -455,7 +460,8 @@ __global__ __launch_bounds__(kOctBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
       // the last valid node closes the tile's slot range
       if (q0 + e == b + nv - 1) s_end = static_cast<int64_t>(g.off[e]) + max(g.cnt[e], 0);
     }
-    reinterpret_cast<int4*>(s_off)[tid] = make_int4(g.off[0], g.off[1], g.off[2], g.off[3]);
+    reinterpret_cast<int4*>(s_off)[tid] = make_int4(g.off[0], q0 + 1 < n ? g.off[1] : kPastEnd,
+                                                   q0 + 2 < n ? g.off[2] : kPastEnd, q0 + 3 < n ? g.off[3] : kPastEnd);
     reinterpret_cast<uint4*>(s_code)[tid] = make_uint4(g.code[0], g.code[1], g.code[2], g.code[3]);
     reinterpret_cast<uint4*>(s_lcf)[tid] = make_uint4(lcf[0], lcf[1], lcf[2], lcf[3]);
     s_lvl[tid] = lv;
@@ -474,13 +480,13 @@ __global__ __launch_bounds__(kOctBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   // 2. the slots, one per thread
   int32_t* row = s_row + tid;
   const FarQueue far{&s_nfar, far_items};
-  const int64_t lo = max<int64_t>(s_off[0], 0);
-  const int64_t hi = min<int64_t>(s_end, rows);
-  for (int64_t slot = lo + tid; slot < hi; slot += kOctBlock) {
-    int k = 0;  // last node with offset <= slot
+  // slots below rows, at most the octree capacity (< 2^31): 32-bit indices
+  const int lo = max(s_off[0], 0);
+  const int hi = static_cast<int>(min<int64_t>(s_end, rows));
+  for (int slot = lo + static_cast<int>(tid); slot < hi; slot += kOctBlock) {
+    int k = 0;  // last node with offset <= slot (the padding past nv is never taken)
 #pragma unroll
-    for (int step = kSpan / 2; step >= 1; step >>= 1)
-      if (k + step < nv && s_off[k + step] <= slot) k += step;
+    for (int step = kSpan / 2; step >= 1; step >>= 1) k = s_off[k + step] <= slot ? k + step : k;
     if (b + k == 0) continue;  // node 0 owns no chain (the reference starts at 1)
     const int j = static_cast<int>(slot - s_off[k]);
     const uint32_t ci = s_code[k];
