@@ -355,8 +355,10 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   __shared__ uint32_t s_ks[T + 2];  // keys lo-1 .. hi at [k - lo + 1]; then the split slots
   __shared__ uint8_t s_e[T + 2];    // delta(k, k+1) (clz) + 1 for k in [lo-1, hi-1] at [k - lo + 1]; 0 outside
   // node lo + x: its split - lo | (0x80 formed here | leaf left << 6 | leaf right << 5 | prefix) << 16
-  __shared__ alignas(16) uint32_t s_rec[T];
-  __shared__ alignas(16) uint16_t s_par[T];  // node lo + x: its parent - lo (0xffff: not formed here)
+  // (entries T + lane of s_rec and s_par: each lane's scratch sink for the
+  // stores that do not apply, distinct so that they share no LDS bank word)
+  __shared__ alignas(16) uint32_t s_rec[T + kWave];
+  __shared__ alignas(16) uint16_t s_par[T + 2 * kWave];  // node lo + x: its parent - lo (0xffff: not formed here)
   __shared__ uint16_t s_pairs[kTreeBlock / kWave][kTreePairCap];  // per wave: left leaves of leaf pairs
   __shared__ uint32_t s_n_cross;
   __shared__ uint32_t s_count_sum;  // the tile's edge counts, summed (with a.tile_sum)
@@ -447,31 +449,29 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   // is an internal node, formed later, so every leaf is done here.
   uint16_t* pairs = s_pairs[wave];
   int n_pairs = 0;
+  const int sink = T + static_cast<int>(tid % kWave);  // this lane's scratch entry of s_rec (s_par: 2 sink - T)
   for (int k0 = wlo; k0 < whi; k0 += kWave) {
     const int k = k0 + static_cast<int>(tid % kWave);
-    bool queued = false;
-    int left = 0;  // the queued pair's left leaf
-    if (k < whi) {
-      const uint32_t e0 = s_e[k], e1 = s_e[k + 1];
-      const bool right = e1 > e0;
-      const int g = right ? k : k - 1;
-      if (static_cast<uint32_t>(g) < n_split) {
-        const uint32_t other = atomicExch(&s_slot[g], right ? (kEnd | e0 << 16 | static_cast<uint32_t>(k))
-                                                            : (kEnd | e1 << 16 | static_cast<uint32_t>(k + 1)));
-        if (other != 0) {  // the sibling is a leaf: the parent is [g, g + 1]
-          const uint32_t es = (other >> 16) & 0xffu;
-          const uint32_t ea = right ? e0 : es, eb = right ? es : e1;
-          const uint32_t ep = right ? e1 : e0;  // delta at the split + 1
-          if ((ea | eb) == 0) {  // the complete root [0, 1]: index 0, whose record is Q2's
-            s_par[0] = 0;
-          } else {
-            s_rec[eb > ea ? g + 1 : g] = static_cast<uint32_t>(g) | (0xe0u | (ep - 2)) << 16;
-            queued = true;
-            left = g;
-          }
-        }
-      }
-    }
+    const int kk = min(k, whi - 1);  // lanes past the block read a valid leaf and do nothing
+    const uint32_t e0 = s_e[kk], e1 = s_e[kk + 1];
+    const bool right = e1 > e0;
+    const int g = right ? k : k - 1;
+    const bool go = k < whi && static_cast<uint32_t>(g) < n_split;
+    uint32_t other = 0;
+    if (go)
+      other = atomicExch(&s_slot[g], right ? (kEnd | e0 << 16 | static_cast<uint32_t>(k))
+                                           : (kEnd | e1 << 16 | static_cast<uint32_t>(k + 1)));
+    // the sibling is a leaf too: the parent is [g, g + 1]; two leaves over
+    // the complete range [0, 1] form the root, index 0, whose record is Q2's
+    const uint32_t es = (other >> 16) & 0xffu;
+    const uint32_t ea = right ? e0 : es, eb = right ? es : e1;
+    const uint32_t ep = right ? e1 : e0;  // delta at the split + 1
+    const bool met = other != 0;
+    const bool root = met && (ea | eb) == 0;
+    const bool queued = met && !root;
+    const int left = g;  // the queued pair's left leaf
+    s_rec[queued ? (eb > ea ? g + 1 : g) : sink] = static_cast<uint32_t>(g) | (0xe0u | (ep - 2)) << 16;
+    s_par[root ? 0 : 2 * sink - T] = 0;
     const uint64_t qm = ballot(queued);
     const int at = n_pairs + static_cast<int>(popc_below(qm));
     if (queued) {
@@ -490,50 +490,45 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_local(TreeArgs a, SegArgs s
   bool have = false;
   int guard = 0;
   for (; guard < kTreeClimbGuard; ++guard) {
+    // lanes without a node take the wave's next queued pairs
     const uint64_t need = ballot(!have);
-    if (!have) {
-      const int q = next + static_cast<int>(popc_below(need));
-      if (q < n_pairs) {
-        const int k = pairs[q];  // the pair [k, k + 1]
-        pa = kEnd | static_cast<uint32_t>(s_e[k]) << 16 | static_cast<uint32_t>(k);
-        pb = kEnd | static_cast<uint32_t>(s_e[k + 2]) << 16 | static_cast<uint32_t>(k + 2);
-        have = true;
-      }
+    const int q = next + static_cast<int>(popc_below(need));
+    if (!have && q < n_pairs) {
+      const int k = pairs[q];  // the pair [k, k + 1]
+      pa = kEnd | static_cast<uint32_t>(s_e[k]) << 16 | static_cast<uint32_t>(k);
+      pb = kEnd | static_cast<uint32_t>(s_e[k + 2]) << 16 | static_cast<uint32_t>(k + 2);
+      have = true;
     }
     next += static_cast<int>(__popcll(need));
     if (ballot(have) == 0) break;
-    if (!have) continue;
+    // a lane whose parent's split lies inside the tile leaves its far end
+    // there; the second to arrive forms the parent and climbs on, every other
+    // lane stops (at a split on the tile's edge, or first at its split)
     const bool right = pb > pa;
     const uint32_t near = right ? pb : pa;
     const int g = static_cast<int>(near & 0xffffu) - 1;  // the parent's split
-    if (static_cast<uint32_t>(g) >= n_split) {  // the split is an edge of the tile
-      have = false;
-      continue;
+    const bool go = have && static_cast<uint32_t>(g) < n_split;
+    uint32_t other = 0;
+    if (go) other = atomicExch(&s_slot[g], right ? pa : pb);
+    have = go && other != 0;
+    if (have) {
+      pb = right ? other : pb;
+      pa = right ? pa : other;
+      const int na = static_cast<int>(pa & 0xffffu), nb = static_cast<int>(pb & 0xffffu) - 1;
+      const bool ll = g == na, lr = g + 1 == nb;
+      // the complete root [0, n] (lo == 0) is index 0, whose record is Q2's:
+      // its internal children's parent entries and parent[0] get 0, no record
+      const bool root = ((pa | pb) & 0x00ff0000u) == 0;
+      const int i = root ? 0 : (pb > pa ? nb : na);
+      const uint32_t ep = (near >> 16) & 0xffu;  // delta at the split + 1: the parent's prefix + 2
+      // stores that do not apply go to the lane's scratch entries past the tile
+      s_rec[root ? sink : i] = static_cast<uint32_t>(g) |
+                               (0x80u | (ll ? 0x40u : 0u) | (lr ? 0x20u : 0u) | (ep - 2)) << 16;
+      s_par[ll ? 2 * sink - T : g] = static_cast<uint16_t>(i);
+      s_par[lr ? 2 * sink - T + 1 : g + 1] = static_cast<uint16_t>(i);
+      s_par[root ? 0 : 2 * sink - T] = 0;
+      have = !root;
     }
-    const uint32_t other = atomicExch(&s_slot[g], right ? pa : pb);
-    if (other == 0) {  // first child at the split: the sibling forms the parent
-      have = false;
-      continue;
-    }
-    if (right) {
-      pb = other;
-    } else {
-      pa = other;
-    }
-    const int na = static_cast<int>(pa & 0xffffu), nb = static_cast<int>(pb & 0xffffu) - 1;
-    const bool ll = g == na, lr = g + 1 == nb;
-    if (((pa | pb) & 0x00ff0000u) == 0) {  // the complete root [0, n] (lo == 0): index 0, whose record is Q2's
-      if (!ll) s_par[g] = 0;
-      if (!lr) s_par[g + 1] = 0;
-      s_par[0] = 0;
-      have = false;
-      continue;
-    }
-    const int i = pb > pa ? nb : na;
-    const uint32_t ep = (near >> 16) & 0xffu;  // delta at the split + 1: the parent's prefix + 2
-    s_rec[i] = static_cast<uint32_t>(g) | (0x80u | (ll ? 0x40u : 0u) | (lr ? 0x20u : 0u) | (ep - 2)) << 16;
-    if (!ll) s_par[g] = static_cast<uint16_t>(i);
-    if (!lr) s_par[g + 1] = static_cast<uint16_t>(i);
   }
   // the bound ended the loop with climbs or queued pairs left
   if (guard == kTreeClimbGuard && (ballot(have) != 0 || next < n_pairs) && tid % kWave == 0)
