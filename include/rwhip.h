@@ -112,6 +112,20 @@ int rwh_sort_plan_count(void);
 int rwh_sort_plan_widths(int plan, uint32_t* widths4);
 int rwh_workspace_set_sort_plan(rwh_workspace* ws, int plan);
 
+/* The sort of a whole single-cloud build of n points, 2^20 < n <= 2^24
+ * (same output as the reference's sort, cu_dispatcher.cu:528-591): with
+ * enable = 1 (a workspace's default) a bucket sort, two passes over the keys
+ * (the top 8..11 bits scatter the codes into buckets of about 8192 keys, one
+ * workgroup sorts each bucket in LDS), which hands over to the digit passes
+ * above on the device, at no host round trip, when a bucket of the input
+ * exceeds 9216 keys (clustered points); enable = 0 always runs the digit
+ * passes.  Other sizes, batches and rwh_sort_u32 run the digit passes. */
+int rwh_workspace_set_bucket_sort(rwh_workspace* ws, int enable);
+/* *used = 1 when the last whole build on ws sorted with the bucket sort, 0
+ * when it ran the digit passes (bucket sort off, n out of its range, or a
+ * bucket too large).  Waits for the stream. */
+int rwh_workspace_sort_used_buckets(rwh_workspace* ws, rwh_stream_t stream, int* used);
+
 /* ---- S1  (reference: cpu::run_stage1 host_dispatchers.cpp:17-41,
  *          cuda::run_stage1 cu_dispatcher.cu:501-522)
  * codes[i] = Morton code of float4 point i (x,y,z used), 10 bits per axis.

@@ -79,6 +79,8 @@ ABI = {
     "rwh_sort_plan_count": (_I, []),
     "rwh_sort_plan_widths": (_I, [_I, _P]),
     "rwh_workspace_set_sort_plan": (_I, [_P, _I]),
+    "rwh_workspace_set_bucket_sort": (_I, [_P, _I]),
+    "rwh_workspace_sort_used_buckets": (_I, [_P, _P, C.POINTER(_I)]),
     "rwh_morton": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_sort_u32": (_I, [_P, _P, _P, _U32, _U32, _P]),
     "rwh_unique_u32": (_I, [_P, _P, _U32, _P, _P, _P]),
@@ -295,7 +297,7 @@ class DeviceArray:
 
 class Workspace:
     def __init__(self, max_points: int, max_segments: int = 1, sort_variant: int | None = None,
-                 sort_plan: int | None = None):
+                 sort_plan: int | None = None, bucket_sort: bool | None = None):
         p = _P()
         check(lib().rwh_workspace_create(max_points, max_segments, C.byref(p)), "rwh_workspace_create")
         self.ptr = p.value
@@ -304,12 +306,26 @@ class Workspace:
             self.set_sort_variant(sort_variant)
         if sort_plan is not None:
             self.set_sort_plan(sort_plan)
+        if bucket_sort is not None:
+            self.set_bucket_sort(bucket_sort)
 
     def set_sort_variant(self, v: int):
         check(lib().rwh_workspace_set_sort_variant(self.ptr, v), "rwh_workspace_set_sort_variant")
 
     def set_sort_plan(self, p: int):
         check(lib().rwh_workspace_set_sort_plan(self.ptr, p), "rwh_workspace_set_sort_plan")
+
+    def set_bucket_sort(self, enable: bool):
+        """Whole single-cloud builds of (2^20, 2^24] points sort with the bucket
+        sort (True, the default) or always with the digit passes (False)."""
+        check(lib().rwh_workspace_set_bucket_sort(self.ptr, 1 if enable else 0), "rwh_workspace_set_bucket_sort")
+
+    def sort_used_buckets(self, stream=None) -> bool:
+        """Whether the last whole build on this workspace ran the bucket sort."""
+        used = _I()
+        check(lib().rwh_workspace_sort_used_buckets(self.ptr, stream, C.byref(used)),
+              "rwh_workspace_sort_used_buckets")
+        return bool(used.value)
 
     @property
     def counts_ptr(self) -> int:
@@ -461,12 +477,13 @@ class PinnedArray:
 
 def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
                  min_coord: float = MIN_COORD, range_: float = RANGE, link_leaf: bool = False,
-                 strict: bool = True, sort_variant: int | None = None, sort_plan: int | None = None):
+                 strict: bool = True, sort_variant: int | None = None, sort_plan: int | None = None,
+                 bucket_sort: bool | None = None):
     """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs).
     link_leaf also runs the opt-in leaf linking after S7.  strict=False returns the
     (capacity-clamped) outputs when n_octree_nodes exceeds the 0.6 n capacity
     instead of raising.  sort_variant picks the sort's tile shape (a new
-    workspace's default otherwise)."""
+    workspace's default otherwise); bucket_sort False forces the digit passes."""
     n = points.shape[0]
     cap = octree_capacity(n)
     ws = ws or Workspace(max(n, 1))
@@ -474,6 +491,8 @@ def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
         ws.set_sort_variant(sort_variant)
     if sort_plan is not None:
         ws.set_sort_plan(sort_plan)
+    if bucket_sort is not None:
+        ws.set_bucket_sort(bucket_sort)
     tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
     check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
     if link_leaf:

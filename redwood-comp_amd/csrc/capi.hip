@@ -120,6 +120,9 @@ struct rwh_workspace {
   uint32_t* tree_list = nullptr;  // radix tree: edge-crossing nodes per tile
   uint32_t* tree_sums = nullptr;  // radix tree: edge counts summed per tile (build path)
   uint2* far_item = nullptr;      // octree: far walks queued per tile (kOctFarCap each)
+  uint32_t* bucket_table = nullptr;  // bucket sort: [tiles][bins] counts, then tile starts
+  uint32_t* bucket_start = nullptr;  // bucket sort: [kBucketMaxBins + 1] starts, then [kBucketMaxBins] totals
+  uint32_t* bucket_ctl = nullptr;    // bucket sort: the flag (kBucketCtlWords)
   uint32_t* h_stage = nullptr;  // pinned staging of tables
   uint32_t* h_counts = nullptr;  // pinned counts read-back
   hipEvent_t upload_done = nullptr;
@@ -128,6 +131,8 @@ struct rwh_workspace {
   uint32_t batch_index = 0;
   int sort_variant = 0;       // index into kSortVariants
   int sort_plan = 0;          // index into kPlanWidths (sorts of 30-bit keys)
+  bool bucket_sort = true;    // single-cloud builds in the bucket sort's range use it
+  bool bucket_tried = false;  // the last whole build launched the bucket sort
 
   uint32_t* seg_base() { return tables; }
   uint32_t* seg_n() { return tables + max_segs; }
@@ -148,7 +153,8 @@ namespace {
 constexpr uint32_t ws_magic() {
   return 0x52575300u ^ static_cast<uint32_t>(sizeof(rwh_workspace)) ^
          (static_cast<uint32_t>(offsetof(rwh_workspace, tree_sums)) << 8) ^
-         (static_cast<uint32_t>(offsetof(rwh_workspace, far_item)) << 16);
+         (static_cast<uint32_t>(offsetof(rwh_workspace, far_item)) << 16) ^
+         (static_cast<uint32_t>(offsetof(rwh_workspace, bucket_ctl)) << 20);
 }
 
 constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -157,7 +163,7 @@ constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 // max_points index-space points and max_segs clouds.
 struct WsLayout {
   uint32_t max_chunks, max_sort_tiles, max_scan_tiles;
-  size_t tmp, hist, ds, lbs, tc, lbu, lbc, ctrl, cnt, tab, tl, ts, fi;
+  size_t tmp, hist, ds, lbs, tc, lbu, lbc, ctrl, cnt, tab, tl, ts, fi, bkt, bks, bkc;
   size_t bytes;
 };
 
@@ -186,6 +192,9 @@ constexpr WsLayout ws_layout(uint32_t max_points, uint32_t max_segs) {
   L.tl = take(sizeof(uint32_t) * static_cast<size_t>(tree_list_words(mp)));
   L.ts = take(sizeof(uint32_t) * static_cast<size_t>(tree_tiles(mp)));
   L.fi = take(sizeof(uint2) * static_cast<size_t>(oct_tiles(mp)) * kOctFarCap);
+  L.bkt = take(sizeof(uint32_t) * std::max<size_t>(1, bucket_table_words(mp)));
+  L.bks = take(sizeof(uint32_t) * (2 * kBucketMaxBins + 1));
+  L.bkc = take(sizeof(uint32_t) * kBucketCtlWords);
   L.bytes = off;
   return L;
 }
@@ -212,6 +221,9 @@ void bind_workspace(rwh_workspace* ws, char* base, const WsLayout& L, uint32_t m
   ws->tree_list = reinterpret_cast<uint32_t*>(base + L.tl);
   ws->tree_sums = reinterpret_cast<uint32_t*>(base + L.ts);
   ws->far_item = reinterpret_cast<uint2*>(base + L.fi);
+  ws->bucket_table = reinterpret_cast<uint32_t*>(base + L.bkt);
+  ws->bucket_start = reinterpret_cast<uint32_t*>(base + L.bks);
+  ws->bucket_ctl = reinterpret_cast<uint32_t*>(base + L.bkc);
 }
 
 // The allocation-free build's workspace: a static device array of the
@@ -236,7 +248,7 @@ namespace {
 
 int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs seg, uint32_t n_segs,
                  uint32_t n_index, uint32_t key_bits, bool plain, hipStream_t st,
-                 bool hist_ready = false) {
+                 bool hist_ready = false, const uint32_t* skip = nullptr) {
   SortLaunch L{};
   L.keys_in = in;
   L.keys_out = out;
@@ -258,6 +270,7 @@ int enqueue_sort(rwh_workspace* ws, const uint32_t* in, uint32_t* out, SegArgs s
   L.ctrl = ws->ctrl + 0 * kCtrlWords;
   L.hist_ready = hist_ready;
   L.tile_counts = ws->tile_counts;  // used when the sort is one cloud
+  L.skip = skip;
   if (n_index == 0) return RWH_OK;
   launch_sort(L, st);
   return launched();
@@ -547,6 +560,24 @@ int rwh_workspace_set_sort_plan(rwh_workspace* ws, int plan) {
   return RWH_OK;
 }
 
+int rwh_workspace_set_bucket_sort(rwh_workspace* ws, int enable) {
+  RWH_CHECK(ws, "null workspace");
+  RWH_CHECK(enable == 0 || enable == 1, "enable must be 0 or 1");
+  ws->bucket_sort = enable == 1;
+  return RWH_OK;
+}
+
+int rwh_workspace_sort_used_buckets(rwh_workspace* ws, rwh_stream_t stream, int* used) {
+  RWH_CHECK(ws && used, "null argument");
+  *used = 0;
+  if (!ws->bucket_tried) return RWH_OK;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  RWH_HIP(hipMemcpyAsync(ws->h_counts, ws->bucket_ctl, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  RWH_HIP(hipStreamSynchronize(st));
+  *used = ws->h_counts[0] != 0 ? 1 : 0;
+  return RWH_OK;
+}
+
 uint32_t* rwh_workspace_counts(rwh_workspace* ws) { return ws ? ws->counts : nullptr; }
 uint32_t rwh_workspace_max_segments(const rwh_workspace* ws) { return ws ? ws->max_segs : 0; }
 
@@ -728,7 +759,22 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   // S1 fused with the sort's histogram sweep, then the digit passes
   if (injected(1) || injected(2)) return RWH_ERR_INJECTED;
   RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
-  if (n > 0) {
+  const BucketPlan bp = bucket_plan(n);
+  ws->bucket_tried = n > 0 && ws->bucket_sort && bp.ok;
+  if (ws->bucket_tried) {
+    // S1 with the bucket counts, the bucket sort, and the digit-pass sort
+    // (its own histogram sweep first) gated on its flag: it runs only when a
+    // bucket is too large for LDS
+    launch_morton_bucket(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n, min_coord, range,
+                         bp, ws->bucket_table, d_status, st);
+    RWH_TRY(launched());
+    const BucketLaunch B{bufs->morton_keys, bufs->morton_keys_alt, ws->keys_tmp, n, bp, ws->bucket_table,
+                         ws->bucket_start + kBucketMaxBins + 1, ws->bucket_start, ws->bucket_ctl};
+    launch_bucket_sort(B, st);
+    RWH_TRY(launched());
+    RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
+                         st, false, ws->bucket_ctl));
+  } else if (n > 0) {
     launch_morton_hist(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, single_seg(n), n,
                        kPlainHistUnit, min_coord, range, make_plan(30, ws->sort_plan), ws->hist,
                        ws->sort_variant, ws->tile_counts, d_status, st);

@@ -32,41 +32,6 @@ namespace rw {
 // the unit's counts are added to copy (u % kHistCopies) of its segment's
 // histogram, which spreads the global atomics over many cache lines.
 // ---------------------------------------------------------------------------
-// True when every active lane holds the same x (wave-uniform).
-__device__ __forceinline__ bool wave_same(uint32_t x) {
-  return ballot(x != __builtin_amdgcn_readfirstlane(x)) == 0;
-}
-
-// Adds one to h[d] for every active lane.  When all active lanes hit the
-// same counter (sorted or clustered keys: consecutive keys share their high
-// digits) one lane adds their number instead: 64 atomics on one LDS address
-// would otherwise be serialised.  try_wave (wave-uniform) false skips the
-// check: the caller saw no digit the wave shares (or only a few keys).
-__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool try_wave = true) {
-  if (try_wave && wave_same(d)) {
-    const uint64_t act = ballot(true);
-    if (lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1))
-      atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
-  } else {
-    atomicAdd(&h[d], 1u);
-  }
-}
-
-// atomicAdd(&h[d], 1) for every active lane, returning each lane's old
-// value (distinct ranks, in no particular order within the wave), with one
-// atomic when all active lanes hold the same digit (checked only when
-// try_wave, as for hist_add).
-__device__ __forceinline__ uint32_t rank_add(uint32_t* h, uint32_t d, bool try_wave) {
-  if (try_wave && wave_same(d)) {
-    const uint64_t act = ballot(true);
-    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1);
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
-    return __shfl(base, leader, kWave) + popc_below(act);
-  }
-  return atomicAdd(&h[d], 1u);
-}
-
 // Pass 0's digit counts of the sort tiles a histogram unit covers (the unit
 // starts at a multiple of the tile size), from LDS to the digit-major table
 // tile_counts[digit][tile] of a sort of n keys.
@@ -98,7 +63,9 @@ __global__ __launch_bounds__(kHistBlock) void k_digit_hist(const uint32_t* __res
                                                            uint32_t n_units, DigitPlan plan,
                                                            uint32_t* __restrict__ hist,
                                                            uint32_t tile_shift,
-                                                           uint32_t* __restrict__ tile_counts) {
+                                                           uint32_t* __restrict__ tile_counts,
+                                                           const uint32_t* __restrict__ skip) {
+  if (skip && *skip) return;  // the bucket sort ran instead
   // the digit of pass p is taken exactly as the binning pass takes it, so
   // the counts match the ranks whatever the key values
   const uint32_t n_passes = plan.n_passes;
@@ -337,10 +304,24 @@ __device__ __forceinline__ void digit_starts(uint32_t* hist, uint32_t* digit_sta
   if (d < 256) digit_start[(static_cast<size_t>(s) * n_passes + p) * 256 + d] = ex;
 }
 
+// Clears the histogram copies of segment s, pass p (thread d < 256: digit d).
+__device__ __forceinline__ void clear_counts(uint32_t* hist, uint32_t s, uint32_t p) {
+  const uint32_t d = threadIdx.x;
+  if (d < 256) {
+#pragma unroll
+    for (uint32_t k = 0; k < kHistCopies; ++k)
+      hist[(static_cast<size_t>(s) * kHistCopies + k) * (kMaxPasses * 256) + p * 256 + d] = 0;
+  }
+}
+
 // Digit starts: one workgroup per (segment, pass).
 __global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ hist,
                                                     uint32_t* __restrict__ digit_start,
-                                                    uint32_t n_passes) {
+                                                    uint32_t n_passes, const uint32_t* __restrict__ skip) {
+  if (skip && *skip) {  // the bucket sort ran: the sweep's counts are only cleared
+    clear_counts(hist, blockIdx.x, blockIdx.y);
+    return;
+  }
   __shared__ uint32_t s_tmp[4];
   digit_starts(hist, digit_start, n_passes, blockIdx.x, blockIdx.y, s_tmp);
 }
@@ -352,7 +333,11 @@ __global__ __launch_bounds__(1024) void k_start_scans(uint32_t* __restrict__ his
                                                       uint32_t* __restrict__ digit_start,
                                                       uint32_t n_passes,
                                                       uint32_t* __restrict__ tile_counts,
-                                                      uint32_t n_tiles) {
+                                                      uint32_t n_tiles, const uint32_t* __restrict__ skip) {
+  if (skip && *skip) {  // the bucket sort ran: the sweep's counts are only cleared
+    if (blockIdx.x >= 256) clear_counts(hist, 0, blockIdx.x - 256);
+    return;
+  }
   __shared__ uint32_t s_tmp[1024 / kWave];
   if (blockIdx.x < 256) {
     tile_scan_row(tile_counts, n_tiles, blockIdx.x, s_tmp);
@@ -378,7 +363,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, SegArgs seg,
     const uint32_t* __restrict__ digit_start, uint32_t pass,
     uint32_t n_passes, uint32_t shift, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctrl,
-    const uint32_t* __restrict__ tile_excl, uint32_t n_tiles) {
+    const uint32_t* __restrict__ tile_excl, uint32_t n_tiles, const uint32_t* __restrict__ skip) {
+  if (skip && *skip) return;  // before the grab: a skipped pass leaves the counter as it was
   constexpr int WAVES = BLOCK / kWave;
   constexpr uint32_t TILE = BLOCK * KPT;
   static_assert(BLOCK >= 256 && BLOCK % kWave == 0, "one thread per digit needed");
@@ -554,34 +540,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
       for (int k = 0; k < KPT; ++k) {
         const uint32_t d = (keys[k] >> shift) & dmask;
         if (kFirst) continue;
-        // Stable multisplit: lanes holding the same digit find each other
-        // with BITS ballots (a lane differs from this one where some ballot
-        // differs from this lane's own bit); the lowest lane of each group
-        // advances the wave's counter for that digit.
-        uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, 0, 1));
-        uint64_t bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
-        uint32_t xl = own ^ static_cast<uint32_t>(bal);
-        uint32_t xh = own ^ static_cast<uint32_t>(bal >> 32);
-#pragma unroll
-        for (int b = 1; b < BITS; ++b) {
-          own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, b, 1));
-          bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
-          // truth tables indexed by (x << 2 | own << 1 | bal): 0xF6 is
-          // x | (own ^ bal); the last bit takes its complement 0x09, so that
-          // x ends as the peer mask
-          if (b + 1 < BITS) {
-            xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0xF6);
-            xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0xF6);
-          } else {
-            xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0x09);
-            xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0x09);
-          }
-        }
-        if (BITS == 1) {
-          xl = ~xl;
-          xh = ~xh;
-        }
-        const uint64_t m = (static_cast<uint64_t>(xh) << 32) | xl;  // peers, this lane included
+        // Stable multisplit: the lowest lane of each group of lanes holding
+        // the same digit advances the wave's counter for that digit.
+        const uint64_t m = match_peers<BITS>(d);  // peers, this lane included
         const uint32_t below = popc_below(m);
         const uint32_t old = wh[d];
         if (below == 0) wh[d] = old + static_cast<uint32_t>(__popcll(m));
@@ -702,7 +663,7 @@ void launch_table_pass(const SortLaunch& L, uint32_t* dst, hipStream_t stream) {
     constexpr auto kern = k_onesweep<BLOCK, KPT, B, true>;
     hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, L.keys_in,
                        dst, L.seg, L.digit_start, 0u, L.plan.n_passes, 0u, L.lookback, L.ctrl, L.tile_counts,
-                       n_tiles);
+                       n_tiles, L.skip);
   });
 }
 
@@ -724,13 +685,15 @@ void launch_passes(const SortLaunch& L, uint32_t p_begin, hipStream_t stream) {
       with_first_bits(bits, [&](auto B) {
         constexpr auto kern = k_onesweep<BLOCK, KPT, B, false, true>;
         hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, src, dst,
-                           L.seg, L.digit_start, 0u, n_passes, shift, L.lookback, L.ctrl, nullptr, n_tiles);
+                           L.seg, L.digit_start, 0u, n_passes, shift, L.lookback, L.ctrl, nullptr, n_tiles,
+                           L.skip);
       });
     } else {
       with_bits(bits, [&](auto B) {
         constexpr auto kern = k_onesweep<BLOCK, KPT, B>;
         hipLaunchKernelGGL(kern, dim3(resident_grid<kern>(BLOCK, n_tiles)), dim3(BLOCK), 0, stream, src, dst,
-                           L.seg, L.digit_start, p, n_passes, shift, L.lookback, L.ctrl, nullptr, n_tiles);
+                           L.seg, L.digit_start, p, n_passes, shift, L.lookback, L.ctrl, nullptr, n_tiles,
+                           L.skip);
       });
     }
     src = dst;
@@ -746,14 +709,14 @@ void launch_sort(const SortLaunch& L, hipStream_t stream) {
   if (!L.hist_ready)
     hipLaunchKernelGGL(k_digit_hist, dim3(L.n_hist_units), dim3(kHistBlock), 0, stream, L.keys_in,
                        L.seg, L.hist_unit_keys, L.n_hist_units, L.plan, L.hist,
-                       sort_tile_shift(variant), table ? L.tile_counts : nullptr);
+                       sort_tile_shift(variant), table ? L.tile_counts : nullptr, L.skip);
   if (table) {
     const uint32_t n_tiles = (L.n_index + (1u << sort_tile_shift(variant)) - 1) >> sort_tile_shift(variant);
     hipLaunchKernelGGL(k_start_scans, dim3(256 + n_passes), dim3(1024), 0, stream, L.hist, L.digit_start,
-                       n_passes, L.tile_counts, n_tiles);
+                       n_passes, L.tile_counts, n_tiles, L.skip);
   } else {
     hipLaunchKernelGGL(k_digit_scan, dim3(L.n_segs, n_passes), dim3(256), 0, stream, L.hist,
-                       L.digit_start, n_passes);
+                       L.digit_start, n_passes, L.skip);
   }
   with_variant(variant, [&](auto V) {
     constexpr int B = kSortVariants[V].block, K = kSortVariants[V].kpt;

@@ -114,6 +114,103 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+// True when every active lane holds the same x (wave-uniform).
+__device__ __forceinline__ bool wave_same(uint32_t x) {
+  return ballot(x != __builtin_amdgcn_readfirstlane(x)) == 0;
+}
+
+// Adds one to h[d] for every active lane.  When all active lanes hit the
+// same counter (sorted or clustered keys: consecutive keys share their high
+// digits) one lane adds their number instead: 64 atomics on one LDS address
+// would otherwise be serialised.  try_wave (wave-uniform) false skips the
+// check: the caller saw no digit the wave shares (or only a few keys).
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d, bool try_wave = true) {
+  if (try_wave && wave_same(d)) {
+    const uint64_t act = ballot(true);
+    if (lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1))
+      atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
+  } else {
+    atomicAdd(&h[d], 1u);
+  }
+}
+
+// atomicAdd(&h[d], 1) for every active lane, returning each lane's old
+// value (distinct ranks, in no particular order within the wave), with one
+// atomic when all active lanes hold the same digit (checked only when
+// try_wave, as for hist_add).
+__device__ __forceinline__ uint32_t rank_add(uint32_t* h, uint32_t d, bool try_wave) {
+  if (try_wave && wave_same(d)) {
+    const uint64_t act = ballot(true);
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
+    return __shfl(base, leader, kWave) + popc_below(act);
+  }
+  return atomicAdd(&h[d], 1u);
+}
+
+// hist_add and rank_add for the lanes with `valid` set, called by every lane
+// of the wave (the collectives then never run in divergent code); the rank
+// of a lane without `valid` is meaningless.
+__device__ __forceinline__ bool wave_same_valid(uint32_t x, bool valid, uint64_t* act, uint32_t* leader) {
+  *act = ballot(valid);
+  if (*act == 0) return false;
+  *leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(*act)) - 1);
+  const uint32_t xl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), static_cast<int>(*leader)));
+  return ballot(valid && x != xl) == 0;
+}
+__device__ __forceinline__ void hist_add_if(uint32_t* h, uint32_t d, bool valid, bool try_wave) {
+  uint64_t act;
+  uint32_t leader;
+  if (try_wave && wave_same_valid(d, valid, &act, &leader)) {
+    if (lane_id() == leader) atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
+    return;
+  }
+  if (valid) atomicAdd(&h[d], 1u);
+}
+__device__ __forceinline__ uint32_t rank_add_if(uint32_t* h, uint32_t d, bool valid, bool try_wave) {
+  uint64_t act;
+  uint32_t leader;
+  if (try_wave && wave_same_valid(d, valid, &act, &leader)) {
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(&h[d], static_cast<uint32_t>(__popcll(act)));
+    return __shfl(base, static_cast<int>(leader), kWave) + popc_below(act);
+  }
+  return valid ? atomicAdd(&h[d], 1u) : 0u;
+}
+
+// Stable multisplit within a wave: the lanes holding the same BITS-bit digit
+// d as this lane (this lane included), found with BITS ballots (a lane
+// differs from this one where some ballot differs from this lane's own bit).
+template <int BITS>
+__device__ __forceinline__ uint64_t match_peers(uint32_t d) {
+  uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, 0, 1));
+  uint64_t bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
+  uint32_t xl = own ^ static_cast<uint32_t>(bal);
+  uint32_t xh = own ^ static_cast<uint32_t>(bal >> 32);
+#pragma unroll
+  for (int b = 1; b < BITS; ++b) {
+    own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(d, b, 1));
+    bal = __builtin_amdgcn_uicmp(own, 0u, 33 /* ne */);
+    // truth tables indexed by (x << 2 | own << 1 | bal): 0xF6 is
+    // x | (own ^ bal); the last bit takes its complement 0x09, so that
+    // x ends as the peer mask
+    if (b + 1 < BITS) {
+      xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0xF6);
+      xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0xF6);
+    } else {
+      xl = __builtin_amdgcn_bitop3_b32(xl, own, static_cast<uint32_t>(bal), 0x09);
+      xh = __builtin_amdgcn_bitop3_b32(xh, own, static_cast<uint32_t>(bal >> 32), 0x09);
+    }
+  }
+  if (BITS == 1) {
+    xl = ~xl;
+    xh = ~xh;
+  }
+  return (static_cast<uint64_t>(xh) << 32) | xl;
+}
+
+
 // ---------------------------------------------------------------------------
 // Decoupled look-back.  A tile publishes a 64-bit word:
 //   [63:34] epoch  [33:32] status (1 aggregate, 2 inclusive prefix)  [31:0] value

@@ -180,6 +180,10 @@ struct SortLaunch {
   // [256][n_tiles]: pass 0 tile counts, for a single-cloud sort of 9+ bits
   // (nullptr, or any other sort: pass 0 runs the look-back chain)
   uint32_t* tile_counts;
+  // nullptr, or a device word: when it is non-zero (the bucket sort below ran
+  // instead) every kernel of the sort returns at once (the scans after
+  // clearing the histogram copies, which then hold no counts)
+  const uint32_t* skip;
 };
 void launch_sort(const SortLaunch& L, hipStream_t stream);
 // Stage 1 fused with the sort's histogram sweep, over units of unit_keys
@@ -191,6 +195,79 @@ void launch_sort(const SortLaunch& L, hipStream_t stream);
 void launch_morton_hist(const float4* pts, uint32_t* codes, SegArgs seg, uint32_t n_index, uint32_t unit_keys,
                         float min_coord, float range, const DigitPlan& plan, uint32_t* hist, int sort_variant,
                         uint32_t* tile_counts, uint32_t* status, hipStream_t stream);
+
+// Bucket sort of a single cloud's 30-bit keys (Morton codes), two passes over
+// HBM instead of four: the keys are scattered by their top D bits into 2^D
+// buckets (about 8192 keys each; no order needed within a bucket, so ranks
+// come from LDS atomics), then one workgroup per bucket sorts it in LDS by
+// its low R = 30 - D bits: an atomic-ranked pass on the low kBucketLowBits
+// bits and one stable multisplit pass on the rest.  A bucket larger than
+// kBucketCap keys (clustered input) cannot be held in LDS: then the bucket
+// kernels return at once and the digit-pass sort runs instead, gated by the
+// same device flag.  Equal keys are indistinguishable, so the output is the
+// same as the digit-pass sort's.
+constexpr uint32_t kBucketTile = 16384;      // keys per scatter tile (and per S1 count unit)
+constexpr int kBucketScatterBlock = 1024;    // 16 keys per thread
+constexpr uint32_t kBucketMaxBins = 2048;    // D <= 11
+constexpr int kBucketLocalBlock = 512;
+constexpr int kBucketLocalKpt = 18;
+constexpr uint32_t kBucketCap = kBucketLocalBlock * kBucketLocalKpt;  // 9216 keys
+constexpr uint32_t kBucketLowBits = 12;      // the atomic-ranked in-bucket pass
+constexpr uint32_t kBucketMinLog = 21, kBucketMaxLog = 24;  // sizes in (2^20, 2^24]
+// tiles per thread of the start scan (32 runs of them cover every tile)
+constexpr uint32_t kBucketScanRun = ((1u << kBucketMaxLog) / kBucketTile + 31) / 32;
+static_assert(kBucketTile % (kHistBlock * 4) == 0, "stage 1 reads whole rows of four loads per thread");
+static_assert(kBucketMaxBins % 32 == 0, "the start scan takes 32 buckets per workgroup");
+struct BucketPlan {
+  bool ok;            // n in the bucket sort's range
+  uint32_t shift;     // R: the bucket of key k is k >> R
+  uint32_t n_bins;    // 2^D
+  uint32_t n_tiles;   // scatter tiles
+  uint32_t hi_bits;   // R - kBucketLowBits: the stable in-bucket pass
+};
+// D = ceil(log2 n) - 13, so that a bucket averages (4096, 8192] keys of
+// uniform input: kBucketCap is more than ten standard deviations above.
+constexpr BucketPlan bucket_plan(uint32_t n) {
+  BucketPlan p{};
+  uint32_t lg = 0;
+  while (lg < 32 && (uint64_t{1} << lg) < n) ++lg;
+  if (lg < kBucketMinLog || lg > kBucketMaxLog) return p;
+  const uint32_t d = lg - 13;
+  p.ok = true;
+  p.shift = 30 - d;
+  p.n_bins = 1u << d;
+  p.n_tiles = (n + kBucketTile - 1) / kBucketTile;
+  p.hi_bits = p.shift - kBucketLowBits;
+  return p;
+}
+// Words of the per-tile bucket count table for any size up to max_points.
+constexpr size_t bucket_table_words(uint32_t max_points) {
+  size_t w = 0;
+  for (uint32_t lg = kBucketMinLog; lg <= kBucketMaxLog; ++lg) {
+    const uint64_t n = std::min<uint64_t>(max_points, uint64_t{1} << lg);
+    const BucketPlan p = bucket_plan(static_cast<uint32_t>(n));
+    if (p.ok) w = std::max<size_t>(w, size_t{p.n_bins} * p.n_tiles);
+  }
+  return w;
+}
+constexpr uint32_t kBucketCtlWords = 4;  // [0]: the flag
+struct BucketLaunch {
+  const uint32_t* keys_in;  // n keys below 2^30
+  uint32_t* keys_out;       // sorted (untouched when the flag refuses the bucket sort)
+  uint32_t* keys_tmp;       // the scattered buckets
+  uint32_t n;
+  BucketPlan plan;
+  uint32_t* table;   // [n_tiles][n_bins] bucket counts of each tile (S1), scanned in place
+  uint32_t* count;   // [n_bins] bucket totals
+  uint32_t* start;   // [n_bins + 1] bucket starts
+  uint32_t* ctl;     // kBucketCtlWords: [0] the flag (1: the bucket sort runs), written by every launch
+};
+// Stage 1 for the bucket sort: codes, and the bucket counts of each scatter
+// tile into table; status (or nullptr) is cleared by the launch.
+void launch_morton_bucket(const float4* pts, uint32_t* codes, uint32_t n, float min_coord, float range,
+                          const BucketPlan& plan, uint32_t* table, uint32_t* status, hipStream_t stream);
+// The tile scan, the scatter (which sets the flag) and the in-bucket sorts.
+void launch_bucket_sort(const BucketLaunch& B, hipStream_t stream);
 
 // status (per segment, or nullptr) is cleared by the launch: the first kernel
 // of a batched build.

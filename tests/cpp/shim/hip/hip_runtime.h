@@ -5,8 +5,9 @@
 //
 // Execution model:
 //   * a launch runs its workgroups one after another; a workgroup's
-//     work-items are std::threads, __syncthreads is a std::barrier over them,
-//     and a __shared__ variable is a static (one copy: one workgroup at a time);
+//     work-items are std::threads (created once per launch and reused by
+//     every workgroup), __syncthreads is a std::barrier over them, and a
+//     __shared__ variable is a static (one copy: one workgroup at a time);
 //   * wave collectives (ballot, shuffles, readfirstlane) synchronise a
 //     "domain" of lanes: the whole 64-lane wave, or, for kernels whose lane
 //     groups loop independently, the group (emu::set_domain);
@@ -146,30 +147,46 @@ void launch(K kernel, dim3 grid, dim3 block, A... args) {
   const uint32_t n = block.x * block.y * block.z;
   uint32_t dom = kWave;
   if (auto it = domains().find(reinterpret_cast<const void*>(kernel)); it != domains().end()) dom = it->second;
+  // one thread per work-item for the whole launch; the workgroups run one
+  // after another, each between two meetings of the threads with this one
+  Block b;
+  dim3 cur;
+  bool quit = false;
+  std::barrier<> start(n + 1), done(n + 1);
+  std::vector<std::thread> th;
+  th.reserve(n);
+  for (uint32_t t = 0; t < n; ++t)
+    th.emplace_back([&, t] {
+      t_thread = dim3(t, 0, 0);
+      for (;;) {
+        start.arrive_and_wait();
+        if (quit) return;
+        t_block = cur;
+        kernel(args...);
+        leave();
+        done.arrive_and_wait();
+      }
+    });
   for (uint32_t bz = 0; bz < grid.z; ++bz)
     for (uint32_t by = 0; by < grid.y; ++by)
       for (uint32_t bx = 0; bx < grid.x; ++bx) {
-        Block b;
         b.threads = n;
         b.domain = dom;
         b.sync = std::make_unique<std::barrier<>>(n);
+        b.domains.clear();
         for (uint32_t d = 0; d < (n + dom - 1) / dom; ++d)
           b.domains.push_back(Domain{std::make_unique<std::barrier<>>(std::min(dom, n - d * dom))});
         b.slot.assign(n, 0);
         b.present.assign(n, 1);
         g_cur = &b;
-        std::vector<std::thread> th;
-        th.reserve(n);
-        for (uint32_t t = 0; t < n; ++t)
-          th.emplace_back([&, t] {
-            t_thread = dim3(t, 0, 0);
-            t_block = dim3(bx, by, bz);
-            kernel(args...);
-            leave();
-          });
-        for (auto& x : th) x.join();
+        cur = dim3(bx, by, bz);
+        start.arrive_and_wait();
+        done.arrive_and_wait();
         g_cur = nullptr;
       }
+  quit = true;
+  start.arrive_and_wait();
+  for (auto& x : th) x.join();
 }
 
 }  // namespace emu
@@ -223,6 +240,9 @@ inline uint32_t __builtin_amdgcn_mbcnt_hi(uint32_t m, uint32_t acc) {
   return acc + static_cast<uint32_t>(__builtin_popcount(m & below));
 }
 inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t v) { return static_cast<uint32_t>(emu::first_lane(v)); }
+inline int __builtin_amdgcn_readlane(int v, int lane) {
+  return static_cast<int>(emu::read_lane(static_cast<uint32_t>(v), static_cast<uint32_t>(lane)));
+}
 // v_perm_b32: byte i of the result selected from the 8 bytes {a, b} (b low) by sel byte i
 inline uint32_t __builtin_amdgcn_perm(uint32_t a, uint32_t b, uint32_t sel) {
   const uint64_t v = (static_cast<uint64_t>(a) << 32) | b;
@@ -253,6 +273,9 @@ inline uint32_t __builtin_amdgcn_bitop3_b32(uint32_t a, uint32_t b, uint32_t c, 
   return r;
 }
 inline void __builtin_amdgcn_s_waitcnt(int) {}
+// on the GPU the lanes of a wave run in lockstep (this builtin only orders the
+// compiler); here they are threads, so the domain meets at a barrier
+inline void __builtin_amdgcn_wave_barrier() { emu::my_domain().bar->arrive_and_wait(); }
 // a cache hint only: a plain load on the CPU
 template <class T>
 inline T __builtin_nontemporal_load(const T* p) { return *p; }

@@ -1,0 +1,153 @@
+"""The bucket-sort kernels themselves, run on the CPU.
+
+tests/cpp/bin/bucket_emul compiles redwood-comp_amd/csrc/k_sort.hip and
+k_bucket.hip unmodified with g++ under a host emulation of the HIP runtime
+(one thread per work-item, barriers for __syncthreads and the wave
+collectives) and AddressSanitizer, and runs the build's first two stages as
+on its bucket-sort path: stage 1 with the bucket counts, the tile scan, the
+scatter and the in-bucket sorts, then the digit-pass sort (its histogram
+sweep and passes) gated on the bucket sort's flag, with every buffer sized
+exactly as the workspace gives it.  An index past any buffer or LDS array
+aborts the run.  Checked: the codes equal the oracle's Morton codes, the
+bucket starts equal the counts of the codes' top D bits, the output equals
+np.sort of the codes, the flag is 1 exactly when every bucket holds at most
+9216 keys, and then the gated digit passes return with the histogram copies
+clear for the next build (the digit passes themselves, which sort when the
+flag is 0, are covered by the GPU tests: their kernels run wave collectives
+in divergent code, which the emulation cannot model).  D = 8..10
+covers the stable in-bucket pass widths 10, 9 and 8 bits (256 digits or
+fewer: fewer digits than threads).  Cases:
+uniform points, a tile and buckets ending inside a wave, buckets near the
+capacity (several waves, partial rows), a bucket of exactly the capacity and
+one key more, heavy duplication, and points in Morton order (whole waves
+in one bucket)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+POISON = 0x5A5A5A5A
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU = os.path.join(ROOT, "tests", "cpp", "bin", "bucket_emul")
+CAP = 9216
+
+
+@pytest.fixture(scope="module")
+def emul():
+    subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "cpp"), "bin/bucket_emul"], check=True,
+                   capture_output=True)
+    return EMU
+
+
+def compact(v):
+    v = v.astype(np.uint64) & 0x09249249
+    v = (v | (v >> 2)) & 0x030C30C3
+    v = (v | (v >> 4)) & 0x0300F00F
+    v = (v | (v >> 8)) & 0x030000FF
+    v = (v | (v >> 16)) & 0x3FF
+    return v
+
+
+def points_of_codes(codes):
+    """Points at the centres of the cells of the given 30-bit Morton codes."""
+    c = codes.astype(np.uint64)
+    pts = np.zeros((c.size, 4), np.float32)
+    for a in range(3):
+        pts[:, a] = compact(c >> a).astype(np.float32) + 0.5
+    return pts
+
+
+def run(emul, pts, d, tmp_path):
+    pp, op = tmp_path / "pts.bin", tmp_path / "out.bin"
+    np.ascontiguousarray(pts, np.float32).tofile(pp)
+    r = subprocess.run([emul, str(pp), str(d), str(op)], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-4000:]
+    b = np.fromfile(op, np.uint32)
+    n = pts.shape[0]
+    return {"status": int(b[0]), "flag": int(b[1]), "hist_clear": int(b[2]), "codes": b[3:3 + n],
+            "sorted": b[3 + n:3 + 2 * n], "scattered": b[3 + 2 * n:3 + 3 * n], "starts": b[3 + 3 * n:]}
+
+
+def check(out, pts, d, orc, pool, expect_flag=None):
+    codes = orc.morton(pool, pts)
+    assert out["status"] == 0
+    assert np.array_equal(out["codes"], codes)
+    counts = np.bincount(codes >> (30 - d), minlength=1 << d)
+    starts = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    assert np.array_equal(out["starts"], starts)
+    fits = counts.max() <= CAP
+    assert out["flag"] == int(fits)
+    if expect_flag is not None:
+        assert out["flag"] == expect_flag
+    if fits:
+        s = np.sort(codes)
+        assert np.array_equal(out["sorted"], s)
+        # the scatter's buckets: every key in its bucket's range
+        assert np.array_equal(out["scattered"] >> (30 - d), s >> (30 - d))
+        assert np.array_equal(np.sort(out["scattered"]), s)
+        # the gated digit passes returned, the scans clearing the sweep's counts
+        assert out["hist_clear"] == 1
+    else:
+        # the bucket kernels returned untouched; the digit passes (not
+        # emulated: the GPU tests cover them) would sort
+        assert np.all(out["sorted"] == POISON) and np.all(out["scattered"] == POISON)
+
+
+def uniform(n, seed):
+    rng = np.random.default_rng(seed)
+    pts = np.zeros((n, 4), np.float32)
+    pts[:, :3] = rng.random((n, 3), dtype=np.float32) * 1024
+    return pts
+
+
+# (D = 11, the 2^24 build's, is left to the GPU tests: 2048 emulated
+# workgroups take minutes, and its in-bucket pass differs from D = 10's only
+# in the width constant)
+@pytest.mark.parametrize("n,d", [(40000, 8), (20011, 9), (9000, 10)])
+def test_uniform(emul, orc, pool, tmp_path, n, d):
+    pts = uniform(n, 11 + d)
+    check(run(emul, pts, d, tmp_path), pts, d, orc, pool, expect_flag=1)
+
+
+def test_buckets_near_capacity(emul, orc, pool, tmp_path):
+    # points in the cube [0, 256)^3: the codes' top six bits are zero, so
+    # four buckets of 2^8 hold every key (about 8000 each: several waves,
+    # partial rows); the last tile ends inside a wave
+    rng = np.random.default_rng(3)
+    n = 32005
+    pts = np.zeros((n, 4), np.float32)
+    pts[:, :3] = rng.random((n, 3), dtype=np.float32) * 256
+    check(run(emul, pts, 8, tmp_path), pts, 8, orc, pool, expect_flag=1)
+
+
+@pytest.mark.parametrize("extra,flag", [(0, 1), (1, 0)])
+def test_bucket_at_capacity(emul, orc, pool, tmp_path, extra, flag):
+    # bucket 5 of 2^8 holds exactly the capacity (flag 1) or one key more
+    rng = np.random.default_rng(17 + extra)
+    d = 8
+    big = (np.uint64(5) << np.uint64(22)) + rng.integers(0, 1 << 22, CAP + extra, dtype=np.uint64)
+    rest = rng.integers(0, 1 << 30, 12000, dtype=np.uint64)
+    rest = rest[(rest >> np.uint64(22)) != 5]
+    codes = np.concatenate([big, rest])
+    rng.shuffle(codes)
+    pts = points_of_codes(codes)
+    check(run(emul, pts, d, tmp_path), pts, d, orc, pool, expect_flag=flag)
+
+
+def test_heavy_duplication(emul, orc, pool, tmp_path):
+    # 30000 points on 300 distinct cells (runs of one key across waves)
+    rng = np.random.default_rng(23)
+    cells = rng.integers(0, 1 << 30, 300, dtype=np.uint64)
+    codes = cells[rng.integers(0, cells.size, 30000)]
+    pts = points_of_codes(codes)
+    check(run(emul, pts, 8, tmp_path), pts, 8, orc, pool, expect_flag=1)
+
+
+def test_morton_ordered_input(emul, orc, pool, tmp_path):
+    # points already in Morton order: whole waves share a bucket, and the
+    # in-bucket low digits arrive ordered
+    pts = uniform(36000, 29)
+    codes = orc.morton(pool, pts)
+    pts = pts[np.argsort(codes, kind="stable")]
+    check(run(emul, pts, 8, tmp_path), pts, 8, orc, pool, expect_flag=1)

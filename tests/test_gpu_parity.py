@@ -89,10 +89,77 @@ def test_full_size_matches_oracle(gpu, orc, kind, n):
     (nu, nb, no, rc), o, _ = orc.build(big, pts)
     assert rc == 0
     ws = rw.Workspace(n)
-    for v, p in sorted({(0, 0), (BENCH_SORT_VARIANT, 0), rw.best_sort_config(n)}):
-        print(f"case full kind={kind} n={n} sort_variant={v} plan={p}", flush=True)
-        g_counts, g = rw.build_octree(pts, ws=ws, sort_variant=v, sort_plan=p)
-        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}p{p}")
+    # the digit passes with each shape, then the bucket sort (the benchmark's
+    # path), which the clustered points hand back to the digit passes
+    runs = [(v, p, False) for v, p in sorted({(0, 0), (BENCH_SORT_VARIANT, 0), rw.best_sort_config(n)})]
+    runs.append(rw.best_sort_config(n) + (True,))
+    for v, p, bucket in runs:
+        print(f"case full kind={kind} n={n} sort_variant={v} plan={p} bucket={bucket}", flush=True)
+        g_counts, g = rw.build_octree(pts, ws=ws, sort_variant=v, sort_plan=p, bucket_sort=bucket)
+        assert ws.sort_used_buckets() == (bucket and kind == "uniform")
+        compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}p{p}b{bucket}")
+
+
+@pytest.mark.parametrize("n", [(1 << 20) + 1, 1 << 21, (1 << 21) + 12345, 3 * (1 << 21) + 7, 1 << 23,
+                               (1 << 23) + 1])
+def test_bucket_sort_sizes(gpu, orc, n):
+    """Whole builds through the bucket sort at the edges of its size range
+    (2^20 < n <= 2^24: 256 to 2048 buckets, stable in-bucket passes of 10 down
+    to 8 bits, last scatter tiles ending inside a wave) against the oracle."""
+    rw = gpu
+    pts = rw.uniform_points(n, seed=n % 1000 + 3)
+    (nu, nb, no, rc), o, _ = orc.build(orc.Pool(16), pts)
+    assert rc == 0
+    ws = rw.Workspace(n)
+    print(f"case bucket n={n}", flush=True)
+    g_counts, g = rw.build_octree(pts, ws=ws)
+    assert ws.sort_used_buckets()
+    compare(rw, n, g_counts, g, (nu, nb, no), o, f"bucket/{n}")
+
+
+def test_bucket_sort_out_of_range_runs_digit_passes(gpu):
+    rw = gpu
+    for n in (1 << 20, 1000):
+        ws = rw.Workspace(n)
+        rw.build_octree(rw.uniform_points(n, 5), ws=ws)
+        assert not ws.sort_used_buckets()
+
+
+def _cells_points(codes):
+    """Points at the centres of the cells of 30-bit Morton codes."""
+    c = codes.astype(np.uint64)
+    pts = np.zeros((c.size, 4), np.float32)
+    for a in range(3):
+        v = (c >> np.uint64(a)) & np.uint64(0x09249249)
+        v = (v | (v >> np.uint64(2))) & np.uint64(0x030C30C3)
+        v = (v | (v >> np.uint64(4))) & np.uint64(0x0300F00F)
+        v = (v | (v >> np.uint64(8))) & np.uint64(0x030000FF)
+        v = (v | (v >> np.uint64(16))) & np.uint64(0x3FF)
+        pts[:, a] = v.astype(np.float32) + np.float32(0.5)
+    return pts
+
+
+@pytest.mark.parametrize("extra", [0, 1])
+def test_bucket_sort_capacity_edge(gpu, orc, pool, extra):
+    """2^21 points in 256 buckets, bucket 5 holding exactly the 9216 keys an
+    in-bucket sort holds (the bucket sort runs) or one more (the digit passes
+    run): both builds equal the oracle's."""
+    rw = gpu
+    n = 1 << 21
+    rng = np.random.default_rng(41 + extra)
+    big = (np.uint64(5) << np.uint64(22)) + rng.integers(0, 1 << 22, 9216 + extra, dtype=np.uint64)
+    rest = rng.integers(0, 1 << 30, 2 * n, dtype=np.uint64)
+    rest = rest[(rest >> np.uint64(22)) != 5][:n - big.size]
+    codes = np.concatenate([big, rest])
+    rng.shuffle(codes)
+    pts = _cells_points(codes)
+    assert np.array_equal(orc.morton(pool, pts), codes.astype(np.uint32))
+    (nu, nb, no, rc), o, _ = orc.build(orc.Pool(16), pts)
+    ws = rw.Workspace(n)
+    print(f"case bucket capacity extra={extra}", flush=True)
+    g_counts, g = rw.build_octree(pts, ws=ws, strict=False)
+    assert ws.sort_used_buckets() == (extra == 0)
+    compare(rw, n, g_counts, g, (nu, nb, no), o, f"bucket cap+{extra}")
 
 
 @pytest.mark.parametrize("n", [1, 3, 4, 1000, 4097, 123457, 1 << 20])
