@@ -6,9 +6,12 @@ inputs resident in HBM.  A step is one whole build (rwh_build_octree); K steps
 are timed between stream synchronisations.  The last build is then checked
 against the CPU oracle, array by array, outside the timed region (exit status
 1 on a mismatch).  Per-stage device times come from HIP events recorded on the
-build's stream around each stage entry point; algorithmic bytes per stage give
-achieved GB/s against the 8 TB/s HBM peak and against the measured
-device-to-device copy rate.  Beside the main line: the clustered single
+build's stream around each stage entry point called alone (`stages`) and at
+the stage boundaries of the whole build itself (`stages_in_build`,
+rwh_build_octree_marked: its fused kernels and the sort it picks, which is
+what `sort_roofline` reports); algorithmic bytes per stage give achieved GB/s
+against the 8 TB/s HBM peak and against the measured device-to-device copy
+rate.  Beside the main line: the clustered single
 builds (2^24 and 2^22 points), the 512-cloud batched build against one plain
 build per cloud, the continuous-batching server under Poisson load, and the
 CPU pthread oracle on this host.
@@ -209,6 +212,55 @@ def timed_builds(rw, ws, tb, n, stream, steps, warmup):
         build()
     stream.sync()
     return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def bucket_table_bytes(n):
+    """Bytes of the bucket sort's per-tile count table (16384-key tiles,
+    2^(ceil(log2 n) - 13) buckets) for a whole build in its size range."""
+    lg = max(int(n - 1).bit_length(), 0)
+    return 4 * ((n + 16383) // 16384) * (1 << (lg - 13))
+
+
+def build_stage_bytes(n, nu, nb, no, bucket):
+    """Algorithmic bytes of each stage of the whole build as it runs: S1 is
+    fused with the sort's counts (the bucket counts, or the digit-pass
+    histograms), the S4 kernels write the edge counts (S5 takes no pass of
+    its own), S6 scans them."""
+    table = bucket_table_bytes(n) if bucket else 0
+    return {
+        "S1_morton": 20 * n + table,               # points in, codes and the count table out
+        # bucket sort: the table scanned (read + write) and read by the
+        # scatter, the keys moved by the scatter and by the in-bucket sorts;
+        # digit passes: 4 passes, each reading and writing every key
+        "S2_sort": (16 * n + 3 * table) if bucket else 32 * n,
+        "S3_unique": 4 * n + 4 * nu,
+        "S4_radix_tree": 4 * nu + 15 * nb,          # + the edge counts
+        "S5_edge_count": 0,
+        "S6_prefix_sum": 8 * nb,
+        "S7_octree": 17 * nb + 32 * no,
+    }
+
+
+def build_stage_section(rw, ws, tb, n, counts, stream, copy_gbs, reps=10):
+    """Per-stage device times of the whole build as it runs
+    (rwh_build_octree_marked: events at its stage boundaries), averaged."""
+    marks = [rw.Event() for _ in range(rw.BUILD_MARKS)]
+    acc = dict.fromkeys(rw.BUILD_STAGES, 0.0)
+    for _ in range(reps):
+        rw.check(rw.build_marked(ws, tb, n, marks, stream.ptr), "marked build")
+        for k, v in rw.stage_ms(marks).items():
+            acc[k] += v / reps
+    bucket = ws.sort_used_buckets()
+    out = {"sort": "bucket sort (MSD scatter + in-LDS bucket sorts)" if bucket else "digit passes (onesweep)"}
+    for (name, b), ms in zip(build_stage_bytes(n, *counts, bucket).items(), acc.values()):
+        e = {"ms": round(ms, 4), "bytes": int(b)}
+        if b and ms > 0:
+            gbs = b / (ms * 1e-3) / 1e9
+            e.update({"GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3),
+                      "frac_of_copy": round(gbs / copy_gbs, 3)})
+        out[name] = e
+    out["sum_ms"] = round(sum(acc.values()), 4)
+    return out
 
 
 def read_counts(rw, ws, cap, stream):
@@ -492,6 +544,8 @@ def main():
         acc += [evs[i].elapsed_ms(evs[i + 1]) for i in range(7)]
     stage_ms = acc / args.stage_reps
     copy_gbs = copy_rate(rw, stream)
+    progress("per-stage times of the whole build")
+    in_build = build_stage_section(rw, ws, tb, n, (nu, nb, no), stream, copy_gbs, args.stage_reps)
     progress("result export (packed result, one transfer)")
     export = export_section(rw, ws, tb, n, stream)
     export["pack_frac_of_copy"] = round(export["pack_GBs"] / copy_gbs, 3)
@@ -511,7 +565,8 @@ def main():
     # together by the stage's events
     s7 = stages["S7_octree"]
     out_bytes_true = 17 * nb + 60 * no  # every slot's row, corner, cell and both masks
-    sort = stages["S2_sort"]
+    sort = in_build["S2_sort"]
+    sort_alone = stages["S2_sort"]
     out = {
         "metric": METRIC,
         "value": round(n / (ms_per_step * 1e-3) / 1e6, 2),
@@ -535,11 +590,18 @@ def main():
                      "algorithmic_bytes": s7["bytes"], "ms": s7["ms"],
                      "bytes_with_all_outputs": out_bytes_true,
                      "frac_with_all_outputs": round(out_bytes_true / (s7["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)},
-        "sort_roofline": {"kernels": "S2 onesweep radix sort (histogram + 4 digit passes)",
+        # S2 as the timed build runs it (its own stage-boundary events)
+        "sort_roofline": {"kernels": f"S2 in the whole build: {in_build['sort']}",
                           "achieved": sort["GB/s"], "peak": HBM_PEAK_GBS, "frac": sort["frac"],
                           "frac_of_copy": sort["frac_of_copy"], "ms": sort["ms"],
-                          "traffic": profile_figure(args.log2n, "sort_hbm_bytes_per_call")},
+                          "algorithmic_bytes": sort["bytes"],
+                          "gkeys_per_s": round(n / (sort["ms"] * 1e-3) / 1e9, 2),
+                          "traffic": profile_figure(args.log2n, "sort_hbm_bytes_per_call"),
+                          "digit_pass_sort_alone": {"ms": sort_alone["ms"], "bytes": sort_alone["bytes"],
+                                                    "frac": sort_alone["frac"],
+                                                    "gkeys_per_s": round(n / (sort_alone["ms"] * 1e-3) / 1e9, 2)}},
         "copy_GBs": round(copy_gbs, 1),
+        "stages_in_build": in_build,
         "stages": stages,
         "result_export": export,
         "sort_config": config_name(rw, variant, plan),

@@ -222,6 +222,17 @@ int rwh_link_leaf(const rwh_tree_buffers* bufs, const uint32_t* d_n_unique,
  * rwh_workspace_counts(ws). */
 int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
                      float range, rwh_stream_t stream);
+/* The same build with the caller's events recorded on the stream at its stage
+ * boundaries, for per-stage timing of the build as it runs (fused kernels and
+ * the sort it picks included); the reference times its stages with one
+ * timer per stage call (apps/tree/pipe_main_baseline.cpp:23-31).
+ * marks[0] before S1; marks[k] after stage k (k = 1..7).  The bucket-sort
+ * path counts its buckets inside S1 (the codes' pass); S5's edge counts are
+ * written by the S4 kernels, so marks[4] and marks[5] are recorded together
+ * (S5 then takes no time of its own). */
+#define RWH_BUILD_MARKS 8
+int rwh_build_octree_marked(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
+                            float range, const rwh_event_t* marks, rwh_stream_t stream);
 /* ---- the same build without allocating (no counterpart in the reference)
  * For callers that cannot allocate device memory: code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc.  The build of

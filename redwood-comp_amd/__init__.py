@@ -90,6 +90,7 @@ ABI = {
     "rwh_octree": (_I, [_P, _P, _P, _P, _U32, _F, _F, _P, _P]),
     "rwh_link_leaf": (_I, [_P, _P, _P, _U32, _P, _P]),
     "rwh_build_octree": (_I, [_P, _P, _U32, _F, _F, _P]),
+    "rwh_build_octree_marked": (_I, [_P, _P, _U32, _F, _F, _P, _P]),
     "rwh_build_octree_noalloc": (_I, [_P, _U32, _F, _F, _P, _P]),
     "rwh_read_counts": (_I, [_P, _U32, C.POINTER(_U32), _P]),
     "rwh_batch_layout": (_I, [_P, _U32, _P, _P, _P, C.POINTER(_U32), C.POINTER(_U32)]),
@@ -475,15 +476,37 @@ class PinnedArray:
             pass
 
 
+BUILD_MARKS = 8  # RWH_BUILD_MARKS
+BUILD_STAGES = ("S1_morton", "S2_sort", "S3_unique", "S4_radix_tree", "S5_edge_count", "S6_prefix_sum",
+                "S7_octree")
+
+
+def build_marked(ws, tb, n: int, marks: list, stream=None, min_coord: float = MIN_COORD,
+                 range_: float = RANGE) -> int:
+    """rwh_build_octree_marked on resident buffers; returns its status."""
+    if len(marks) != BUILD_MARKS:
+        raise ValueError(f"marks must hold {BUILD_MARKS} events")
+    arr = (_P * BUILD_MARKS)(*[e.ptr for e in marks])
+    return lib().rwh_build_octree_marked(ws.ptr, tb.ptr(), n, min_coord, range_, arr, stream)
+
+
+def stage_ms(marks: list) -> dict:
+    """Stage times (ms) between the marks of a completed marked build."""
+    marks[-1].sync()
+    return {name: marks[i].elapsed_ms(marks[i + 1]) for i, name in enumerate(BUILD_STAGES)}
+
+
 def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
                  min_coord: float = MIN_COORD, range_: float = RANGE, link_leaf: bool = False,
                  strict: bool = True, sort_variant: int | None = None, sort_plan: int | None = None,
-                 bucket_sort: bool | None = None):
+                 bucket_sort: bool | None = None, marks: list | None = None):
     """Runs S1..S7 on the GPU for host points (n, 4); returns (counts, host outputs).
     link_leaf also runs the opt-in leaf linking after S7.  strict=False returns the
     (capacity-clamped) outputs when n_octree_nodes exceeds the 0.6 n capacity
     instead of raising.  sort_variant picks the sort's tile shape (a new
-    workspace's default otherwise); bucket_sort False forces the digit passes."""
+    workspace's default otherwise); bucket_sort False forces the digit passes.
+    marks: RWH_BUILD_MARKS Events recorded at the build's stage boundaries
+    (rwh_build_octree_marked); stage_ms(marks) turns them into stage times."""
     n = points.shape[0]
     cap = octree_capacity(n)
     ws = ws or Workspace(max(n, 1))
@@ -494,7 +517,10 @@ def build_octree(points: np.ndarray, ws: Workspace | None = None, stream=None,
     if bucket_sort is not None:
         ws.set_bucket_sort(bucket_sort)
     tb = TreeBuild(n, cap, DeviceArray.from_numpy(points.astype(np.float32).reshape(-1)))
-    check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
+    if marks is None:
+        check(lib().rwh_build_octree(ws.ptr, tb.ptr(), n, min_coord, range_, stream), "rwh_build_octree")
+    else:
+        check(build_marked(ws, tb, n, marks, stream, min_coord, range_), "rwh_build_octree_marked")
     if link_leaf:
         check(lib().rwh_link_leaf(tb.ptr(), ws.n_unique_ptr(), ws.n_oct_ptr(), n, ws.status_ptr(), stream),
               "rwh_link_leaf")

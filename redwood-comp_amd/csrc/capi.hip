@@ -325,9 +325,11 @@ int check_bufs(const rwh_tree_buffers* b) {
 
 // The whole build of one cloud on a workspace (or a view of the scratch
 // array); n_unique, n_octree_nodes and the status word go to d_nu, d_no and
-// d_status (device).
+// d_status (device).  marks (or nullptr): RWH_BUILD_MARKS events recorded
+// at the stage boundaries.
 int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
-                 uint32_t* d_nu, uint32_t* d_no, uint32_t* d_status, hipStream_t st);
+                 uint32_t* d_nu, uint32_t* d_no, uint32_t* d_status, hipStream_t st,
+                 const rwh_event_t* marks = nullptr);
 
 // Per-device state of the allocation-free build.
 constexpr int kNoallocDevices = 64;
@@ -713,6 +715,15 @@ int rwh_build_octree(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n
                       static_cast<hipStream_t>(stream));
 }
 
+int rwh_build_octree_marked(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord,
+                            float range, const rwh_event_t* marks, rwh_stream_t stream) {
+  RWH_TRY(check_ws(ws, n));
+  RWH_CHECK(marks, "null marks");
+  for (int k = 0; k < RWH_BUILD_MARKS; ++k) RWH_CHECK(marks[k], "null event in marks");
+  return build_single(ws, bufs, n, min_coord, range, ws->n_unique(), ws->n_oct(), ws->status(),
+                      static_cast<hipStream_t>(stream), marks);
+}
+
 int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
                              uint32_t* d_counts, rwh_stream_t stream) {
   RWH_CHECK(d_counts, "null counts pointer");
@@ -753,9 +764,16 @@ int rwh_build_octree_noalloc(const rwh_tree_buffers* bufs, uint32_t n, float min
 namespace {
 
 int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, float min_coord, float range,
-                 uint32_t* d_nu, uint32_t* d_no, uint32_t* d_status, hipStream_t st) {
+                 uint32_t* d_nu, uint32_t* d_no, uint32_t* d_status, hipStream_t st,
+                 const rwh_event_t* marks) {
   RWH_TRY(check_ws(ws, n));
   RWH_TRY(check_bufs(bufs));
+  // the caller's stage-boundary events (measurement only)
+  const auto mark = [&](int k) -> int {
+    if (marks) RWH_HIP(hipEventRecord(static_cast<hipEvent_t>(marks[k]), st));
+    return RWH_OK;
+  };
+  RWH_TRY(mark(0));
   // S1 fused with the sort's histogram sweep, then the digit passes
   if (injected(1) || injected(2)) return RWH_ERR_INJECTED;
   RWH_CHECK(bufs->morton_keys != bufs->morton_keys_alt, "the two key buffers must differ");
@@ -768,6 +786,7 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
     launch_morton_bucket(reinterpret_cast<const float4*>(bufs->points), bufs->morton_keys, n, min_coord, range,
                          bp, ws->bucket_table, d_status, st);
     RWH_TRY(launched());
+    RWH_TRY(mark(1));
     const BucketLaunch B{bufs->morton_keys, bufs->morton_keys_alt, ws->keys_tmp, n, bp, ws->bucket_table,
                          ws->bucket_start + kBucketMaxBins + 1, ws->bucket_start, ws->bucket_ctl};
     launch_bucket_sort(B, st);
@@ -779,10 +798,15 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
                        kPlainHistUnit, min_coord, range, make_plan(30, ws->sort_plan), ws->hist,
                        ws->sort_variant, ws->tile_counts, d_status, st);
     RWH_TRY(launched());
+    RWH_TRY(mark(1));
     RWH_TRY(enqueue_sort(ws, bufs->morton_keys, bufs->morton_keys_alt, single_seg(n), 1, n, 30, true,
                          st, true));
+  } else {
+    RWH_TRY(mark(1));
   }
+  RWH_TRY(mark(2));
   RWH_TRY(rwh_unique_u32(ws, bufs->morton_keys_alt, n, bufs->morton_keys, d_nu, st));
+  RWH_TRY(mark(3));
   if (injected(4) || injected(5)) return RWH_ERR_INJECTED;
   if (n == 0) {
     launch_fill_u32(d_no, 0, 1, st);
@@ -795,9 +819,17 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
     TreeArgs ta{bufs->morton_keys, bufs->prefix_n, bufs->has_leaf_left, bufs->has_leaf_right,
                 bufs->left_child, bufs->parents, bufs->edge_count, ws->tree_sums, d_status};
     launch_radix_tree(ta, single_seg_dev(d_nu), n, ws->tree_list, st);
+    RWH_TRY(launched());
+    RWH_TRY(mark(4));
+    RWH_TRY(mark(5));
     launch_prefix_sum_tiles(bufs->edge_count, bufs->edge_offset, single_seg_dev(d_nu), n, d_no, ws->tree_sums, st);
     RWH_TRY(launched());
   }
+  if (n == 0) {
+    RWH_TRY(mark(4));
+    RWH_TRY(mark(5));
+  }
+  RWH_TRY(mark(6));
   // S7 on the tree just built (its kernels checked their own bounds)
   OctArgs oa = oct_args(bufs, min_coord, range);
   oa.n_oct = d_no;
@@ -805,7 +837,8 @@ int build_single(rwh_workspace* ws, const rwh_tree_buffers* bufs, uint32_t n, fl
   oa.far_item = ws->far_item;
   if (injected(7)) return RWH_ERR_INJECTED;
   if (n > 0) launch_octree(oa, single_seg_dev(d_nu), n, false, st);
-  return launched();
+  RWH_TRY(launched());
+  return mark(7);
 }
 
 }  // namespace

@@ -54,6 +54,7 @@ def test_argument_errors_need_no_gpu(rw):
     assert L.rwh_radix_tree(None, None, None, (1 << 30) + 1, None, None, None, None, None, None) == \
         rw.RWH_ERR_INVALID_ARG
     assert L.rwh_octree(None, None, None, None, 10, 0.0, 1024.0, None, None) == rw.RWH_ERR_INVALID_ARG
+    assert L.rwh_build_octree_marked(None, None, 10, 0.0, 1024.0, None, None) == rw.RWH_ERR_INVALID_ARG
     assert L.rwh_error_string(rw.RWH_ERR_CAPACITY) == b"capacity exceeded"
     counts = np.zeros(4, np.uint32)
     n = np.array([1, 2], np.uint32)

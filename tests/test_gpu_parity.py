@@ -100,6 +100,26 @@ def test_full_size_matches_oracle(gpu, orc, kind, n):
         compare(rw, n, g_counts, g, (nu, nb, no), o, f"{kind}/{n}/v{v}p{p}b{bucket}")
 
 
+@pytest.mark.parametrize("kind,n", [("uniform", 100003), ("uniform", (1 << 21) + 12345), ("gmm", 1 << 21),
+                                    ("uniform", 1)])
+def test_marked_build_matches_oracle(gpu, orc, pool, kind, n):
+    """rwh_build_octree_marked (the build with stage-boundary events, which the
+    benchmark times stage by stage) gives the plain build's output, on the
+    digit-pass path, the bucket-sort path and its fallback; its stage times
+    are non-negative and S5's is empty (written by the S4 kernels)."""
+    rw = gpu
+    pts = inputs(rw, kind, n, seed=n % 977 + 5)
+    (nu, nb, no, rc), o, _ = orc.build(pool, pts)
+    marks = [rw.Event() for _ in range(rw.BUILD_MARKS)]
+    print(f"case marked kind={kind} n={n}", flush=True)
+    g_counts, g = rw.build_octree(pts, strict=False, marks=marks)
+    compare(rw, n, g_counts, g, (nu, nb, no), o, f"marked {kind}/{n}")
+    ms = rw.stage_ms(marks)
+    assert list(ms) == list(rw.BUILD_STAGES)
+    assert all(v >= 0.0 for v in ms.values()), ms
+    assert ms["S5_edge_count"] < 0.01, ms
+
+
 @pytest.mark.parametrize("n", [(1 << 20) + 1, 1 << 21, (1 << 21) + 12345, 3 * (1 << 21) + 7, 1 << 23,
                                (1 << 23) + 1])
 def test_bucket_sort_sizes(gpu, orc, n):
