@@ -113,3 +113,12 @@ def test_sizes_generator(rw):
     s = rw.log_uniform_sizes(7, 512, 10, 20)
     assert s.min() >= 1024 and s.max() <= 1 << 20
     assert 60_000 < s.mean() < 250_000
+
+
+def test_marked_build_arguments(rw):
+    # the stage-marked build takes exactly RWH_BUILD_MARKS events, all set
+    with pytest.raises(ValueError):
+        rw.build_marked(None, None, 10, [None] * 3)
+    assert rw.BUILD_MARKS == 8 and len(rw.BUILD_STAGES) == rw.BUILD_MARKS - 1
+    h = open(os.path.join(ROOT, "include", "rwhip.h")).read()
+    assert "#define RWH_BUILD_MARKS 8" in h
