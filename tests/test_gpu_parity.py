@@ -447,6 +447,57 @@ def test_batch_equals_oracle_per_cloud(gpu, orc, pool, shape):
                 (nu, nb, no), o, f"cloud{s}")
 
 
+def test_batch_graph_replays_match_eager_and_oracle(gpu, orc, pool):
+    """The batched build captured as a HIP graph (rwh_batch_graph_create) and
+    replayed three times, with the program's own eager work between the
+    replays (every output array and the per-cloud counts and status words
+    poisoned by eager memsets, then a plain build of another cloud on the same
+    stream): every replay's arrays and counts equal the eager batched build's
+    (rwh_batch_prepare + rwh_build_octree_batch_enqueue), which equals the
+    oracle cloud by cloud."""
+    import ctypes as C
+    rw = gpu
+    L = rw.lib()
+    sizes = [1000, 8193, 30000, 5, 70000, 0, 150000]
+    clouds = [rw.uniform_points(n, 300 + s) if s % 2 == 0 else rw.gmm_points(n, 300 + s, 16, 10.24)
+              for s, n in enumerate(sizes)]
+    k = len(clouds)
+    b = rw.Batch(clouds)
+    st = rw.Stream()
+    print("case batch graph eager", flush=True)
+    rw.check(L.rwh_batch_prepare(b.ws.ptr, b.sizes.ctypes.data, b.pb.ctypes.data, b.ob.ctypes.data,
+                                 b.oc.ctypes.data, k, st.ptr), "rwh_batch_prepare")
+    rw.check(L.rwh_build_octree_batch_enqueue(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, st.ptr), "enqueue")
+    st.sync()
+    eager = b.read()
+    for s, c in enumerate(clouds):
+        (nu, nb, no, rc), o, _ = orc.build(pool, c)
+        r = eager[s]
+        compare(rw, c.shape[0], (r["n_unique"], max(r["n_unique"] - 1, 0), r["n_oct"]),
+                dict(r, morton_keys=r["unique"]), (nu, nb, no), o, f"eager cloud{s}")
+    g = C.c_void_p()
+    rw.check(L.rwh_batch_graph_create(b.ws.ptr, b.tb.ptr(), 0.0, 1024.0, st.ptr, C.byref(g)), "graph")
+    other = rw.uniform_points(20000, 9)
+    try:
+        for rep in range(3):
+            print(f"case batch graph replay {rep}", flush=True)
+            for name in rw.TREE_FIELDS:
+                a = b.tb[name]
+                rw.check(L.rwh_memset(a.ptr, 0x5A, a.nbytes, st.ptr), "poison")
+            rw.check(L.rwh_memset(b.ws.counts_ptr, 0x5A, 4 * 3 * k, st.ptr), "poison counts")
+            rw.build_octree(other, stream=st.ptr)
+            rw.check(L.rwh_graph_launch(g, st.ptr), "rwh_graph_launch")
+            st.sync()
+            res = b.read()
+            for s in range(k):
+                for key, v in eager[s].items():
+                    got = res[s][key]
+                    same = got == v if np.isscalar(v) else np.array_equal(bits(got), bits(v))
+                    assert same, f"replay {rep} cloud {s} {key}"
+    finally:
+        rw.check(L.rwh_graph_destroy(g), "rwh_graph_destroy")
+
+
 def distinct_cells(n, seed):
     """n points in distinct cells of the 1024^3 lattice (cell centres): every
     Morton code differs, so n_unique = n and the radix tree has n - 1 nodes."""
