@@ -263,6 +263,21 @@ def build_stage_section(rw, ws, tb, n, counts, stream, copy_gbs, reps=10):
     return out
 
 
+def median_build_ms(rw, ws, tb, n, stream, runs=10, warmup=2):
+    """Median of `runs` builds, each between its own pair of HIP events on
+    the build stream, after `warmup` untimed builds."""
+    L = rw.lib()
+    ev = [(rw.Event(), rw.Event()) for _ in range(runs)]
+    for _ in range(warmup):
+        rw.check(L.rwh_build_octree(ws.ptr, tb.ptr(), n, 0.0, 1024.0, stream.ptr), "build")
+    for e0, e1 in ev:
+        e0.record(stream.ptr)
+        rw.check(L.rwh_build_octree(ws.ptr, tb.ptr(), n, 0.0, 1024.0, stream.ptr), "build")
+        e1.record(stream.ptr)
+    ev[-1][1].sync()
+    return float(np.median([e0.elapsed_ms(e1) for e0, e1 in ev]))
+
+
 def read_counts(rw, ws, cap, stream):
     import ctypes as C
     counts = (C.c_uint32 * 3)()
@@ -525,6 +540,7 @@ def main():
     progress(f"timing {args.steps} builds of 2^{args.log2n} points")
     ms_per_step = timed_builds(rw, ws, tb, n, stream, args.steps, args.warmup)
     nu, nb, no = read_counts(rw, ws, cap, stream)
+    median_ms = median_build_ms(rw, ws, tb, n, stream)
 
     orc = None
     threads, _ = oracle_threads()
@@ -575,6 +591,9 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
+        # beside the K-step mean: the median of 10 builds timed one by one
+        # (HIP events around each, after 2 warm-up builds)
+        "ms_per_build_median_of_10": round(median_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
